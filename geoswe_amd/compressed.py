@@ -1,0 +1,224 @@
+"""CompressedSolver: the compressed active-cell API of GeoSWE on the MI355X fused step.
+
+Drop-in for ``geoswe.CompressedSolver`` (src/geoswe/compressed_solver.py:2882-3450) on
+the production scheme: ``from_dense`` packs a dense solver's active set (its inside
+mask, or every cell) into the flat layout of ``flatmesh``; ``run``/``step`` call
+``gs_flat_step``, whose arithmetic is the dense kernel's, so the residual is bitwise the
+dense one at a fixed state.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .config import Config, check_numeric_inputs
+from .flatmesh import build_flat_mesh
+from .forcing import RainfallForcing
+from .solver import Solver2D, _Forcing, host_bc_fill
+
+
+class CompressedSolver:
+    """Flat-storage solver over the active cells plus a 2-cell ring (compressed_solver.py:2882)."""
+
+    def __init__(self, mesh, cfg: Config, fm, q_pad: np.ndarray, b_pad: np.ndarray, flags_pad: np.ndarray,
+                 ntab: np.ndarray, n_field_pad: Optional[np.ndarray], cfl_linf: bool, track_max_depth: bool = True):
+        cfg.require_production_scheme("CompressedSolver")
+        self.mesh, self.cfg, self.fm = mesh, cfg, fm
+        self.dtype = np.dtype(cfg.dtype)
+        self.cfl_linf = bool(cfl_linf)
+        N = fm.n_stored
+        qf = fm.pack(q_pad).astype(self.dtype)
+        self._bufs = [[L.DeviceArray.from_host(np.ascontiguousarray(qf[k])) for k in range(3)] for _ in range(2)]
+        self._bed = L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(b_pad).astype(self.dtype)))
+        self._flags = L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(flags_pad).astype(np.uint8)))
+        t = np.zeros(64, self.dtype)
+        t[:len(ntab)] = ntab
+        self._ntab = L.DeviceArray.from_host(t)
+        self._n_classes = max(1, len(ntab))
+        self._n_field = (L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(n_field_pad).astype(self.dtype)))
+                         if n_field_pad is not None else None)
+        self._max_h = L.DeviceArray((N,), self.dtype, zero=True) if track_max_depth else None
+        self._runs = [L.DeviceArray.from_host(a) for a in (fm.run_k0, fm.run_len, fm.run_i, fm.run_j0,
+                                                            fm.run_d.reshape(-1), fm.chunk_run)]
+        self._fo = _Forcing(cfg, min(mesh.dx, mesh.dy), self.dtype, (mesh.nx, mesh.ny))
+        self.stream = None
+        self._clock = L.new_clock(0.0)
+        self._lam_valid = False
+        self._target = None
+        self._any_step = False
+        self.diagnostics = {"wallclock": 0.0, "steps": 0}
+
+    # ------------------------------------------------------------------ construction
+    @classmethod
+    def from_dense(cls, s: Solver2D, cfl_linf: bool = False, track_max_depth: bool = True, **_ignored):
+        """Pack a dense solver (its current state, bed, Manning, mask, forcings)."""
+        mesh, cfg = s.mesh, s.cfg
+        g, nx, ny = mesh.ngh, mesh.nx, mesh.ny
+        nxp, nyp = nx + 2 * g, ny + 2 * g
+        inside = np.zeros((nxp, nyp), bool)
+        inside[g:g + nx, g:g + ny] = s.inside_mask if s.inside_mask is not None else True
+        fm = build_flat_mesh(inside, g, ring=2)
+        q_pad = s.q                                  # ghosts filled from the boundary kinds
+        b_pad = s.b
+        flags = np.zeros((nxp, nyp), np.uint8)
+        if s._flags_host is not None:
+            flags[g:g + nx, g:g + ny] = s._flags_host[:, :ny] & np.uint8(~L.FLAG_INACTIVE & 0xFF)
+        ntab = s._ntab.download()[:s._n_classes]
+        nf = None
+        if s._n_field is not None:
+            nf = np.zeros((nxp, nyp), s.dtype)
+            nf[g:g + nx, g:g + ny] = s._n_field.download()[:, :ny]
+        out = cls(mesh, cfg, fm, q_pad, b_pad, flags, ntab, nf, cfl_linf, track_max_depth)
+        c = s._clockv()
+        if c.t != 0.0:
+            L.call("gs_clock_reset", C.c_void_p(out._clock.ptr), C.c_double(float(c.t)), None)
+        return out
+
+    @property
+    def n_active(self) -> int:
+        return self.fm.n_active
+
+    @property
+    def n_stored(self) -> int:
+        return self.fm.n_stored
+
+    def set_rain(self, forcing: RainfallForcing):
+        """Replace the rain forcing (compressed_solver.py:3157-3183)."""
+        self.cfg.rainfall_forcing = forcing
+        self._fo = _Forcing(self.cfg, min(self.mesh.dx, self.mesh.dy), self.dtype, (self.mesh.nx, self.mesh.ny))
+        self._lam_valid = False
+        self._target = None
+
+    # ------------------------------------------------------------------ structs
+    def _phys(self) -> L.GsPhys:
+        cfg = self.cfg
+        return L.phys_struct(cfg.g, cfg.h_min, cfg.h_cfl(), cfg.cfl, min(self.mesh.dx, self.mesh.dy),
+                             cfg.friction_velocity_cap_ms, cfg.friction == "manning_implicit",
+                             cfg.friction_quadratic_alpha, True, not self.cfl_linf, self._n_classes)
+
+    def _desc(self) -> L.GsFlat:
+        f = L.GsFlat()
+        fm = self.fm
+        f.n_stored, f.n_runs = fm.n_stored, len(fm.run_k0)
+        f.dtype = L.GS_F32 if self.dtype == np.float32 else L.GS_F64
+        f.nyp, f.ngh = fm.nyp, fm.ngh
+        f.dx, f.dy = float(self.mesh.dx), float(self.mesh.dy)
+        for s in range(2):
+            for k in range(3):
+                f.q[s][k] = self._bufs[s][k].ptr
+        f.bed, f.flags, f.ntab = self._bed.ptr, self._flags.ptr, self._ntab.ptr
+        f.n_field = self._n_field.ptr if self._n_field is not None else None
+        f.max_h = self._max_h.ptr if self._max_h is not None else None
+        f.run_k0, f.run_len, f.run_i, f.run_j0, f.run_d, f.chunk_run = (r.ptr for r in self._runs)
+        return f
+
+    # ------------------------------------------------------------------ state
+    def _clockv(self):
+        return L.read_clock(self._clock)
+
+    @property
+    def t(self) -> float:
+        return float(self._clockv().t)
+
+    @property
+    def nsteps(self) -> int:
+        return int(self._clockv().nsteps)
+
+    def q_flat(self) -> np.ndarray:
+        cur = int(self._clockv().nsteps) & 1
+        return np.stack([self._bufs[cur][k].download() for k in range(3)])
+
+    def depth(self) -> np.ndarray:
+        """Depth on the unpadded (nx, ny) grid; cells outside the stored set are 0."""
+        g, nx, ny = self.mesh.ngh, self.mesh.nx, self.mesh.ny
+        return self.fm.unpack(self.q_flat()[0])[g:g + nx, g:g + ny]
+
+    def max_depth(self) -> np.ndarray:
+        g, nx, ny = self.mesh.ngh, self.mesh.nx, self.mesh.ny
+        if self._max_h is None or not self._any_step:
+            return self.depth()
+        return self.fm.unpack(self._max_h.download())[g:g + nx, g:g + ny]
+
+    # ------------------------------------------------------------------ stepping
+    def _ensure_lam(self):
+        if not self._lam_valid:
+            L.call("gs_flat_cfl", C.byref(self._desc()), C.byref(self._phys()), self._fo.s,
+                   C.c_void_p(self._clock.ptr), self.stream)
+            self._lam_valid = True
+
+    def _raise_if_bad(self, c):
+        if c.status & L.ST_NONFINITE:
+            raise FloatingPointError(f"CompressedSolver: non-finite wave speed at t={c.t:.3f}s "
+                                     f"(NaN/Inf in an active cell)")
+
+    def _open_mode(self):
+        if self._target != "open":
+            L.call("gs_clock_set_target", C.c_void_p(self._clock.ptr), C.c_double(math.inf), C.c_double(0.0),
+                   C.c_int64(-1), C.byref(self._phys()), self._fo.s, self.stream)
+            self._target = "open"
+
+    def step(self, dt: Optional[float] = None) -> float:
+        self._ensure_lam()
+        self._open_mode()
+        if dt is not None:
+            dt = float(dt)
+            L.call("gs_clock_set_dt", C.c_void_p(self._clock.ptr), C.c_double(dt), C.byref(self._phys()),
+                   self._fo.s, self.stream)
+        else:
+            c = self._clockv()
+            self._raise_if_bad(c)
+            dt = float(c.dt)
+        L.call("gs_flat_step", C.byref(self._desc()), C.byref(self._phys()), self._fo.s,
+               C.c_void_p(self._clock.ptr), self.stream)
+        self._any_step = True
+        return dt
+
+    def advance(self, nsteps: int, sync: bool = True) -> None:
+        self._ensure_lam()
+        self._open_mode()
+        d, p, clk = C.byref(self._desc()), C.byref(self._phys()), C.c_void_p(self._clock.ptr)
+        for _ in range(int(nsteps)):
+            L.call("gs_flat_step", d, p, self._fo.s, clk, self.stream)
+        self._any_step = True
+        if sync:
+            L.sync(self.stream)
+            self._raise_if_bad(self._clockv())
+
+    def run(self, t_end: float, dt_max: Optional[float] = None, max_steps: int = 10 ** 9, batch: int = 64) -> int:
+        """Step to t_end (compressed_solver.py:3359-3450); dt planned on the device."""
+        if dt_max is not None:
+            check_numeric_inputs(where="CompressedSolver.run", cfl=None)
+        t0 = time.perf_counter()
+        self._ensure_lam()
+        n0 = self.nsteps
+        L.call("gs_clock_set_target", C.c_void_p(self._clock.ptr), C.c_double(float(t_end)),
+               C.c_double(float(dt_max) if dt_max else 0.0), C.c_int64(int(max_steps)), C.byref(self._phys()),
+               self._fo.s, self.stream)
+        d, p, clk = C.byref(self._desc()), C.byref(self._phys()), C.c_void_p(self._clock.ptr)
+        while True:
+            c = self._clockv()
+            self._raise_if_bad(c)
+            if c.status & L.ST_DONE:
+                break
+            for _ in range(batch):
+                L.call("gs_flat_step", d, p, self._fo.s, clk, self.stream)
+            self._any_step = True
+        self._target = None
+        steps = self.nsteps - n0
+        self.diagnostics["wallclock"] = time.perf_counter() - t0
+        self.diagnostics["steps"] = steps
+        return steps
+
+    def residual(self) -> np.ndarray:
+        """Residual (3, N) of the current flat state (ring cells 0)."""
+        N = self.fm.n_stored
+        r = [L.DeviceArray((N,), self.dtype, zero=True) for _ in range(3)]
+        cur = int(self._clockv().nsteps) & 1
+        L.call("gs_flat_residual", C.byref(self._desc()), C.byref(self._phys()), cur, C.c_void_p(r[0].ptr),
+               C.c_void_p(r[1].ptr), C.c_void_p(r[2].ptr), self.stream)
+        return np.stack([x.download() for x in r])

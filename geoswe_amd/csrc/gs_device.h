@@ -1,0 +1,202 @@
+// gs_device.h -- device arithmetic of the SRM-HLLC finite-volume step, shared by every
+// layout (dense sweep, flat runs, ensemble) so that all of them produce the same bits.
+//
+// Evaluation order follows the reference kernel text src/geoswe/rhs_cuda.py:900-1208 and
+// the update tail rhs_cuda.py:1757-1796 term by term. The library is compiled with
+// -ffp-contract=off and correctly rounded fp32 division/sqrt, so every operation below is
+// one IEEE-754 rounding and the NumPy oracle reproduces it exactly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/geoswe_abi.h"
+
+namespace gs {
+
+template <typename T> __device__ __forceinline__ T vmax(T a, T b) { return fmax(a, b); }
+template <typename T> __device__ __forceinline__ T vmin(T a, T b) { return fmin(a, b); }
+template <> __device__ __forceinline__ float vmax<float>(float a, float b) { return fmaxf(a, b); }
+template <> __device__ __forceinline__ float vmin<float>(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float vsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double vsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float vabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double vabs(double x) { return fabs(x); }
+
+// Per-face SRM reconstruction + HLLC flux + the two one-sided bed sources.
+// L is the lower-index cell, R the upper one, along the face normal.
+template <typename T>
+struct Face {
+  T Fh, FN, FT;   // HLLC mass, normal-momentum and transverse-momentum fluxes
+  T srcL, srcR;   // bed source of L (subtracted) and of R (added), rhs_cuda.py:1060-1115
+  bool gate;      // false when both cells are dry: the face contributes nothing
+};
+
+// HLLC in the face frame, rhs_cuda.py:986-1036. Only the branch the cascade selects is
+// evaluated for the star state; the selected value is computed with the reference's
+// expressions, so the result is the reference's bit pattern.
+template <typename T>
+__device__ __forceinline__ void hllc(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min,
+                                     T& Fh, T& FN, T& FT) {
+  const T z = T(0);
+  const T aL = vsqrt(g * (hL > z ? hL : z));
+  const T aR = vsqrt(g * (hR > z ? hR : z));
+  const T u_st = (T(0.5) * (uNL + uNR) + aL) - aR;
+  const T h_rt = T(0.5) * (aL + aR) + T(0.25) * (uNL - uNR);
+  const T h_st = (h_rt > z) ? (h_rt * h_rt) / g : z;
+  const T a_st = vsqrt(g * (h_st > z ? h_st : z));
+  const T SL_w = vmin(uNL - aL, u_st - a_st);
+  const T SR_w = vmax(uNR + aR, u_st + a_st);
+  const bool dryL = (hL <= h_min);
+  const bool dryR = (hR <= h_min);
+  T SL = dryL ? (uNR - T(2) * aR) : SL_w;
+  T SR = dryR ? (uNL + T(2) * aL) : SR_w;
+  if (dryL && dryR) { SL = z; SR = z; }
+  const T denom = hR * (uNR - SR) - hL * (uNL - SL);
+  T SM;
+  if (vabs(denom) > T(1.0e-14)) {
+    SM = ((SL * hR) * (uNR - SR) - (SR * hL) * (uNL - SL)) / denom;
+  } else {
+    SM = T(0.5) * (uNL + uNR);
+  }
+  const T huNL = hL * uNL, huNR = hR * uNR;
+  const T pL = (T(0.5) * g * hL) * hL;
+  const T pR = (T(0.5) * g * hR) * hR;
+  const T FL_N = huNL * uNL + pL;
+  const T FR_N = huNR * uNR + pR;
+  if (SL >= z) {
+    Fh = huNL; FN = FL_N; FT = huNL * uTL;
+  } else if (!(SM >= z) && !(SR >= z)) {
+    Fh = huNR; FN = FR_N; FT = huNR * uTR;
+  } else {
+    // star state of the side the cascade picks: SM >= 0 -> left, else right
+    const bool lft = (SM >= z);
+    const T S = lft ? SL : SR;
+    const T uN = lft ? uNL : uNR;
+    const T uT = lft ? uTL : uTR;
+    const T hh = lft ? hL : hR;
+    const T huN = lft ? huNL : huNR;
+    const T F_N = lft ? FL_N : FR_N;
+    const T fac = (S - uN) / ((S - SM) + T(1.0e-30));
+    const T hs = hh * fac;
+    Fh = huN + S * (hs - hh);
+    FN = F_N + S * (hs * SM - huN);
+    FT = huN * uT + S * (hs * uT - hh * uT);
+  }
+}
+
+// One face: SRM face depths (rhs_cuda.py:1042-1056), flux, and both cells' sources with
+// the gate on the other cell's depth (rhs_cuda.py:1060-1073, 1101-1115). The reference
+// evaluates each face twice, once in each cell's frame; the two evaluations are equal
+// bit for bit (IEEE negation is exact), so one evaluation serves both cells.
+template <typename T>
+__device__ __forceinline__ void face(T bL, T bR, T gzL, T gzR, T etaL, T etaR, T hL, T hR,
+                                     T uNL, T uTL, T uNR, T uTR, T half_d, T g, T h_min,
+                                     Face<T>& f) {
+  const T z = T(0);
+  f.gate = !(hL < h_min && hR < h_min);
+  const T zL = bL + half_d * gzL;
+  const T zR = bR - half_d * gzR;
+  const T zf0 = (bL > bR) ? bL : bR;
+  const T dzc = zR - zL;
+  const T dz = (bR - bL) - dzc;
+  const T detaL = vmax(z, vmin(dz, etaR - etaL));
+  const T detaR = vmax(z, vmin(-dz, etaL - etaR));
+  const T hLf = vmax(z, (etaL + detaL) - zf0);
+  const T hRf = vmax(z, (etaR + detaR) - zf0);
+  hllc<T>(hLf, uNL, uTL, hRf, uNR, uTR, g, h_min, f.Fh, f.FN, f.FT);
+  const T headL = zf0 - etaL;
+  const T dzL = (hR < h_min) ? vmax(z, headL) : vmax(z, vmin(dzc, headL));
+  f.srcL = ((T(0.5) * g) * (hLf + hL)) * ((zf0 - dzL) - bL);
+  const T headR = zf0 - etaR;
+  const T dzR = (hL < h_min) ? vmax(z, headR) : vmax(z, vmin(zL - zR, headR));
+  f.srcR = ((T(0.5) * g) * (hRf + hR)) * ((zf0 - dzR) - bR);
+}
+
+// Primitive velocities with the dry gate (rhs_cuda.py:931-938).
+template <typename T>
+__device__ __forceinline__ void prims(T h, T hu, T hv, T h_min, T& u, T& v) {
+  const bool wet = h > h_min;
+  const T hs = wet ? h : h_min;
+  u = wet ? hu / hs : T(0);
+  v = wet ? hv / hs : T(0);
+}
+
+// h^(-4/3) from IEEE +,-,* only: Newton on y^-3 = h for y = h^(-1/3) from the
+// magic-constant seed (Moroz et al. 2021), three iterations, then (y*y)*(y*y).
+__device__ __forceinline__ float pow_m43(float h) {
+  uint32_t bits = __float_as_uint(h);
+  float y = __uint_as_float(0x54A2FA8Cu - bits / 3u);
+  const float xt = h * (float)(1.0 / 3.0);
+  const float c43 = (float)(4.0 / 3.0);
+  y = y * (c43 - xt * ((y * y) * y));
+  y = y * (c43 - xt * ((y * y) * y));
+  y = y * (c43 - xt * ((y * y) * y));
+  const float y2 = y * y;
+  return y2 * y2;
+}
+__device__ __forceinline__ double pow_m43(double h) { return pow(h, -4.0 / 3.0); }
+
+// Scalars of the cell update, converted once per launch.
+template <typename T>
+struct Phys {
+  T g, h_min, h_cfl, vcap, inv_dx, inv_dy, half_dx, half_dy;
+  int friction, quadratic, keep_h, cfl_l2, has_vcap;
+};
+
+// Fused update tail, rhs_cuda.py:1757-1796 (flat form compressed_rhs.py:716-748):
+// Euler update, wet/dry rule, point-implicit Manning friction with the velocity cap.
+template <typename T>
+__device__ __forceinline__ void update(T q0, T q1, T q2, T rr0, T r1, T r2, T dt, T n,
+                                       const Phys<T>& P, T& h, T& hu, T& hv) {
+  h = q0 + dt * rr0;
+  hu = q1 + dt * r1;
+  hv = q2 + dt * r2;
+  if (h < P.h_min) {
+    h = P.keep_h ? (h > T(0) ? h : T(0)) : T(0);
+    hu = T(0);
+    hv = T(0);
+  } else if (P.friction) {
+    const T hs = (h > P.h_min) ? h : P.h_min;
+    const T u = hu / hs;
+    const T v = hv / hs;
+    const T modU = vsqrt(u * u + v * v);
+    const T h43 = pow_m43(hs);
+    if (P.has_vcap && modU > P.vcap) {
+      const T n_cri = vsqrt(T(1) / ((((T(1.0e-10) + dt) * P.g) * h43) * (modU + T(1.0e-30))));
+      if (n_cri > n) n = n_cri;
+    }
+    const T Cf = ((P.g * n) * n) * h43;
+    T alpha;
+    if (P.quadratic) {
+      const T two = ((T(2) * dt) * Cf) * modU;
+      alpha = T(2) / (vsqrt(T(1) + T(2) * two) + T(1));
+    } else {
+      alpha = T(1) / (T(1) + ((dt * Cf) * modU));
+    }
+    hu = hu * alpha;
+    hv = hv * alpha;
+  }
+}
+
+// CFL wave speed of one cell as float64 bits; 0 for a skipped cell, canonical NaN bits
+// for a NaN speed (solver.py:152-205 expressions; compressed_solver.py:200-240 NaN rule).
+template <typename T>
+__device__ __forceinline__ uint64_t lam_bits(T h, T hu, T hv, const Phys<T>& P) {
+  if (h < P.h_cfl) return 0ull;
+  const T u = hu / h;
+  const T v = hv / h;
+  T vel;
+  if (P.cfl_l2) {
+    vel = vsqrt(u * u + v * v);
+  } else {
+    const T au = vabs(u), av = vabs(v);
+    vel = (au > av) ? au : av;
+  }
+  const T lam = vel + vsqrt(P.g * h);
+  const double l = (double)lam;
+  if (l != l) return 0x7FF8000000000000ull;
+  return (uint64_t)__double_as_longlong(l);
+}
+
+}  // namespace gs

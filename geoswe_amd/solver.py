@@ -1,0 +1,504 @@
+"""Solver2D: the dense-grid solver API of GeoSWE on the MI355X fused step.
+
+Drop-in for ``geoswe.Solver2D`` (src/geoswe/solver.py:1273-3032) on the production
+scheme. State, bed, Manning classes and forcings live in device memory allocated through
+libgeoswe_hip; every step is one ``gs_dense_step`` launch (residual + rain + update +
+friction + wet/dry + stage + next-step CFL), and ``run`` keeps dt on the device so it
+synchronises once per batch of steps instead of once per step.
+
+Device layout: each field is an interior-only plane of ``nx`` rows with a row pitch
+rounded up to 64 elements (256 B in float32); ghost cells are synthesised inside the
+kernel from the boundary kinds, so ``q`` (padded, ghosts filled) is assembled on request.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+import warnings
+from typing import Callable, Optional
+
+import numpy as np
+
+from . import _lib as L
+from .config import Config, Mesh2D
+from .forcing import rain_tables
+
+_PITCH_ALIGN = 64
+
+
+def _pitch(ny: int) -> int:
+    return ((ny + _PITCH_ALIGN - 1) // _PITCH_ALIGN) * _PITCH_ALIGN
+
+
+def host_bc_fill(q: np.ndarray, ngh: int, bc_x: str, bc_y: str) -> None:
+    """Ghost fill of a padded (3, nxp, nyp) host state, x then y (bc.py:50-98)."""
+    for axis, kind in ((1, bc_x), (2, bc_y)):
+        n = q.shape[axis]
+        sl = lambda a, b: (slice(None),) * axis + (slice(a, b),)
+        if kind == "periodic":
+            q[sl(0, ngh)] = q[sl(n - 2 * ngh, n - ngh)]
+            q[sl(n - ngh, n)] = q[sl(ngh, 2 * ngh)]
+        elif kind == "extrapolate":
+            q[sl(0, ngh)] = q[sl(ngh, ngh + 1)]
+            q[sl(n - ngh, n)] = q[sl(n - ngh - 1, n - ngh)]
+        elif kind == "wall":
+            lo = np.flip(q[sl(ngh, 2 * ngh)], axis=axis).copy()
+            hi = np.flip(q[sl(n - 2 * ngh, n - ngh)], axis=axis).copy()
+            lo[axis] = -lo[axis]
+            hi[axis] = -hi[axis]
+            q[sl(0, ngh)] = lo
+            q[sl(n - ngh, n)] = hi
+        elif kind == "fall":
+            q[sl(0, ngh)] = 0
+            q[sl(n - ngh, n)] = 0
+
+
+def host_pad_bed(b_int: np.ndarray, ngh: int, bc_x: str, bc_y: str) -> np.ndarray:
+    """Padded bed, ghosts edge-replicated or wrapped on periodic axes (solver.py:1466-1484)."""
+    nx, ny = b_int.shape
+    b = np.zeros((nx + 2 * ngh, ny + 2 * ngh), dtype=b_int.dtype)
+    b[ngh:ngh + nx, ngh:ngh + ny] = b_int
+    if bc_x == "periodic":
+        b[:ngh, :] = b[-2 * ngh:-ngh, :]
+        b[-ngh:, :] = b[ngh:2 * ngh, :]
+    else:
+        b[:ngh, :] = b[ngh:ngh + 1, :]
+        b[-ngh:, :] = b[-ngh - 1:-ngh, :]
+    if bc_y == "periodic":
+        b[:, :ngh] = b[:, -2 * ngh:-ngh]
+        b[:, -ngh:] = b[:, ngh:2 * ngh]
+    else:
+        b[:, :ngh] = b[:, ngh:ngh + 1]
+        b[:, -ngh:] = b[:, -ngh - 1:-ngh]
+    return b
+
+
+class _Forcing:
+    """Device copies of the rain and stage tables of a Config, plus the ABI struct."""
+
+    def __init__(self, cfg: Config, dmin: float, dtype, frame_shape, frame_pitch=None):
+        self.keep = []
+        self.s = L.GsForcing()
+        self.rain_host = rain_tables(cfg.rainfall_forcing, cfg.rainfall, cfg.cfl, cfg.g, dmin, dtype)
+        if self.rain_host is not None:
+            t, rate, cap, frames = self.rain_host
+            dt_ = L.DeviceArray.from_host(np.ascontiguousarray(t, np.float64))
+            dr = L.DeviceArray.from_host(np.ascontiguousarray(rate, np.float64))
+            dc = L.DeviceArray.from_host(np.ascontiguousarray(cap, np.float64))
+            self.keep += [dt_, dr, dc]
+            self.s.n_rain = len(t)
+            self.s.rain_t, self.s.rain_rate, self.s.rain_cap = dt_.ptr, dr.ptr, dc.ptr
+            if frames is not None:
+                if tuple(frames.shape[1:]) != tuple(frame_shape):
+                    raise ValueError(f"gridded rain must be (nt, nx, ny) = (nt, {frame_shape[0]}, "
+                                     f"{frame_shape[1]}); got {tuple(frames.shape)}")
+                nt = frames.shape[0]
+                pitch = frame_pitch or frames.shape[2]
+                buf = np.zeros((nt, frames.shape[1], pitch), dtype=frames.dtype)
+                buf[:, :, :frames.shape[2]] = frames
+                df = L.DeviceArray.from_host(buf)
+                self.keep.append(df)
+                self.s.rain_gridded = 1
+                self.s.rain_frames = df.ptr
+                self.s.rain_frame_stride = frames.shape[1] * pitch
+                self.s.rain_row_pitch = pitch
+        sb = cfg.stage_boundary
+        if isinstance(sb, (list, tuple)):
+            if len(sb) != 1:
+                raise NotImplementedError("one StageBoundary series per solver is supported")
+            sb = sb[0]
+        self.stage = sb
+        if sb is not None:
+            st = L.DeviceArray.from_host(np.ascontiguousarray(sb.time_s, np.float64))
+            sv = L.DeviceArray.from_host(np.ascontiguousarray(sb.stage_m, np.float64))
+            self.keep += [st, sv]
+            self.s.n_stage = len(sb.time_s)
+            self.s.stage_t, self.s.stage_v = st.ptr, sv.ptr
+
+    def rain_rate_at(self, t: float):
+        """(rate, rmax) at time t on the host, for the host-side conveniences."""
+        if self.rain_host is None:
+            return None, 0.0
+        import bisect
+        tl = list(self.rain_host[0].tolist())
+        idx = min(max(0, bisect.bisect_right(tl, float(t)) - 1), len(tl) - 1)
+        if self.rain_host[3] is None:
+            r = float(self.rain_host[1][idx])
+            return r, r
+        fr = self.rain_host[3][idx]
+        return fr, float(fr.max())
+
+
+class Solver2D:
+    """Two-dimensional shallow-water solver on the dense padded grid (solver.py:1273)."""
+
+    def __init__(self, mesh: Mesh2D, cfg: Config, q0: np.ndarray, b: np.ndarray, comm=None, dims=None,
+                 track_max_depth: bool = True, tile_rows: int = 0):
+        if comm is not None and getattr(comm, "size", 1) > 1:
+            raise NotImplementedError("multi-GPU domain decomposition is not part of this build; one GPU "
+                                      "runs the whole domain")
+        cfg.require_production_scheme("Solver2D")
+        self.mesh, self.cfg = mesh, cfg
+        q0 = np.asarray(q0)
+        b = np.asarray(b)
+        if tuple(q0.shape) != (3, mesh.nx, mesh.ny):
+            raise ValueError(f"q0 must have shape (3, nx, ny) = (3, {mesh.nx}, {mesh.ny}); got {tuple(q0.shape)}")
+        if tuple(b.shape) != (mesh.nx, mesh.ny):
+            raise ValueError(f"the bed must have shape (nx, ny) = ({mesh.nx}, {mesh.ny}); got {tuple(b.shape)}")
+        for what, arr in (("the bed", b), ("the initial state q0", q0)):
+            if not bool(np.isfinite(arr).all()):
+                raise ValueError(f"Solver2D: {what} holds non-finite values (NaN or inf)")
+        if mesh.ngh < 2:
+            raise ValueError(f"ngh={mesh.ngh} is too small: the SRM-HLLC stencil reads two ghost layers")
+        if mesh.nx < 4 or mesh.ny < 4:
+            raise ValueError("the grid needs at least 4 x 4 interior cells")
+        rf = cfg.rainfall_forcing
+        if rf is not None and np.asarray(rf.rate_mm_h).ndim == 3 and tuple(rf.rate_mm_h.shape[1:]) != (mesh.nx, mesh.ny):
+            raise ValueError(f"gridded rain must be (nt, nx, ny) = (nt, {mesh.nx}, {mesh.ny}); got "
+                             f"{tuple(rf.rate_mm_h.shape)}")
+        self.dtype = np.dtype(cfg.dtype)
+        nx, ny = mesh.nx, mesh.ny
+        self.pitch = _pitch(ny)
+        self._plane = (nx, self.pitch)
+        self.stream = None
+        self._bufs = [[L.DeviceArray(self._plane, self.dtype, zero=True) for _ in range(3)] for _ in range(2)]
+        host = np.zeros((3, nx, self.pitch), self.dtype)
+        host[:, :, :ny] = q0.astype(self.dtype, copy=False)
+        for k in range(3):
+            self._bufs[0][k].upload(host[k])
+            self._bufs[1][k].upload(host[k])
+        bh = np.zeros(self._plane, self.dtype)
+        bh[:, :ny] = b.astype(self.dtype, copy=False)
+        self._bed = L.DeviceArray.from_host(bh)
+        self._b_host = b.astype(self.dtype, copy=True)
+        self._flags_host = None          # (nx, pitch) uint8
+        self._flags = None
+        self._n_field = None
+        self._ntab = L.DeviceArray((64,), self.dtype, zero=True)
+        self._n_classes = 1
+        self._manning_uniform = float(cfg.manning_n)
+        self.inside_mask = None
+        self._max_h = L.DeviceArray(self._plane, self.dtype, zero=True) if track_max_depth else None
+        self._any_step = False
+        self._tile_rows = int(tile_rows)
+        self._fo = _Forcing(cfg, min(mesh.dx, mesh.dy), self.dtype, (nx, ny), self.pitch)
+        if self._fo.stage is not None:
+            self._mark_stage(self._fo.stage)
+        if cfg.manning_field is not None:
+            mf = np.asarray(cfg.manning_field)
+            exp = (nx + 2 * mesh.ngh, ny + 2 * mesh.ngh)
+            if tuple(mf.shape) != exp:
+                raise ValueError(f"Config.manning_field must be the padded {exp} array, got {tuple(mf.shape)}")
+            g = mesh.ngh
+            self._set_n_plane(mf[g:g + nx, g:g + ny])
+        self._upload_ntab(np.array([self._manning_uniform]))
+        self._clock = L.new_clock(0.0)
+        self._lam_valid = False
+        self._target = None
+        self.diagnostics = {"wallclock": 0.0, "steps": 0}
+
+    # ------------------------------------------------------------------ structs
+    def _phys(self) -> L.GsPhys:
+        cfg = self.cfg
+        fr = cfg.friction == "manning_implicit"
+        return L.phys_struct(cfg.g, cfg.h_min, cfg.h_cfl(), cfg.cfl, min(self.mesh.dx, self.mesh.dy),
+                             cfg.friction_velocity_cap_ms, fr, cfg.friction_quadratic_alpha, True, False,
+                             self._n_classes)
+
+    def _desc(self) -> L.GsDense:
+        d = L.GsDense()
+        d.nx, d.ny, d.pitch = self.mesh.nx, self.mesh.ny, self.pitch
+        d.bc_x, d.bc_y = L.GS_BC[self.cfg.bc_x], L.GS_BC[self.cfg.bc_y]
+        d.dtype = L.GS_F32 if self.dtype == np.float32 else L.GS_F64
+        d.tile_rows = self._tile_rows
+        d.dx, d.dy = float(self.mesh.dx), float(self.mesh.dy)
+        for s in range(2):
+            for k in range(3):
+                d.q[s][k] = self._bufs[s][k].ptr
+        d.bed = self._bed.ptr
+        d.flags = self._flags.ptr if self._flags is not None else None
+        d.ntab = self._ntab.ptr
+        d.n_field = self._n_field.ptr if self._n_field is not None else None
+        d.max_h = self._max_h.ptr if self._max_h is not None else None
+        return d
+
+    # ------------------------------------------------------------------ set-up
+    def _ensure_flags(self):
+        if self._flags_host is None:
+            self._flags_host = np.zeros(self._plane, np.uint8)
+
+    def _push_flags(self):
+        self._flags = L.DeviceArray.from_host(self._flags_host)
+
+    def _upload_ntab(self, table):
+        t = np.zeros(64, self.dtype)
+        t[:len(table)] = np.asarray(table, dtype=np.float64).astype(self.dtype)
+        self._ntab.upload(t)
+        self._n_classes = max(1, len(table))
+
+    def _set_n_plane(self, n_int):
+        h = np.zeros(self._plane, self.dtype)
+        h[:, :self.mesh.ny] = np.asarray(n_int, dtype=np.float64).astype(self.dtype)
+        self._n_field = L.DeviceArray.from_host(h)
+
+    def _mark_stage(self, sb):
+        g = self.mesh.ngh
+        c = np.asarray(sb.cells, dtype=np.int64).reshape(-1, 2)
+        if c.size and (c[:, 0].min() < g or c[:, 0].max() >= g + self.mesh.nx or c[:, 1].min() < g
+                       or c[:, 1].max() >= g + self.mesh.ny):
+            raise ValueError("StageBoundary.cells must index the padded grid (unpadded index + ngh) and lie "
+                             "in the interior")
+        self._ensure_flags()
+        self._flags_host[c[:, 0] - g, c[:, 1] - g] |= L.FLAG_STAGE
+        self._push_flags()
+
+    def set_inside_mask(self, mask, cfl_robust_pct=None):
+        """Active set (solver.py:1486-1522): cells outside get a zero residual and no rain."""
+        if cfl_robust_pct is not None:
+            raise NotImplementedError("cfl_robust_pct (percentile CFL) is not supported by the fused step")
+        self._ensure_flags()
+        if mask is None:
+            self.inside_mask = None
+            self._flags_host &= np.uint8(~L.FLAG_INACTIVE & 0xFF)
+            self._push_flags()
+            return
+        nx, ny = self.mesh.nx, self.mesh.ny
+        mask = np.asarray(mask)
+        if mask.shape != (nx, ny):
+            raise ValueError(f"inside_mask must have shape ({nx}, {ny}); got {mask.shape}")
+        self.inside_mask = mask.astype(bool)
+        fl = self._flags_host[:, :ny]
+        fl &= np.uint8(~L.FLAG_INACTIVE & 0xFF)
+        fl |= np.where(self.inside_mask, 0, L.FLAG_INACTIVE).astype(np.uint8)
+        self._push_flags()
+
+    def set_manning_table(self, cls_padded, table):
+        """Manning n as a uint8 class per padded cell plus a table (solver.py:1524-1556)."""
+        g, nx, ny = self.mesh.ngh, self.mesh.nx, self.mesh.ny
+        cls_padded = np.asarray(cls_padded)
+        table = np.asarray(table, dtype=np.float32)
+        exp = (nx + 2 * g, ny + 2 * g)
+        if tuple(cls_padded.shape) != exp:
+            raise ValueError(f"set_manning_table expects a PADDED (nxp, nyp)={exp} class array; got "
+                             f"{tuple(cls_padded.shape)}")
+        if cls_padded.dtype != np.uint8:
+            raise ValueError(f"set_manning_table: cls_padded must be uint8; got {cls_padded.dtype}")
+        if int(cls_padded.max()) >= int(table.size):
+            raise ValueError(f"set_manning_table: class id {int(cls_padded.max())} out of range for a "
+                             f"{int(table.size)}-entry table (OOB device read)")
+        if table.size > 64:
+            raise ValueError("set_manning_table: at most 64 classes fit the fused step's class byte")
+        if self.cfg.friction is None:
+            warnings.warn("set_manning_table: friction is off (Config.friction is None), so this roughness "
+                          "table will not be applied", stacklevel=2)
+        self._ensure_flags()
+        fl = self._flags_host[:, :ny]
+        fl &= np.uint8(~L.FLAG_CLASS_MASK & 0xFF)
+        fl |= cls_padded[g:g + nx, g:g + ny].astype(np.uint8)
+        self._push_flags()
+        self._n_field = None
+        self._upload_ntab(table.astype(np.float64))
+        self._manning_table = table
+
+    def set_manning(self, n):
+        """Manning n from a scalar or an unpadded (nx, ny) array; switches friction on (solver.py:1558-1591)."""
+        nx, ny = self.mesh.nx, self.mesh.ny
+        nh = np.asarray(n, dtype=np.float64)
+        if nh.ndim == 0:
+            nh = np.full((nx, ny), float(nh))
+        if nh.shape != (nx, ny):
+            raise ValueError(f"set_manning expects a scalar or an unpadded ({nx}, {ny}) array; got {nh.shape}")
+        if not np.all(np.isfinite(nh)) or float(nh.min()) < 0.0:
+            raise ValueError("set_manning: Manning's n must be finite and non-negative")
+        if self.cfg.friction is None:
+            self.cfg.friction = "manning_implicit"
+        vals, inv = np.unique(nh, return_inverse=True)
+        if vals.size <= 64 and self.dtype == np.float32:
+            self._ensure_flags()
+            fl = self._flags_host[:, :ny]
+            fl &= np.uint8(~L.FLAG_CLASS_MASK & 0xFF)
+            fl |= inv.reshape(nx, ny).astype(np.uint8)
+            self._push_flags()
+            self._n_field = None
+            self._upload_ntab(vals.astype(np.float32).astype(np.float64))
+        else:
+            self._set_n_plane(nh)
+
+    # ------------------------------------------------------------------ state access
+    def _clockv(self) -> L.GsClock:
+        return L.read_clock(self._clock)
+
+    @property
+    def t(self) -> float:
+        return float(self._clockv().t)
+
+    @property
+    def nsteps(self) -> int:
+        return int(self._clockv().nsteps)
+
+    def _cur(self, c: Optional[L.GsClock] = None) -> int:
+        c = c or self._clockv()
+        return int(c.nsteps) & 1
+
+    def _read_planes(self, which: int) -> np.ndarray:
+        ny = self.mesh.ny
+        return np.stack([self._bufs[which][k].download()[:, :ny] for k in range(3)])
+
+    @property
+    def q_interior(self) -> np.ndarray:
+        """Conserved state without ghosts, (3, nx, ny), as a host array."""
+        return self._read_planes(self._cur())
+
+    @property
+    def q(self) -> np.ndarray:
+        """Padded (3, nxp, nyp) host copy of the state with the ghost cells filled."""
+        g = self.mesh.ngh
+        nx, ny = self.mesh.nx, self.mesh.ny
+        out = np.zeros((3, nx + 2 * g, ny + 2 * g), self.dtype)
+        out[:, g:g + nx, g:g + ny] = self.q_interior
+        host_bc_fill(out, g, self.cfg.bc_x, self.cfg.bc_y)
+        return out
+
+    @property
+    def b(self) -> np.ndarray:
+        return host_pad_bed(self._b_host, self.mesh.ngh, self.cfg.bc_x, self.cfg.bc_y)
+
+    def depth(self) -> np.ndarray:
+        return self.q_interior[0]
+
+    def max_depth(self) -> np.ndarray:
+        if self._max_h is None:
+            raise RuntimeError("max_depth: this solver was built with track_max_depth=False")
+        if not self._any_step:
+            return self.depth()
+        return self._max_h.download()[:, :self.mesh.ny]
+
+    def set_state(self, q_interior: np.ndarray) -> None:
+        """Replace the current state (3, nx, ny) in both buffers."""
+        nx, ny = self.mesh.nx, self.mesh.ny
+        q_interior = np.asarray(q_interior)
+        if q_interior.shape != (3, nx, ny):
+            raise ValueError(f"set_state expects (3, {nx}, {ny})")
+        host = np.zeros((3, nx, self.pitch), self.dtype)
+        host[:, :, :ny] = q_interior.astype(self.dtype, copy=False)
+        for s in range(2):
+            for k in range(3):
+                self._bufs[s][k].upload(host[k])
+        self._lam_valid = False
+
+    # ------------------------------------------------------------------ stepping
+    def _ensure_lam(self):
+        if not self._lam_valid:
+            L.call("gs_dense_cfl", C.byref(self._desc()), C.byref(self._phys()), self._fo.s,
+                   C.c_void_p(self._clock.ptr), self.stream)
+            self._lam_valid = True
+
+    def _raise_if_bad(self, c: L.GsClock):
+        if c.status & L.ST_NONFINITE:
+            lam = np.uint64(c.lam_bits).view(np.float64)
+            raise FloatingPointError(f"cfl_dt: non-finite max wave speed {lam} at t={c.t:.3f}s -- NaN/Inf in "
+                                     f"the state; check forcing/inputs")
+
+    def cfl_dt(self) -> float:
+        """CFL step of the current state (solver.py:2569-2693), from the device wave speed."""
+        self._ensure_lam()
+        c = self._clockv()
+        lam = float(np.uint64(c.lam_bits).view(np.float64))
+        if not math.isfinite(lam):
+            self._raise_if_bad(c)
+            raise FloatingPointError(f"cfl_dt: non-finite max wave speed {lam}")
+        floor = (self.cfg.g * self.cfg.h_cfl()) ** 0.5
+        if lam < floor:
+            lam = floor
+        return self.cfg.cfl * min(self.mesh.dx, self.mesh.dy) / (lam + 1.0e-12)
+
+    def _open_mode(self):
+        if self._target != "open":
+            L.call("gs_clock_set_target", C.c_void_p(self._clock.ptr), C.c_double(math.inf), C.c_double(0.0),
+                   C.c_int64(-1), C.byref(self._phys()), self._fo.s, self.stream)
+            self._target = "open"
+
+    def step(self, dt: Optional[float] = None) -> float:
+        """Advance one step of size dt (the CFL step if None), solver.py:2695-2973.
+
+        With an explicit dt nothing is read back from the device.
+        """
+        self._ensure_lam()
+        self._open_mode()
+        if dt is not None:
+            dt = float(dt)
+            L.call("gs_clock_set_dt", C.c_void_p(self._clock.ptr), C.c_double(dt), C.byref(self._phys()),
+                   self._fo.s, self.stream)
+        else:
+            c = self._clockv()
+            self._raise_if_bad(c)
+            dt = float(c.dt)
+        L.call("gs_dense_step", C.byref(self._desc()), C.byref(self._phys()), self._fo.s,
+               C.c_void_p(self._clock.ptr), self.stream)
+        self._any_step = True
+        self.last_dt = dt
+        return dt
+
+    def advance(self, nsteps: int, sync: bool = True) -> None:
+        """Enqueue ``nsteps`` CFL steps back to back with no host synchronisation in between."""
+        self._ensure_lam()
+        self._open_mode()
+        d, p = C.byref(self._desc()), C.byref(self._phys())
+        clk = C.c_void_p(self._clock.ptr)
+        for _ in range(int(nsteps)):
+            L.call("gs_dense_step", d, p, self._fo.s, clk, self.stream)
+        self._any_step = True
+        if sync:
+            L.sync(self.stream)
+            self._raise_if_bad(self._clockv())
+
+    def run(self, t_end: float, max_steps: int = 10 ** 7, callback: Optional[Callable] = None,
+            dt_max: Optional[float] = None, batch: int = 64) -> int:
+        """Step to t_end with CFL steps (solver.py:2975-3003); dt is planned on the device."""
+        t0 = time.perf_counter()
+        self._ensure_lam()
+        n0 = self.nsteps
+        if callback is not None:
+            steps = 0
+            while steps < max_steps:
+                L.call("gs_clock_set_target", C.c_void_p(self._clock.ptr), C.c_double(float(t_end)),
+                       C.c_double(float(dt_max) if dt_max else 0.0), C.c_int64(1), C.byref(self._phys()),
+                       self._fo.s, self.stream)
+                c = self._clockv()
+                self._raise_if_bad(c)
+                if c.status & L.ST_DONE:
+                    break
+                L.call("gs_dense_step", C.byref(self._desc()), C.byref(self._phys()), self._fo.s,
+                       C.c_void_p(self._clock.ptr), self.stream)
+                self._any_step = True
+                steps += 1
+                callback(self, steps)
+        else:
+            L.call("gs_clock_set_target", C.c_void_p(self._clock.ptr), C.c_double(float(t_end)),
+                   C.c_double(float(dt_max) if dt_max else 0.0), C.c_int64(int(max_steps)),
+                   C.byref(self._phys()), self._fo.s, self.stream)
+            d, p = C.byref(self._desc()), C.byref(self._phys())
+            clk = C.c_void_p(self._clock.ptr)
+            while True:
+                c = self._clockv()
+                self._raise_if_bad(c)
+                if c.status & L.ST_DONE:
+                    break
+                for _ in range(batch):
+                    L.call("gs_dense_step", d, p, self._fo.s, clk, self.stream)
+                self._any_step = True
+        self._target = None
+        steps = self.nsteps - n0
+        self.diagnostics["wallclock"] = time.perf_counter() - t0
+        self.diagnostics["steps"] = steps
+        return steps
+
+    # ------------------------------------------------------------------ parity helpers
+    def residual(self) -> np.ndarray:
+        """SRM-HLLC residual (3, nx, ny) of the current state (inactive cells 0)."""
+        nx, ny = self.mesh.nx, self.mesh.ny
+        r = [L.DeviceArray(self._plane, self.dtype, zero=True) for _ in range(3)]
+        L.call("gs_dense_residual", C.byref(self._desc()), C.byref(self._phys()), self._cur(),
+               C.c_void_p(r[0].ptr), C.c_void_p(r[1].ptr), C.c_void_p(r[2].ptr), self.stream)
+        return np.stack([x.download()[:, :ny] for x in r])

@@ -1,0 +1,245 @@
+/*
+ * geoswe_abi.h -- C ABI of the MI355X (gfx950) fused finite-volume time step.
+ *
+ * The library replaces the reference's kernel-factory boundary: GeoSWE compiles CUDA C
+ * strings at run time through backend.raw_kernel / cupy.RawKernel
+ * (src/geoswe/backend.py:271-302) and calls them positionally from Solver2D and
+ * CompressedSolver (src/geoswe/solver.py:2275-2279, compressed_solver.py:894-898).
+ * Here the kernels are compiled ahead of time for gfx950 and reached through these
+ * extern "C" entry points with plain pointers and sizes.
+ *
+ * Conventions
+ *   - Every function returns 0 on success, a positive GS_E* code on a usage error, or
+ *     a negative HIP error code (-hipError_t). gs_last_error() describes the last
+ *     failure of the calling thread.
+ *   - The caller owns every buffer. Step entry points never allocate and never
+ *     synchronise the host: dt, t and the CFL wave speed stay in a device-resident
+ *     GsClock, so K steps can be enqueued (or captured in a graph) back to back.
+ *   - Streams are hipStream_t passed as void* (NULL = the null stream).
+ *   - Arrays are row-major with the second index (j, "y") contiguous, as the
+ *     reference's C-order (nx, ny) fields (src/geoswe/mesh.py:45-87).
+ */
+#ifndef GEOSWE_ABI_H
+#define GEOSWE_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_ABI_VERSION 1
+
+/* error codes (positive); negative values are -hipError_t */
+#define GS_OK 0
+#define GS_E_ARG 1        /* invalid argument (shape, pointer, enum) */
+#define GS_E_DTYPE 2      /* unsupported dtype */
+#define GS_E_RANGE 3      /* size out of the supported range */
+#define GS_E_STATE 4      /* call order / state error */
+
+/* boundary kinds: src/geoswe/bc.py:50-98 */
+#define GS_BC_EXTRAPOLATE 0
+#define GS_BC_PERIODIC 1
+#define GS_BC_WALL 2
+#define GS_BC_FALL 3
+
+/* dtypes */
+#define GS_F32 0
+#define GS_F64 1
+
+/* per-cell flags byte (dense "flags" plane, flat "flags" array) */
+#define GS_FLAG_CLASS_MASK 0x3F  /* Manning class id (0..63) into the ntab array */
+#define GS_FLAG_INACTIVE 0x40    /* outside the inside mask: zero residual, no rain */
+#define GS_FLAG_STAGE 0x80       /* stage-boundary cell: h = max(eta(t)-b, 0), hu = hv = 0 */
+
+/* GsClock.status bits */
+#define GS_ST_DONE 1      /* t reached t_end or nsteps reached max_steps: steps are no-ops */
+#define GS_ST_NONFINITE 2 /* the wave speed of the current state is NaN or Inf */
+
+/*
+ * Device-resident schedule of a run (one per solver). Mirrors the host loop of
+ * Solver2D.run / cfl_dt (solver.py:2569-2693, 2975-3027): the last workgroup of each
+ * step turns the step's wave-speed maximum into the next dt on the device.
+ */
+typedef struct GsClock {
+  double t;            /* simulated time of the current state */
+  double dt;           /* step to take next (float64, as the reference's host dt) */
+  double t_end;        /* run() target; +inf when stepping open-ended */
+  double dt_max;       /* cap on dt, 0 = none */
+  double eta64;        /* stage level at t + dt */
+  int64_t nsteps;      /* steps taken so far; parity selects the input buffer */
+  int64_t max_steps;   /* status gets GS_ST_DONE when nsteps reaches this */
+  uint64_t lam_acc;    /* running maximum of the step in flight (float64 IEEE bits) */
+  uint64_t lam_bits;   /* CFL wave speed of the current state (float64 IEEE bits) */
+  uint32_t blocks_done;/* last-workgroup-done counter */
+  int32_t status;      /* GS_ST_* */
+  float dt32;          /* (float)dt */
+  float rain32;        /* uniform rain rate of the next step, m/s, float */
+  float eta32;         /* (float)eta64 */
+  int32_t rain_idx;    /* rain segment / frame index of the next step */
+  int32_t use_host_dt; /* 1: dt was set by the host for the next step */
+  int32_t run_mode;    /* 1: Solver2D.run rules (rain-film cap, dt_max, clip to t_end) */
+  int32_t pad_[6];
+} GsClock;
+
+/* Physics and scheme constants (Config, solver.py:629-860). */
+typedef struct GsPhys {
+  double g;            /* gravity */
+  double h_min;        /* wet/dry threshold */
+  double h_cfl;        /* CFL depth floor (h_min_cfl, floored at 1e-6 in float32) */
+  double cfl;          /* Courant number */
+  double dmin;         /* min(dx, dy) */
+  double lam_floor;    /* (g*h_cfl)**0.5 computed by the host, solver.py:2663-2665 */
+  double vcap;         /* friction velocity cap (m/s); inf disables it */
+  int32_t friction;    /* 0 none, 1 Manning (point-implicit) */
+  int32_t quadratic;   /* 1 quadratic-alpha friction root, 0 linearised */
+  int32_t keep_h;      /* 1 keep the sub-threshold depth on dry cells, 0 zero it */
+  int32_t cfl_l2;      /* CFL velocity norm: 0 max(|u|,|v|), 1 sqrt(u^2+v^2) */
+  int32_t n_classes;   /* entries of the Manning table (1..64) */
+  int32_t pad_;
+} GsPhys;
+
+/* Forcing tables (device pointers, float64), shared by all layouts. */
+typedef struct GsForcing {
+  int32_t n_rain;            /* rain segments (0 = no rain) */
+  int32_t rain_gridded;      /* 1: rain_frames holds one frame per segment */
+  const double* rain_t;      /* [n_rain] segment start times, non-decreasing */
+  const double* rain_rate;   /* [n_rain] uniform rate in m/s (rate_mm_h/3.6e6) */
+  const double* rain_cap;    /* [n_rain] rain-film dt bound per segment (inf if dry) */
+  const void* rain_frames;   /* gridded: [n_rain][frame_stride] rates in m/s (state dtype) */
+  int64_t rain_frame_stride; /* elements between frames */
+  int64_t rain_row_pitch;    /* elements between rows of a frame (dense: interior rows) */
+  int32_t n_stage;           /* stage samples (0 = no stage boundary) */
+  int32_t pad_;
+  const double* stage_t;     /* [n_stage] */
+  const double* stage_v;     /* [n_stage] */
+} GsForcing;
+
+/*
+ * Dense layout. The state and bed hold interior cells only, nx rows of `pitch`
+ * elements; ghost cells are not stored but synthesised inside the stencil loads
+ * from the boundary kinds (bc.py:50-98, ghost bed solver.py:1466-1484).
+ */
+typedef struct GsDense {
+  int32_t nx, ny;            /* interior cells */
+  int64_t pitch;             /* elements between rows of every plane */
+  int32_t bc_x, bc_y;        /* GS_BC_* */
+  int32_t dtype;             /* GS_F32 or GS_F64 */
+  int32_t tile_rows;         /* rows swept per wavefront (0 = library default) */
+  double dx, dy;
+  void* q[2][3];             /* double-buffered state planes h, hu, hv */
+  const void* bed;           /* bed plane */
+  const uint8_t* flags;      /* nullable per-cell GS_FLAG_* byte plane */
+  const void* ntab;          /* Manning n per class, device array of n_classes (state dtype) */
+  const void* n_field;       /* nullable per-cell Manning n plane (overrides ntab) */
+  void* max_h;               /* nullable running-max depth plane */
+} GsDense;
+
+/*
+ * Flat (compressed active-cell) layout: stored cells in row-major padded (i, j) order
+ * (compressed_mesh.py:139-157). Active cells are listed by runs of consecutive flat
+ * ids in one row whose +-1/+-2 row neighbours sit at constant id offsets; cells in no
+ * run are the frozen ring (compressed_rhs.py:241-369). Within a row the +-j
+ * neighbours of an active cell are k+-1, k+-2 (the ring-2 cross dilation).
+ */
+typedef struct GsFlat {
+  int64_t n_stored;          /* stored cells N */
+  int32_t n_runs;            /* runs of active cells */
+  int32_t dtype;
+  int32_t nyp;               /* padded row length (rain indexing) */
+  int32_t ngh;               /* ghost width of the padded frame */
+  double dx, dy;
+  void* q[2][3];             /* double-buffered flat state */
+  const void* bed;           /* flat bed */
+  const uint8_t* flags;      /* nullable per-cell GS_FLAG_* byte (class, stage) */
+  const void* ntab;          /* Manning n per class, device array (state dtype) */
+  const void* n_field;       /* nullable per-cell Manning n */
+  void* max_h;               /* nullable running max */
+  const int64_t* run_k0;     /* [n_runs] first flat id of the run */
+  const int32_t* run_len;    /* [n_runs] cells in the run */
+  const int32_t* run_i;      /* [n_runs] padded row of the run */
+  const int32_t* run_j0;     /* [n_runs] padded column of the first cell */
+  const int32_t* run_d;      /* [n_runs][4] id offsets of rows i+1, i-1, i+2, i-2 */
+  const int32_t* chunk_run;  /* [ceil(N/64)] first run whose end is past the chunk start */
+} GsFlat;
+
+/* ---- library ---- */
+int gs_version(void);
+const char* gs_last_error(void);
+int gs_get_device_count(int* n);
+int gs_set_device(int dev);
+
+/* ---- memory and streams (the library is the allocator of the plain solver path) ---- */
+int gs_malloc(void** ptr, size_t bytes);
+int gs_free(void* ptr);
+int gs_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int gs_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int gs_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int gs_memcpy2d(void* dst, size_t dpitch, const void* src, size_t spitch, size_t width,
+                size_t height, int kind, void* stream); /* kind: 1 h2d, 2 d2h, 3 d2d */
+int gs_memset(void* dst, int value, size_t bytes, void* stream);
+int gs_stream_create(void** stream);
+int gs_stream_destroy(void* stream);
+int gs_stream_synchronize(void* stream);
+int gs_event_create(void** ev);
+int gs_event_destroy(void* ev);
+int gs_event_record(void* ev, void* stream);
+int gs_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ---- clock ---- */
+/* Reset a device clock to t0 (t_end = inf, no dt cap, nsteps 0). */
+int gs_clock_reset(GsClock* clock_dev, double t0, void* stream);
+/* Set the run target: t_end (inf = open), dt_max (0 = none), max_steps (<0 = none);
+ * clears GS_ST_DONE and re-plans dt for the current state. */
+int gs_clock_set_target(GsClock* clock_dev, double t_end, double dt_max, int64_t max_steps,
+                        const GsPhys* phys, GsForcing forc, void* stream);
+/* Use `dt` (float64) for the next step instead of the CFL step (Solver2D.step(dt)). */
+int gs_clock_set_dt(GsClock* clock_dev, double dt, const GsPhys* phys, GsForcing forc,
+                    void* stream);
+
+/* ---- dense layout ---- */
+/*
+ * Replaces: the SRM-HLLC residual + update + friction + wet/dry + running-max fused
+ * step fused_rhs_wb_srm_hllc_compact_fstepcfl_* (rhs_cuda.py:1746-1796, 2095-2149,
+ * launched at solver.py:2271-2299), the ghost fill apply_bc_2d (bc.py:50-98), the
+ * stage overwrite StageBoundary.apply (forcing.py:349-377) and the next step's
+ * lammax_fp32_lean + cfl_dt host arithmetic (solver.py:152-205, 2569-2666).
+ * Reads q[nsteps & 1], writes q[(nsteps+1) & 1], advances the clock.
+ */
+int gs_dense_step(const GsDense* d, const GsPhys* phys, GsForcing forc, GsClock* clock_dev,
+                  void* stream);
+/*
+ * Replaces: fused_rhs_wb_srm_hllc_2d_* (rhs_cuda.py:900-1208) for parity checks:
+ * residual of q[which] into rhs planes (interior cells; flags' inactive cells get 0).
+ */
+int gs_dense_residual(const GsDense* d, const GsPhys* phys, int which, void* rhs0, void* rhs1,
+                      void* rhs2, void* stream);
+/*
+ * Replaces: lammax_fp32_lean (solver.py:152-205) + the dt rule of cfl_dt
+ * (solver.py:2657-2666): wave speed of q[nsteps & 1] into clock->lam_bits and the
+ * next dt planned on the device.
+ */
+int gs_dense_cfl(const GsDense* d, const GsPhys* phys, GsForcing forc, GsClock* clock_dev,
+                 void* stream);
+
+/* ---- flat layout ---- */
+/*
+ * Replaces: flat_srm_hllc_fstepcfl_* (compressed_rhs.py:673-865, 1000-1118) launched
+ * from CompressedStepper.rhs_fused (compressed_solver.py:880-904) plus the dt rule of
+ * dt_from_lam (compressed_solver.py:713-719).
+ */
+int gs_flat_step(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev,
+                 void* stream);
+/* Replaces: flat_srm_hllc_* residual (compressed_rhs.py:129-158) for parity checks. */
+int gs_flat_residual(const GsFlat* f, const GsPhys* phys, int which, void* rhs0, void* rhs1,
+                     void* rhs2, void* stream);
+/* Replaces: cfl_lammax_flat_blk + CompressedStepper.cfl_dt (compressed_solver.py:200-240, 721-748). */
+int gs_flat_cfl(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev,
+                void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GEOSWE_ABI_H */

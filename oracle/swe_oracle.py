@@ -1,0 +1,582 @@
+"""NumPy restatement of GeoSWE's production finite-volume time step (test oracle).
+
+Test infrastructure only: imported by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py``; never by the product package.
+
+The arithmetic follows the reference's fused CUDA kernel text term by term and in the
+same evaluation order (no FMA contraction), so in float32 it is the bit pattern the
+production kernel is specified to produce, and in float64 it is the FP64 oracle. The
+array-op structure (face arrays, ``np.where`` chains) mirrors the reference's own NumPy
+path, which makes its timing representative of that CPU path.
+
+Sources followed (paths relative to the reference checkout):
+
+* primitives with dry gate ............ src/geoswe/rhs_cuda.py:931-938 (swe.py:12-27)
+* central bed gradients ............... rhs_cuda.py:1644-1650 (well_balanced.py:234-241)
+* SRM face reconstruction ............. rhs_cuda.py:1042-1184 (well_balanced.py:244-269)
+* HLLC flux ........................... rhs_cuda.py:986-1036 (flux.py:56-151)
+* one-sided SRM bed source ............ rhs_cuda.py:1060-1183 (well_balanced.py:303-362)
+* fused update tail ................... rhs_cuda.py:1757-1796; flat form compressed_rhs.py:716-748
+* CFL wave speed ...................... solver.py:152-205, 2569-2666; compressed_solver.py:200-240
+* dt rule and rain step limit ......... solver.py:2657-2666, 2975-3027
+* ghost-cell fill ..................... bc.py:50-98; ghost bed solver.py:1466-1484
+* rain lookup / stage boundary ........ forcing.py:170-216, 315-377
+
+Documented departures (shared with the HIP kernels, which implement the same rule):
+
+* ``h ** (-4/3)`` in the friction coefficient uses ``pow_m43`` (an inverse-cube-root
+  Newton iteration built from IEEE +,-,* only) instead of ``powf(h, -4/3)``, so CPU and
+  GPU results are reproducible bit for bit. Its error against the exact power is below
+  13 ulp, the same order as ``powf(h, (float)(-4/3))`` itself (which carries the
+  rounding of the exponent).
+* The CFL reduction keeps NaN (IEEE-bit maximum), as the reference flat kernels do
+  (compressed_solver.py:200-240); the dense reference kernel drops a NaN lambda.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+G = 9.81
+
+BC_KINDS = ("extrapolate", "periodic", "wall", "fall")
+
+
+# ---------------------------------------------------------------------------
+# boundary fill (bc.py:50-98) and ghost bed (solver.py:1466-1484)
+# ---------------------------------------------------------------------------
+
+def bc_fill(q: np.ndarray, ngh: int, bc_x: str, bc_y: str) -> None:
+    """Fill the ghost layers of a padded state ``q`` (3, nxp, nyp) in place, x then y.
+
+    Restates bc.py:50-98: extrapolate copies the last interior layer, periodic wraps,
+    wall mirrors with the normal momentum negated, fall zeroes the ghosts.
+    Pinned by tests/test_oracle_bc.py (hand-built 1-cell cases per kind).
+    """
+    for axis, kind in ((1, bc_x), (2, bc_y)):
+        n = q.shape[axis]
+        sl = lambda a, b: (slice(None),) * axis + (slice(a, b),)
+        if kind == "periodic":
+            q[sl(0, ngh)] = q[sl(n - 2 * ngh, n - ngh)]
+            q[sl(n - ngh, n)] = q[sl(ngh, 2 * ngh)]
+        elif kind == "extrapolate":
+            q[sl(0, ngh)] = q[sl(ngh, ngh + 1)]
+            q[sl(n - ngh, n)] = q[sl(n - ngh - 1, n - ngh)]
+        elif kind == "wall":
+            lo = np.flip(q[sl(ngh, 2 * ngh)], axis=axis).copy()
+            hi = np.flip(q[sl(n - 2 * ngh, n - ngh)], axis=axis).copy()
+            lo[axis] = -lo[axis]          # normal momentum: hu for x (axis 1), hv for y (axis 2)
+            hi[axis] = -hi[axis]
+            q[sl(0, ngh)] = lo
+            q[sl(n - ngh, n)] = hi
+        elif kind == "fall":
+            q[sl(0, ngh)] = 0
+            q[sl(n - ngh, n)] = 0
+        else:
+            raise ValueError(f"unknown BC kind {kind!r}")
+
+
+def pad_bed(b_int: np.ndarray, ngh: int, bc_x: str, bc_y: str, dtype=np.float32) -> np.ndarray:
+    """Padded bed with ghosts edge-replicated (periodic axes wrap), x first (solver.py:1466-1484)."""
+    nx, ny = b_int.shape
+    b = np.zeros((nx + 2 * ngh, ny + 2 * ngh), dtype=dtype)
+    b[ngh:ngh + nx, ngh:ngh + ny] = b_int.astype(dtype, copy=False)
+    if bc_x == "periodic":
+        b[:ngh, :] = b[-2 * ngh:-ngh, :]
+        b[-ngh:, :] = b[ngh:2 * ngh, :]
+    else:
+        b[:ngh, :] = b[ngh:ngh + 1, :]
+        b[-ngh:, :] = b[-ngh - 1:-ngh, :]
+    if bc_y == "periodic":
+        b[:, :ngh] = b[:, -2 * ngh:-ngh]
+        b[:, -ngh:] = b[:, ngh:2 * ngh]
+    else:
+        b[:, :ngh] = b[:, ngh:ngh + 1]
+        b[:, -ngh:] = b[:, -ngh - 1:-ngh]
+    return b
+
+
+# ---------------------------------------------------------------------------
+# scalar helpers
+# ---------------------------------------------------------------------------
+
+def kernel_consts(dx: float, dy: float, g: float, h_min: float, dtype):
+    """The kernel's scalar arguments as the reference casts them (solver.py:2275-2279):
+    ``inv_dx = (T)(1/dx)`` and, inside the kernel, ``dx = 1/inv_dx`` in T (rhs_cuda.py:928-929)."""
+    T = np.dtype(dtype).type
+    inv_dx = T(1.0 / dx)
+    inv_dy = T(1.0 / dy)
+    return dict(T=T, inv_dx=inv_dx, inv_dy=inv_dy, dxk=T(1) / inv_dx, dyk=T(1) / inv_dy,
+                g=T(g), h_min=T(h_min))
+
+
+def pow_m43(h: np.ndarray) -> np.ndarray:
+    """``h ** (-4/3)`` for float32 ``h > 0`` from IEEE +,-,* only (bit-reproducible).
+
+    Inverse cube root y = h^(-1/3) by Newton's method on f(y) = y^-3 - h,
+    ``y <- y * (4/3 - (h/3) * y^3)``, three iterations from the exponent-halving seed
+    ``bits(y0) = 0x54a2fa8c - bits(h)/3`` (the magic-constant seed of Moroz et al.,
+    "Fast calculation of cube and inverse cube roots using a magic constant", 2021),
+    then ``h^(-4/3) = (y*y)*(y*y)``. Float64 input returns the exact power (FP64 oracle).
+    Pinned by tests/test_oracle_pow43.py against float64 ``h**(-4/3)`` (< 13 ulp).
+    """
+    h = np.asarray(h)
+    if h.dtype == np.float64:
+        return h ** (-4.0 / 3.0)
+    T = np.float32
+    bits = h.astype(T, copy=False).view(np.uint32)
+    y = (np.uint32(0x54A2FA8C) - bits // np.uint32(3)).view(T)
+    xt = h * T(1.0 / 3.0)
+    c43 = T(4.0 / 3.0)
+    for _ in range(3):
+        y = y * (c43 - xt * ((y * y) * y))
+    y2 = y * y
+    return y2 * y2
+
+
+# ---------------------------------------------------------------------------
+# HLLC (rhs_cuda.py:986-1036) and SRM face reconstruction (rhs_cuda.py:1042-1184)
+# ---------------------------------------------------------------------------
+
+def hllc(hL, uNL, uTL, hR, uNR, uTR, g, h_min):
+    """HLLC flux (Fh, FN, FT) in the face frame, kernel form.
+
+    Differs from flux.py:80-151 exactly as the production kernel does: cell velocities
+    gated on the cell depth, no SM clamp (rhs_cuda.py:1001-1007), Sigma = 0.
+    Pinned by tests/test_oracle_hllc.py: consistency F(q,q) = physical flux, x/y and
+    mirror symmetry, both-dry zero, and the reference NumPy flux on wet states.
+    """
+    T = type(g)
+    z = T(0)
+    aL = np.sqrt(g * np.where(hL > z, hL, z))
+    aR = np.sqrt(g * np.where(hR > z, hR, z))
+    u_st = (T(0.5) * (uNL + uNR) + aL) - aR
+    h_rt = T(0.5) * (aL + aR) + T(0.25) * (uNL - uNR)
+    h_st = np.where(h_rt > z, (h_rt * h_rt) / g, z)
+    a_st = np.sqrt(g * np.where(h_st > z, h_st, z))
+    SL_w = np.minimum(uNL - aL, u_st - a_st)
+    SR_w = np.maximum(uNR + aR, u_st + a_st)
+    dryL = hL <= h_min
+    dryR = hR <= h_min
+    SL = np.where(dryL, uNR - T(2.0) * aR, SL_w)
+    SR = np.where(dryR, uNL + T(2.0) * aL, SR_w)
+    both = dryL & dryR
+    SL = np.where(both, z, SL)
+    SR = np.where(both, z, SR)
+    denom = hR * (uNR - SR) - hL * (uNL - SL)
+    big = np.abs(denom) > T(1.0e-14)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        SM = np.where(big, ((SL * hR) * (uNR - SR) - (SR * hL) * (uNL - SL)) / np.where(big, denom, T(1)),
+                      T(0.5) * (uNL + uNR))
+    huNL = hL * uNL
+    huNR = hR * uNR
+    huTL = hL * uTL
+    huTR = hR * uTR
+    pL = (T(0.5) * g) * hL * hL
+    pR = (T(0.5) * g) * hR * hR
+    FL_h, FL_N, FL_T = huNL, huNL * uNL + pL, huNL * uTL
+    FR_h, FR_N, FR_T = huNR, huNR * uNR + pR, huNR * uTR
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        facL = (SL - uNL) / ((SL - SM) + T(1.0e-30))
+        facR = (SR - uNR) / ((SR - SM) + T(1.0e-30))
+    hLs = hL * facL
+    hRs = hR * facR
+    FLs_h = FL_h + SL * (hLs - hL)
+    FLs_N = FL_N + SL * (hLs * SM - huNL)
+    FLs_T = FL_T + SL * (hLs * uTL - huTL)
+    FRs_h = FR_h + SR * (hRs - hR)
+    FRs_N = FR_N + SR * (hRs * SM - huNR)
+    FRs_T = FR_T + SR * (hRs * uTR - huTR)
+    c1, c2, c3 = SL >= z, SM >= z, SR >= z
+    Fh = np.where(c1, FL_h, np.where(c2, FLs_h, np.where(c3, FRs_h, FR_h)))
+    FN = np.where(c1, FL_N, np.where(c2, FLs_N, np.where(c3, FRs_N, FR_N)))
+    FT = np.where(c1, FL_T, np.where(c2, FLs_T, np.where(c3, FRs_T, FR_T)))
+    return Fh, FN, FT
+
+
+def srm_face(bL, bR, gzL, gzR, etaL, etaR, hL, hR, half_d, g, h_min):
+    """SRM face depths and the two one-sided bed sources of one face (rhs_cuda.py:1042-1115).
+
+    Returns (h_Lf, h_Rf, gate, srcL, srcR): ``srcL`` is the lower-index cell's source in
+    its own frame (it subtracts ``srcL*inv_d``), ``srcR`` the upper cell's (it adds
+    ``srcR*inv_d``), each with the dry gate on the *other* cell's depth. The face values
+    are the same bits in either cell's frame (IEEE negation is exact), so computing a
+    face once is the reference's per-cell evaluation.
+    Pinned by lake-at-rest (tests/test_oracle_invariants.py) and the reference NumPy
+    residual golden vectors.
+    """
+    T = type(g)
+    z = T(0)
+    zL = bL + half_d * gzL
+    zR = bR - half_d * gzR
+    zf0 = np.where(bL > bR, bL, bR)
+    dzc = zR - zL
+    dz = (bR - bL) - dzc
+    detaL = np.maximum(z, np.minimum(dz, etaR - etaL))
+    detaR = np.maximum(z, np.minimum(-dz, etaL - etaR))
+    hLf = np.maximum(z, (etaL + detaL) - zf0)
+    hRf = np.maximum(z, (etaR + detaR) - zf0)
+    gate = ~((hL < h_min) & (hR < h_min))
+    # owner L: gate on the neighbour R being dry
+    headL = zf0 - etaL
+    dzL = np.where(hR < h_min, np.maximum(z, headL), np.maximum(z, np.minimum(dzc, headL)))
+    srcL = ((T(0.5) * g) * (hLf + hL)) * ((zf0 - dzL) - bL)
+    # owner R (its left face): dz_clip in R's frame is zL - zR
+    headR = zf0 - etaR
+    dzR = np.where(hL < h_min, np.maximum(z, headR), np.maximum(z, np.minimum(zL - zR, headR)))
+    srcR = ((T(0.5) * g) * (hRf + hR)) * ((zf0 - dzR) - bR)
+    return hLf, hRf, gate, srcL, srcR
+
+
+def primitives(q0, q1, q2, h_min):
+    """Depth, velocities with the dry gate, as the kernel's PRIMS macro (rhs_cuda.py:931-938)."""
+    T = type(h_min)
+    wet = q0 > h_min
+    hs = np.where(wet, q0, h_min)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        u = np.where(wet, q1 / hs, T(0))
+        v = np.where(wet, q2 / hs, T(0))
+    return u, v
+
+
+# ---------------------------------------------------------------------------
+# residual on a padded grid (the kernel window [2, n-3]^2)
+# ---------------------------------------------------------------------------
+
+def residual(q: np.ndarray, b: np.ndarray, dx: float, dy: float, g: float = G,
+             h_min: float = 1e-6, active: Optional[np.ndarray] = None) -> np.ndarray:
+    """SRM-HLLC residual of a padded, ghost-filled state (rhs_cuda.py:900-1208).
+
+    ``q`` (3, nxp, nyp) and ``b`` (nxp, nyp) share a dtype (float32 or float64). The
+    residual is evaluated on the kernel window [2, nxp-3] x [2, nyp-3] intersected with
+    ``active`` (padded uint8/bool, None = all); every other cell gets 0, like the
+    compact launch plus the ``inside_mask == 0`` branch. Accumulation order per cell is
+    x right face (h, hu, hv, hu source), x left, y top, y bottom (Appendix A.2 order).
+    """
+    dt_ = q.dtype
+    c = kernel_consts(dx, dy, g, h_min, dt_)
+    T, ix, iy, gk, hm = c["T"], c["inv_dx"], c["inv_dy"], c["g"], c["h_min"]
+    z = T(0)
+    h, hu, hv = q[0], q[1], q[2]
+    nxp, nyp = h.shape
+    u, v = primitives(h, hu, hv, hm)
+    eta = h + b
+    gzx = np.zeros_like(b)
+    gzy = np.zeros_like(b)
+    gzx[1:-1, :] = (T(0.5) * (b[2:, :] - b[:-2, :])) * ix
+    gzy[:, 1:-1] = (T(0.5) * (b[:, 2:] - b[:, :-2])) * iy
+    hdx = T(0.5) * c["dxk"]
+    hdy = T(0.5) * c["dyk"]
+    # x faces between rows i and i+1 (shape nxp-1, nyp)
+    xhL, xhR, xg, xsL, xsR = srm_face(b[:-1], b[1:], gzx[:-1], gzx[1:], eta[:-1], eta[1:],
+                                      h[:-1], h[1:], hdx, gk, hm)
+    xFh, xFN, xFT = hllc(xhL, u[:-1], v[:-1], xhR, u[1:], v[1:], gk, hm)
+    # y faces between columns j and j+1 (shape nxp, nyp-1): normal v, transverse u
+    yhL, yhR, yg, ysL, ysR = srm_face(b[:, :-1], b[:, 1:], gzy[:, :-1], gzy[:, 1:], eta[:, :-1],
+                                      eta[:, 1:], h[:, :-1], h[:, 1:], hdy, gk, hm)
+    yFh, yFN, yFT = hllc(yhL, v[:, :-1], u[:, :-1], yhR, v[:, 1:], u[:, 1:], gk, hm)
+
+    W = (slice(2, nxp - 2), slice(2, nyp - 2))          # the kernel window
+    # face views per window cell
+    R = (slice(2, nxp - 2), slice(2, nyp - 2))          # x face i+1/2 has index i
+    L = (slice(1, nxp - 3), slice(2, nyp - 2))          # x face i-1/2 has index i-1
+    Tt = (slice(2, nxp - 2), slice(2, nyp - 2))         # y face j+1/2 has index j
+    B = (slice(2, nxp - 2), slice(1, nyp - 3))          # y face j-1/2 has index j-1
+    shp = (nxp - 4, nyp - 4)
+    rh = np.zeros(shp, dt_)
+    rhu = np.zeros(shp, dt_)
+    rhv = np.zeros(shp, dt_)
+    gR, gL, gT, gB = xg[R], xg[L], yg[Tt], yg[B]
+    with np.errstate(over="ignore", invalid="ignore"):
+        rh = np.where(gR, rh - xFh[R] * ix, rh)
+        rhu = np.where(gR, rhu - xFN[R] * ix, rhu)
+        rhv = np.where(gR, rhv - xFT[R] * ix, rhv)
+        rhu = np.where(gR, rhu - xsL[R] * ix, rhu)
+        rh = np.where(gL, rh + xFh[L] * ix, rh)
+        rhu = np.where(gL, rhu + xFN[L] * ix, rhu)
+        rhv = np.where(gL, rhv + xFT[L] * ix, rhv)
+        rhu = np.where(gL, rhu + xsR[L] * ix, rhu)
+        rh = np.where(gT, rh - yFh[Tt] * iy, rh)
+        rhu = np.where(gT, rhu - yFT[Tt] * iy, rhu)
+        rhv = np.where(gT, rhv - yFN[Tt] * iy, rhv)
+        rhv = np.where(gT, rhv - ysL[Tt] * iy, rhv)
+        rh = np.where(gB, rh + yFh[B] * iy, rh)
+        rhu = np.where(gB, rhu + yFT[B] * iy, rhu)
+        rhv = np.where(gB, rhv + yFN[B] * iy, rhv)
+        rhv = np.where(gB, rhv + ysR[B] * iy, rhv)
+    out = np.zeros_like(q)
+    out[0][W], out[1][W], out[2][W] = rh, rhu, rhv
+    if active is not None:
+        out *= (np.asarray(active) != 0)
+        out[out == 0] = 0   # keep +0 (masked cells are written as +0 in the kernel)
+    return out
+
+
+# ---------------------------------------------------------------------------
+# fused update tail (rhs_cuda.py:1757-1796) and the CFL wave speed
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Physics:
+    """Scalars of the step (Config production defaults, solver.py:629-860)."""
+    g: float = G
+    h_min: float = 1e-6
+    h_cfl: float = 1e-6          # CFL floor (h_min_cfl, floored at 1e-6 for float32)
+    cfl: float = 0.5
+    friction: bool = False       # Config.friction == 'manning_implicit'
+    vcap: float = 15.0           # friction_velocity_cap_ms
+    quadratic: bool = True       # friction_quadratic_alpha
+    keep_h: bool = True          # SWE_WETDRY_ZERO_H unset (solver.py:428)
+    cfl_l2: bool = False         # velocity norm of the CFL: L-inf (dense) or L2 (flat default)
+
+
+def update(q_int, r_int, dt: float, ph: Physics, n_cell=None, rain=None, active=None):
+    """One cell update from the residual, interior cells only (rhs_cuda.py:1757-1796).
+
+    ``q_int``/``r_int`` are (3, ...) arrays of the same dtype; ``n_cell`` Manning n per
+    cell (array or scalar; used when ``ph.friction``); ``rain`` m/s per cell or scalar
+    (already in the state dtype), added to ``rhs_h`` in the state precision; ``active``
+    marks cells whose residual and rain are kept (inactive cells update with a zero
+    residual, like the reference's dense carry kernel).
+    """
+    T = q_int.dtype.type
+    h_min = T(ph.h_min)
+    g = T(ph.g)
+    dtT = T(dt)
+    z = T(0)
+    rr0, r1, r2 = r_int[0], r_int[1], r_int[2]
+    if rain is not None:
+        rr0 = rr0 + (np.asarray(rain, dtype=q_int.dtype) if np.ndim(rain) else T(rain))
+    if active is not None:
+        act = np.asarray(active) != 0
+        rr0 = np.where(act, rr0, z)
+        r1 = np.where(act, r1, z)
+        r2 = np.where(act, r2, z)
+    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+        h = q_int[0] + dtT * rr0
+        hu = q_int[1] + dtT * r1
+        hv = q_int[2] + dtT * r2
+        dry = h < h_min
+        if ph.friction:
+            hs = np.where(h > h_min, h, h_min)
+            u = hu / hs
+            v = hv / hs
+            modU = np.sqrt(u * u + v * v)
+            n = (np.asarray(n_cell, dtype=q_int.dtype) if np.ndim(n_cell) else
+                 np.full(h.shape, T(n_cell)))
+            h43 = pow_m43(np.where(dry, T(1), hs))
+            if math.isfinite(ph.vcap):
+                n_cri = np.sqrt(T(1) / ((((T(1.0e-10) + dtT) * g) * h43) * (modU + T(1.0e-30))))
+                n = np.where((modU > T(ph.vcap)) & (n_cri > n), n_cri, n)
+            Cf = ((g * n) * n) * h43
+            if ph.quadratic:
+                two = ((T(2.0) * dtT) * Cf) * modU
+                alpha = T(2.0) / (np.sqrt(T(1.0) + T(2.0) * two) + T(1.0))
+            else:
+                alpha = T(1.0) / (T(1.0) + ((dtT * Cf) * modU))
+            hu = np.where(dry, hu, hu * alpha)
+            hv = np.where(dry, hv, hv * alpha)
+        if ph.keep_h:
+            h = np.where(dry, np.where(h > z, h, z), h)
+        else:
+            h = np.where(dry, z, h)
+        hu = np.where(dry, z, hu)
+        hv = np.where(dry, z, hv)
+    return np.stack([h, hu, hv])
+
+
+def lam_max(q_int, ph: Physics, skip=None) -> float:
+    """Max over cells of the CFL wave speed, as a Python float (solver.py:152-205 + NaN kept).
+
+    The speed is evaluated in the state dtype and maximised as float64 IEEE bits, the
+    order-independent reduction the device performs. Cells with ``h < h_cfl`` (or flagged
+    in ``skip``) are skipped; a NaN depth passes the test and yields a NaN lambda, which
+    dominates the bit maximum.
+    """
+    T = q_int.dtype.type
+    h, hu, hv = q_int[0], q_int[1], q_int[2]
+    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+        take = ~(h < T(ph.h_cfl))
+        if skip is not None:
+            take &= ~(np.asarray(skip) != 0)
+        u = hu / h
+        v = hv / h
+        if ph.cfl_l2:
+            vel = np.sqrt(u * u + v * v)
+        else:
+            au, av = np.abs(u), np.abs(v)
+            vel = np.where(au > av, au, av)
+        lam = vel + np.sqrt(T(ph.g) * h)
+    lam = lam[take].astype(np.float64)
+    if lam.size == 0:
+        return 0.0
+    if np.isnan(lam).any():
+        return math.nan
+    return float(lam.view(np.uint64).max().view(np.float64))
+
+
+def dt_from_lam(lam_max: float, ph: Physics, dmin: float) -> float:
+    """dt = cfl*min(dx,dy)/(max(lam, sqrt(g*h_cfl)) + 1e-12) in float64 (solver.py:2657-2666)."""
+    if not math.isfinite(lam_max):
+        raise FloatingPointError(f"non-finite max wave speed {lam_max}")
+    floor = (ph.g * ph.h_cfl) ** 0.5
+    if lam_max < floor:
+        lam_max = floor
+    return ph.cfl * dmin / (lam_max + 1.0e-12)
+
+
+def rain_step_cap(rmax: float, ph: Physics, dmin: float) -> float:
+    """The rain-film step bound (cfl*dmin)^(2/3)/(g*R)^(1/3) (solver.py:3005-3027); inf if dry."""
+    if rmax <= 0.0:
+        return math.inf
+    return (ph.cfl * dmin) ** (2.0 / 3.0) / (ph.g * rmax) ** (1.0 / 3.0)
+
+
+# ---------------------------------------------------------------------------
+# forcings (forcing.py:170-216, 315-377)
+# ---------------------------------------------------------------------------
+
+def rain_index(time_s, t: float) -> int:
+    """Piecewise-constant segment index: bisect_right(time_s, t) - 1, clamped (forcing.py:198-199)."""
+    import bisect
+    tl = list(np.asarray(time_s, dtype=np.float64).tolist())
+    idx = max(0, bisect.bisect_right(tl, float(t)) - 1)
+    return min(idx, len(tl) - 1)
+
+
+def stage_at_time(time_s, stage_m, t: float) -> float:
+    """Linear interpolation of the stage, clamped to the end values (forcing.py:315-347)."""
+    import bisect
+    tl = list(np.asarray(time_s, dtype=np.float64).tolist())
+    sl = list(np.asarray(stage_m, dtype=np.float64).tolist())
+    t = float(t)
+    if t <= tl[0]:
+        return sl[0]
+    if t >= tl[-1]:
+        return sl[-1]
+    i = bisect.bisect_left(tl, t)
+    t0, t1 = tl[i - 1], tl[i]
+    s0, s1 = sl[i - 1], sl[i]
+    w = (t - t0) / (t1 - t0)
+    return s0 + w * (s1 - s0)
+
+
+# ---------------------------------------------------------------------------
+# a dense run (Solver2D.run/step, solver.py:2695-3003)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class DenseCase:
+    """Inputs of a dense run on an unpadded ``(nx, ny)`` grid."""
+    h: np.ndarray
+    hu: np.ndarray
+    hv: np.ndarray
+    bed: np.ndarray
+    dx: float
+    dy: float
+    bc_x: str = "wall"
+    bc_y: str = "wall"
+    ngh: int = 4
+    manning: object = 0.0              # scalar or (nx, ny) n
+    rain: object = None                # None, scalar m/s, or (time_s, rate_mm_h[, frames]) tuple
+    active: Optional[np.ndarray] = None   # (nx, ny) bool inside mask
+    stage: Optional[dict] = None       # {"cells": (M,2) unpadded ij, "time_s", "stage_m"}
+
+
+class DenseOracle:
+    """Dense time stepping in the reference order: BC fill -> residual -> update -> t += dt
+    -> stage -> (next) CFL. Works in float32 or float64."""
+
+    def __init__(self, case: DenseCase, ph: Physics, dtype=np.float32):
+        self.case, self.ph = case, ph
+        self.dtype = np.dtype(dtype)
+        nx, ny = case.h.shape
+        ngh = case.ngh
+        self.nx, self.ny, self.ngh = nx, ny, ngh
+        self.q = np.zeros((3, nx + 2 * ngh, ny + 2 * ngh), dtype=self.dtype)
+        for k, a in enumerate((case.h, case.hu, case.hv)):
+            self.q[k, ngh:ngh + nx, ngh:ngh + ny] = np.asarray(a).astype(self.dtype)
+        self.b = pad_bed(np.asarray(case.bed), ngh, case.bc_x, case.bc_y, self.dtype)
+        self.act_pad = None
+        if case.active is not None:
+            self.act_pad = np.zeros(self.q.shape[1:], np.uint8)
+            self.act_pad[ngh:ngh + nx, ngh:ngh + ny] = np.asarray(case.active).astype(np.uint8)
+        self.t = 0.0
+        self.nsteps = 0
+        self.dmin = min(case.dx, case.dy)
+        self.dts: list = []
+
+    @property
+    def interior(self):
+        g = self.ngh
+        return self.q[:, g:g + self.nx, g:g + self.ny]
+
+    def rain_now(self):
+        """(rate array/scalar in the state dtype, rmax float64) at self.t, or (None, 0)."""
+        r = self.case.rain
+        if r is None:
+            return None, 0.0
+        if np.isscalar(r):
+            return (self.dtype.type(r), float(r)) if r > 0.0 else (None, 0.0)
+        time_s, rate = r[0], np.asarray(r[1])
+        idx = rain_index(time_s, self.t)
+        if rate.ndim == 1:
+            m = float(rate[idx]) / 3.6e6
+            return self.dtype.type(m), m
+        frame = (rate[idx] / 3.6e6).astype(self.dtype)
+        return frame, float(frame.max())
+
+    def cfl_dt(self) -> float:
+        return dt_from_lam(lam_max(self.interior, self.ph), self.ph, self.dmin)
+
+    def step(self, dt: float) -> None:
+        c, ph = self.case, self.ph
+        bc_fill(self.q, self.ngh, c.bc_x, c.bc_y)
+        r = residual(self.q, self.b, c.dx, c.dy, ph.g, ph.h_min, active=self.act_pad)
+        g = self.ngh
+        I = (slice(None), slice(g, g + self.nx), slice(g, g + self.ny))
+        rain, _ = self.rain_now()
+        n = c.manning
+        if np.ndim(n):
+            n = np.asarray(n, dtype=np.float32)
+        act = None if c.active is None else np.asarray(c.active)
+        self.q[I] = update(self.q[I], r[I], dt, ph, n_cell=n, rain=rain, active=act)
+        self.t += dt
+        if c.stage is not None:
+            eta = self.dtype.type(stage_at_time(c.stage["time_s"], c.stage["stage_m"], self.t))
+            ii = np.asarray(c.stage["cells"])[:, 0]
+            jj = np.asarray(c.stage["cells"])[:, 1]
+            bed = self.b[ii + g, jj + g]
+            self.q[0, ii + g, jj + g] = np.maximum(eta - bed, self.dtype.type(0))
+            self.q[1, ii + g, jj + g] = 0
+            self.q[2, ii + g, jj + g] = 0
+        self.nsteps += 1
+
+    def next_dt(self, t_end=None, dt_max=None) -> float:
+        """The step Solver2D.run takes next: CFL, rain cap, dt_max, clip to t_end."""
+        dt = self.cfl_dt()
+        _, rmax = self.rain_now()
+        dt = min(dt, rain_step_cap(rmax, self.ph, self.dmin))
+        if dt_max is not None and dt > dt_max:
+            dt = dt_max
+        if t_end is not None and self.t + dt > t_end:
+            dt = t_end - self.t
+        return dt
+
+    def run_steps(self, nsteps: int, dt_max=None) -> None:
+        for _ in range(nsteps):
+            dt = self.next_dt(dt_max=dt_max)
+            self.dts.append(dt)
+            self.step(dt)
+
+    def run(self, t_end: float, dt_max=None, max_steps=10 ** 7) -> int:
+        steps = 0
+        while self.t < t_end - 1.0e-12 and steps < max_steps:
+            dt = self.next_dt(t_end=t_end, dt_max=dt_max)
+            self.dts.append(dt)
+            self.step(dt)
+            steps += 1
+        return steps

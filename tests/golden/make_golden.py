@@ -1,0 +1,135 @@
+"""Write golden vectors from the reference's own NumPy path (run once, in the build container).
+
+Imports GeoSWE from /root/reference/src with GEOSWE_BACKEND=numpy and records, as small
+.npz fixtures next to this script:
+
+* rhs_srm40_{f64,f32}.npz  -- Solver2D._rhs at the rough, partially dry 40x40 state of the
+  reference's tests/test_srm_cpu_matches_gpu.py:44-57 (seed 7, dx=2, wall BCs, ngh=2).
+* rhs_bowl64_f64.npz        -- the same at the 64x64 bowl + dam state of
+  tests/test_gpu_compressed_equiv.py:37-44 (extrapolate BCs, ngh=4), plus one full step
+  with Manning friction and rain.
+* c1_traj_f64.npz           -- the 256x256 circular dam break (walls, no friction), h at
+  steps 1, 10 and 100 and the dt sequence, from Solver2D.step(cfl_dt()).
+* flatmesh_small.npz        -- CompressedMesh2D flat order and E/W/N/S neighbour table on a
+  small irregular mask after the ring-2 dilation of CompressedSWE.
+* forcing_samples.npz       -- RainfallForcing.rate_at_time / StageBoundary.stage_at_time.
+
+The reference cannot travel to the GPU box; only these data files do.
+Usage: python tests/golden/make_golden.py
+"""
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REF_SRC = Path("/root/reference/src")
+OUT = Path(__file__).resolve().parent
+
+
+def main():
+    os.environ["GEOSWE_BACKEND"] = "numpy"
+    sys.path.insert(0, str(REF_SRC))
+    from geoswe import Config, Mesh2D, Solver2D
+    from geoswe.forcing import RainfallForcing, StageBoundary
+    from geoswe.compressed_mesh import CompressedMesh2D
+
+    # 1. rough partially-dry 40x40 state (test_srm_cpu_matches_gpu.py:44-57)
+    nx = ny = 40
+    rng = np.random.default_rng(7)
+    bed = rng.random((nx, ny)) * 1.2
+    q0 = np.zeros((3, nx, ny))
+    q0[0] = np.maximum(0.8 - bed, 0.0)
+    wet = q0[0] > 1e-6
+    q0[1] = (rng.random((nx, ny)) - 0.5) * 0.04 * wet
+    q0[2] = (rng.random((nx, ny)) - 0.5) * 0.04 * wet
+    for dt, tag in (("float64", "f64"), ("float32", "f32")):
+        mesh = Mesh2D(nx=nx, ny=ny, dx=2.0, dy=2.0, ngh=2)
+        cfg = Config(bc_x="wall", bc_y="wall", dtype=dt, h_min=1e-6)
+        s = Solver2D(mesh, cfg, q0.astype(dt), bed.astype(dt))
+        s._apply_bc()
+        t0 = time.perf_counter()
+        rhs = s._rhs(s.q)
+        el = time.perf_counter() - t0
+        np.savez_compressed(OUT / f"rhs_srm40_{tag}.npz", q0=q0.astype(dt), bed=bed.astype(dt),
+                            q_pad=s.q, b_pad=s.b, rhs=rhs, ngh=2, dx=2.0, h_min=1e-6, seconds=el)
+
+    # 2. bowl + dam 64x64 (test_gpu_compressed_equiv.py:37-44), residual and one full step
+    NX = 64
+    ii, jj = np.meshgrid(np.arange(NX), np.arange(NX), indexing="ij")
+    coef = 5.0 / ((NX / 2.0) ** 2)
+    bedb = coef * ((ii - NX / 2.0) ** 2 + (jj - NX / 2.0) ** 2)
+    eta = np.where(ii < NX // 2, 1.2, 0.4)
+    hb = np.maximum(eta - bedb, 0.0)
+    qb = np.zeros((3, NX, NX))
+    qb[0] = hb
+    mesh = Mesh2D(nx=NX, ny=NX, dx=1.0, dy=1.0, ngh=4)
+    rain = 40.0 / 3.6e6
+    cfg = Config(bc_x="extrapolate", bc_y="extrapolate", dtype="float64", h_min=1e-6, cfl=0.4,
+                 friction="manning_implicit", manning_n=0.03, rainfall=rain)
+    s = Solver2D(mesh, cfg, qb, bedb)
+    s._apply_bc()
+    rhs = s._rhs(s.q)
+    dt = float(s.cfl_dt())
+    s.step(dt)
+    np.savez_compressed(OUT / "rhs_bowl64_f64.npz", q0=qb, bed=bedb, rhs=rhs, dt=dt, q1=s.q_interior.copy(),
+                        ngh=4, dx=1.0, h_min=1e-6, manning=0.03, rain=rain, cfl=0.4)
+
+    # 3. C1 circular dam break, float64 reference NumPy path, first 100 steps
+    N = 256
+    i = np.arange(N)[:, None]
+    j = np.arange(N)[None, :]
+    r = np.hypot(i - 127.5, j - 127.5)
+    h1 = np.where(r < 40, 10.0, 1.0)
+    q1 = np.zeros((3, N, N))
+    q1[0] = h1
+    mesh = Mesh2D(nx=N, ny=N, dx=1.0, dy=1.0, ngh=4)
+    cfg = Config(bc_x="wall", bc_y="wall", dtype="float64", h_min=1e-6, cfl=0.5)
+    s = Solver2D(mesh, cfg, q1, np.zeros((N, N)))
+    keep = {}
+    dts = []
+    t0 = time.perf_counter()
+    for k in range(1, 101):
+        dt = float(s.cfl_dt())
+        dts.append(dt)
+        s.step(dt)
+        if k in (1, 10, 100):
+            keep[f"h{k}"] = s.q_interior[0].copy()
+            keep[f"hu{k}"] = s.q_interior[1].copy()
+            keep[f"hv{k}"] = s.q_interior[2].copy()
+    el = time.perf_counter() - t0
+    np.savez_compressed(OUT / "c1_traj_f64.npz", dts=np.array(dts), seconds_100_steps=el, **keep)
+
+    # 4. flat order and neighbours on a small irregular mask (CompressedSWE ring=2)
+    from scipy.ndimage import binary_dilation
+    rng = np.random.default_rng(3)
+    m = np.zeros((20, 18), bool)
+    m[4:15, 3:12] = True
+    m[8:12, 12:15] = True
+    m[rng.random(m.shape) < 0.1] = False
+    m[6, 5] = False
+    stored = binary_dilation(m, iterations=2)
+    mesh = Mesh2D(nx=20, ny=18, dx=1.0, dy=1.0, ngh=4)
+    cm = CompressedMesh2D(mesh, stored)
+    np.savez_compressed(OUT / "flatmesh_small.npz", inside=m, stored=stored, ij_active=cm._ij_host,
+                        neighbors=np.asarray(cm.neighbors), ngh=4)
+
+    # 5. forcing lookups
+    rf = RainfallForcing(time_s=np.array([0.0, 300.0, 300.0, 900.0]), rate_mm_h=np.array([10.0, 50.0, 80.0, 0.0]))
+    ts = np.array([-5.0, 0.0, 1.0, 299.999, 300.0, 600.0, 900.0, 1e6])
+    rates = np.array([rf.rate_at_time(t) for t in ts])
+    sb = StageBoundary(cells=np.zeros((1, 2), int), time_s=np.array([0.0, 3600.0, 7200.0]),
+                       stage_m=np.array([0.0, 1.0, -0.5]), bed_b=np.zeros(1))
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ts2 = np.array([-10.0, 0.0, 1800.0, 3600.0, 5000.5, 7200.0, 9000.0])
+        stages = np.array([sb.stage_at_time(t) for t in ts2])
+    np.savez_compressed(OUT / "forcing_samples.npz", rain_t=rf.time_s, rain_mm_h=rf.rate_mm_h, t=ts, rates=rates,
+                        stage_t=sb.time_s, stage_m=sb.stage_m, t2=ts2, stages=stages)
+    print("golden vectors written to", OUT)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,112 @@
+"""The CPU oracle against golden vectors written by the reference's own NumPy path.
+
+Fixtures come from tests/golden/make_golden.py (reference GeoSWE, GEOSWE_BACKEND=numpy).
+Where the oracle follows the production kernel rather than the NumPy path (SM clamp,
+face velocity, both-dry faces, float64 source accumulator; see the oracle's header),
+differences are bounded by round-off and the tolerance is stated per comparison.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import swe_oracle as O
+
+G = Path(__file__).resolve().parent / "golden"
+W = (slice(None), slice(2, -2), slice(2, -2))     # the kernel window
+
+
+def test_residual_rough_partially_dry_f64():
+    """test_srm_cpu_matches_gpu.py:44-59 state: the reference bounds CPU vs kernel by 1e-12."""
+    z = np.load(G / "rhs_srm40_f64.npz")
+    r = O.residual(z["q_pad"], z["b_pad"], 2.0, 2.0, 9.81, 1e-6)
+    d = np.abs(r[W] - z["rhs"][W]).max()
+    assert d < 1e-12, d
+    assert np.abs(z["rhs"][W]).max() > 1e-2        # the state is not trivially at rest
+
+
+def test_residual_rough_partially_dry_f32():
+    """float32: the reference NumPy path mixes float64 temporaries; agreement to a few ulp."""
+    z = np.load(G / "rhs_srm40_f32.npz")
+    r = O.residual(z["q_pad"], z["b_pad"], 2.0, 2.0, 9.81, 1e-6)
+    assert r.dtype == np.float32
+    d = np.abs(r[W].astype(np.float64) - z["rhs"][W]).max()
+    assert d < 5e-7, d
+
+
+def test_bowl_residual_and_step_f64():
+    """Bowl + dam (test_gpu_compressed_equiv.py:37-44) with Manning and rain, one full step."""
+    z = np.load(G / "rhs_bowl64_f64.npz")
+    q0, bed, rain = z["q0"], z["bed"], float(z["rain"])
+    qp = np.zeros((3, 72, 72))
+    qp[:, 4:-4, 4:-4] = q0
+    O.bc_fill(qp, 4, "extrapolate", "extrapolate")
+    bp = O.pad_bed(bed, 4, "extrapolate", "extrapolate", np.float64)
+    r = O.residual(qp, bp, 1.0, 1.0, 9.81, 1e-6)
+    ref = z["rhs"].copy()
+    ref[0, 4:-4, 4:-4] -= rain                      # the reference's _rhs includes the rain
+    assert np.abs(r[W] - ref[W]).max() < 1e-12
+    case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=bed, dx=1.0, dy=1.0, bc_x="extrapolate",
+                       bc_y="extrapolate", manning=0.03, rain=rain)
+    ph = O.Physics(h_min=1e-6, h_cfl=1e-6, cfl=0.4, friction=True)
+    o = O.DenseOracle(case, ph, np.float64)
+    assert o.cfl_dt() == float(z["dt"])
+    o.step(float(z["dt"]))
+    # friction: the NumPy path's alpha = 2/(1+sqrt(1+4x)) vs the kernel's 2/(sqrt(1+2(2x))+1)
+    assert np.abs(o.interior - z["q1"]).max() < 1e-13
+
+
+def test_c1_dam_break_trajectory_f64():
+    """256^2 circular dam break (walls): h after 1, 10, 100 steps and every dt."""
+    z = np.load(G / "c1_traj_f64.npz")
+    N = 256
+    i = np.arange(N)[:, None]
+    j = np.arange(N)[None, :]
+    h = np.where(np.hypot(i - 127.5, j - 127.5) < 40, 10.0, 1.0)
+    case = O.DenseCase(h=h, hu=0 * h, hv=0 * h, bed=0 * h, dx=1.0, dy=1.0, bc_x="wall", bc_y="wall")
+    o = O.DenseOracle(case, O.Physics(), np.float64)
+    dts = []
+    for k in range(1, 101):
+        dt = o.next_dt()
+        dts.append(dt)
+        o.step(dt)
+        if k in (1, 10, 100):
+            for c, name in enumerate(("h", "hu", "hv")):
+                assert np.abs(o.interior[c] - z[f"{name}{k}"]).max() < 1e-12, (k, name)
+    assert np.abs(np.array(dts) - z["dts"]).max() < 1e-15
+
+
+def test_forcing_lookups():
+    from geoswe_amd.forcing import RainfallForcing, StageBoundary
+    z = np.load(G / "forcing_samples.npz")
+    rf = RainfallForcing(time_s=z["rain_t"], rate_mm_h=z["rain_mm_h"])
+    assert np.array_equal(np.array([rf.rate_at_time(t) for t in z["t"]]), z["rates"])
+    got = np.array([float(z["rain_mm_h"][O.rain_index(z["rain_t"], t)]) / 3.6e6 for t in z["t"]])
+    assert np.array_equal(got, z["rates"])
+    with pytest.warns(RuntimeWarning):
+        sb = StageBoundary(cells=np.zeros((1, 2), int), time_s=z["stage_t"], stage_m=z["stage_m"], bed_b=np.zeros(1))
+        s1 = np.array([sb.stage_at_time(t) for t in z["t2"]])
+    assert np.array_equal(s1, z["stages"])
+    s2 = np.array([O.stage_at_time(z["stage_t"], z["stage_m"], t) for t in z["t2"]])
+    assert np.array_equal(s2, z["stages"])
+
+
+def test_flat_mesh_order_and_neighbours():
+    """Flat order and E/W/N/S neighbours equal the reference CompressedMesh2D's."""
+    from geoswe_amd.flatmesh import build_flat_mesh
+    z = np.load(G / "flatmesh_small.npz")
+    g = int(z["ngh"])
+    inside = np.zeros((z["inside"].shape[0] + 2 * g, z["inside"].shape[1] + 2 * g), bool)
+    inside[g:-g, g:-g] = z["inside"]
+    fm = build_flat_mesh(inside, g, ring=2)
+    assert np.array_equal(fm.ij, z["ij_active"])
+    nb = z["neighbors"]
+    act = np.nonzero(fm.active)[0]
+    # every active cell's E/W neighbours from the run table, N/S are k+-1
+    for r in range(len(fm.run_k0)):
+        for t in range(fm.run_len[r]):
+            k = fm.run_k0[r] + t
+            assert nb[k, 0] == k + fm.run_d[r, 0] and nb[k, 1] == k + fm.run_d[r, 1]
+            assert nb[k, 2] == k + 1 and nb[k, 3] == k - 1
+            assert nb[nb[k, 0], 0] == k + fm.run_d[r, 2] and nb[nb[k, 1], 1] == k + fm.run_d[r, 3]
+    assert sum(int(x) for x in fm.run_len) == act.size == int(z["inside"].sum())
