@@ -18,7 +18,8 @@ LIB_PATH = _HERE / "_lib" / "libgeoswe_hip.so"
 
 GS_BC = {"extrapolate": 0, "periodic": 1, "wall": 2, "fall": 3}
 GS_F32, GS_F64 = 0, 1
-FLAG_CLASS_MASK, FLAG_INACTIVE, FLAG_STAGE = 0x3F, 0x40, 0x80
+FLAG_CLASS_MASK, FLAG_CFL_SKIP, FLAG_INACTIVE, FLAG_STAGE = 0x1F, 0x20, 0x40, 0x80
+MAX_CLASSES = 32
 ST_DONE, ST_NONFINITE = 1, 2
 
 
@@ -92,6 +93,12 @@ _SIGS = {
     "gs_flat_step": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, _P, _P]),
     "gs_flat_residual": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), C.c_int, _P, _P, _P, _P]),
     "gs_flat_cfl": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, _P, _P]),
+    "gs_fill": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_double, _P]),
+    "gs_gen_bed": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_uint32,
+                             C.c_int,
+                             C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                             _P]),
+    "gs_bench_copy": (C.c_int, [_P, _P, C.c_size_t, _P]),
 }
 
 _LIB = None

@@ -36,7 +36,7 @@ class CompressedSolver:
         self._bufs = [[L.DeviceArray.from_host(np.ascontiguousarray(qf[k])) for k in range(3)] for _ in range(2)]
         self._bed = L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(b_pad).astype(self.dtype)))
         self._flags = L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(flags_pad).astype(np.uint8)))
-        t = np.zeros(64, self.dtype)
+        t = np.zeros(L.MAX_CLASSES, self.dtype)
         t[:len(ntab)] = ntab
         self._ntab = L.DeviceArray.from_host(t)
         self._n_classes = max(1, len(ntab))

@@ -89,28 +89,36 @@ __device__ __forceinline__ void hllc(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g
 // the gate on the other cell's depth (rhs_cuda.py:1060-1073, 1101-1115). The reference
 // evaluates each face twice, once in each cell's frame; the two evaluations are equal
 // bit for bit (IEEE negation is exact), so one evaluation serves both cells.
+//
+// Elevations are taken relative to the lower cell's bed bL (a per-face shift of the
+// vertical datum): eta_L -> hL, eta_R -> hR + (bR - bL), z -> z - bL. The expressions
+// are the reference's term by term in exact arithmetic, but (z_f - b_c) and the face
+// depths no longer cancel against |b|: with beds of tens of metres one float32 ulp of b
+// is several micrometres, which the absolute-elevation form adds to every face depth
+// and to the bed-slope source.
 template <typename T>
-__device__ __forceinline__ void face(T bL, T bR, T gzL, T gzR, T etaL, T etaR, T hL, T hR,
-                                     T uNL, T uTL, T uNR, T uTR, T half_d, T g, T h_min,
-                                     Face<T>& f) {
+__device__ __forceinline__ void face(T bL, T bR, T gzL, T gzR, T hL, T hR, T uNL, T uTL, T uNR, T uTR,
+                                     T half_d, T g, T h_min, Face<T>& f) {
   const T z = T(0);
   f.gate = !(hL < h_min && hR < h_min);
-  const T zL = bL + half_d * gzL;
-  const T zR = bR - half_d * gzR;
-  const T zf0 = (bL > bR) ? bL : bR;
+  const T db = bR - bL;                   // bed of R in L's datum
+  const T zL = half_d * gzL;              // reconstructed face bed from L
+  const T zR = db - half_d * gzR;         // reconstructed face bed from R
+  const T zf0 = (db > z) ? db : z;        // max(bL, bR) - bL
   const T dzc = zR - zL;
-  const T dz = (bR - bL) - dzc;
-  const T detaL = vmax(z, vmin(dz, etaR - etaL));
-  const T detaR = vmax(z, vmin(-dz, etaL - etaR));
-  const T hLf = vmax(z, (etaL + detaL) - zf0);
+  const T dz = db - dzc;
+  const T etaR = hR + db;                 // eta_R - bL; eta_L - bL is hL
+  const T detaL = vmax(z, vmin(dz, etaR - hL));
+  const T detaR = vmax(z, vmin(-dz, hL - etaR));
+  const T hLf = vmax(z, (hL + detaL) - zf0);
   const T hRf = vmax(z, (etaR + detaR) - zf0);
   hllc<T>(hLf, uNL, uTL, hRf, uNR, uTR, g, h_min, f.Fh, f.FN, f.FT);
-  const T headL = zf0 - etaL;
+  const T headL = zf0 - hL;
   const T dzL = (hR < h_min) ? vmax(z, headL) : vmax(z, vmin(dzc, headL));
-  f.srcL = ((T(0.5) * g) * (hLf + hL)) * ((zf0 - dzL) - bL);
+  f.srcL = ((T(0.5) * g) * (hLf + hL)) * (zf0 - dzL);
   const T headR = zf0 - etaR;
   const T dzR = (hL < h_min) ? vmax(z, headR) : vmax(z, vmin(zL - zR, headR));
-  f.srcR = ((T(0.5) * g) * (hRf + hR)) * ((zf0 - dzR) - bR);
+  f.srcR = ((T(0.5) * g) * (hRf + hR)) * ((zf0 - dzR) - db);
 }
 
 // Primitive velocities with the dry gate (rhs_cuda.py:931-938).
@@ -122,20 +130,35 @@ __device__ __forceinline__ void prims(T h, T hu, T hv, T h_min, T& u, T& v) {
   v = wet ? hv / hs : T(0);
 }
 
-// h^(-4/3) from IEEE +,-,* only: Newton on y^-3 = h for y = h^(-1/3) from the
-// magic-constant seed (Moroz et al. 2021), three iterations, then (y*y)*(y*y).
-__device__ __forceinline__ float pow_m43(float h) {
-  uint32_t bits = __float_as_uint(h);
+// h^(-1/3) from IEEE +,-,* only: Newton on f(y) = y^-3 - h, y <- y*(4/3 - (h/3)*y^3),
+// three iterations from the magic-constant seed of Moroz et al. (2021),
+// bits(y0) = 0x54a2fa8c - bits(h)/3.
+__device__ __forceinline__ float icbrt(float h) {
+  const uint32_t bits = __float_as_uint(h);
   float y = __uint_as_float(0x54A2FA8Cu - bits / 3u);
   const float xt = h * (float)(1.0 / 3.0);
   const float c43 = (float)(4.0 / 3.0);
   y = y * (c43 - xt * ((y * y) * y));
   y = y * (c43 - xt * ((y * y) * y));
   y = y * (c43 - xt * ((y * y) * y));
+  return y;
+}
+// h^(-4/3) = (y*y)*(y*y); bit-reproducible on the CPU (numpy float32 ops).
+__device__ __forceinline__ float pow_m43(float h) {
+  const float y = icbrt(h);
   const float y2 = y * y;
   return y2 * y2;
 }
-__device__ __forceinline__ double pow_m43(double h) { return pow(h, -4.0 / 3.0); }
+// float64: the float32 root refined by two more Newton steps in float64.
+__device__ __forceinline__ double pow_m43(double h) {
+  double y = (double)icbrt((float)h);
+  const double xt = h * (1.0 / 3.0);
+  const double c43 = 4.0 / 3.0;
+  y = y * (c43 - xt * ((y * y) * y));
+  y = y * (c43 - xt * ((y * y) * y));
+  const double y2 = y * y;
+  return y2 * y2;
+}
 
 // Scalars of the cell update, converted once per launch.
 template <typename T>

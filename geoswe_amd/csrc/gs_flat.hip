@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
       eta = (T)c->eta64;
     }
   }
-  __shared__ T s_ntab[64];
+  __shared__ T s_ntab[32];
   if (MODE == 1 && threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
   __syncthreads();
 
@@ -93,14 +93,13 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
     const T bC = a.bed[kl];
     T uC, vC;
     prims(hC, huC, hvC, hm, uC, vC);
-    const T etaC = hC + bC;
     const T bl = fl_lower(bC), br = fl_upper(bC);
     const T gzy = (T(0.5) * (br - bl)) * iy;
     // upper y-face (k, k+1)
-    const T hR = fl_upper(hC), uR = fl_upper(uC), vR = fl_upper(vC), etaR = fl_upper(etaC);
+    const T hR = fl_upper(hC), uR = fl_upper(uC), vR = fl_upper(vC);
     const T gzyR = fl_upper(gzy);
     Face<T> fy;
-    face<T>(bC, br, gzy, gzyR, etaC, etaR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
+    face<T>(bC, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
     const T yFhB = fl_lower(fy.Fh), yFNB = fl_lower(fy.FN), yFTB = fl_lower(fy.FT);
     const T ysRB = fl_lower(fy.srcR);
     const int ygB = fl_lower((int)fy.gate);
@@ -121,13 +120,12 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
       T uE, vE, uW, vW;
       prims(hE, huE, hvE, hm, uE, vE);
       prims(hW, huW, hvW, hm, uW, vW);
-      const T etaE = hE + bE, etaW = hW + bW;
       const T gzxC = (T(0.5) * (bE - bW)) * ix;
       const T gzxE = (T(0.5) * (bEE - bC)) * ix;
       const T gzxW = (T(0.5) * (bC - bWW)) * ix;
       Face<T> fr, fl;
-      face<T>(bC, bE, gzxC, gzxE, etaC, etaE, hC, hE, uC, vC, uE, vE, hdx, g, hm, fr);
-      face<T>(bW, bC, gzxW, gzxC, etaW, etaC, hW, hC, uW, vW, uC, vC, hdx, g, hm, fl);
+      face<T>(bC, bE, gzxC, gzxE, hC, hE, uC, vC, uE, vE, hdx, g, hm, fr);
+      face<T>(bW, bC, gzxW, gzxC, hW, hC, uW, vW, uC, vC, hdx, g, hm, fl);
       T rh = T(0), rhu = T(0), rhv = T(0);
       if (fr.gate) {
         rh = rh - fr.Fh * ix;
@@ -181,7 +179,7 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
         qout[0][k] = h;
         qout[1][k] = hu;
         qout[2][k] = hv;
-        lam = lam_bits<T>(h, hu, hv, P);
+        if (!(fg & GS_FLAG_CFL_SKIP)) lam = lam_bits<T>(h, hu, hv, P);
       }
     }
   }
@@ -200,6 +198,7 @@ __global__ __launch_bounds__(kFBlock) void k_flat_cfl(FlatK<T> a) {
     const int len = a.run_len[r];
     for (int t = threadIdx.x & 63; t < len; t += 64) {
       const int64_t k = k0 + t;
+      if (a.flags && (a.flags[k] & GS_FLAG_CFL_SKIP)) continue;
       const uint64_t l = lam_bits<T>(q[0][k], q[1][k], q[2][k], a.P);
       lam = l > lam ? l : lam;
     }

@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
       eta = (T)c->eta64;
     }
   }
-  __shared__ T s_ntab[64];
+  __shared__ T s_ntab[32];
   if (MODE == 1 && threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
   __syncthreads();
 
@@ -188,15 +188,13 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
     T hP, huP, hvP, uP, vP;
     load_q(a, qin, i0 - 1, cl, hP, huP, hvP);
     prims(hP, huP, hvP, hm, uP, vP);
-    const T etaP = hP + bm1;
     const T gzxP = (T(0.5) * (b0 - bm2)) * ix;
     T hC, huC, hvC, uC, vC;
     load_q(a, qin, i0, cl, hC, huC, hvC);
     prims(hC, huC, hvC, hm, uC, vC);
-    T etaC = hC + b0;
     T gzxC = (T(0.5) * (bp1 - bm1)) * ix;
     Face<T> fx;
-    face<T>(bm1, b0, gzxP, gzxC, etaP, etaC, hP, hC, uP, vP, uC, vC, hdx, g, hm, fx);
+    face<T>(bm1, b0, gzxP, gzxC, hP, hC, uP, vP, uC, vC, hdx, g, hm, fx);
     T xFh = fx.Fh, xFN = fx.FN, xFT = fx.FT, xsR = fx.srcR;
     bool xg = fx.gate;
 
@@ -205,18 +203,17 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
       T hN, huN, hvN, uN, vN;
       load_q(a, qin, i + 1, cl, hN, huN, hvN);
       prims(hN, huN, hvN, hm, uN, vN);
-      const T etaN = hN + bp1;
       const T gzxN = (T(0.5) * (bp2 - b0)) * ix;
       Face<T> fn;
-      face<T>(b0, bp1, gzxC, gzxN, etaC, etaN, hC, hN, uC, vC, uN, vN, hdx, g, hm, fn);
+      face<T>(b0, bp1, gzxC, gzxN, hC, hN, uC, vC, uN, vN, hdx, g, hm, fn);
 
       // y faces of row i: lane computes its upper face (c, c+1)
       const T bl = from_lower(b0), br = from_upper(b0);
       const T gzy = (T(0.5) * (br - bl)) * iy;
       const T hR = from_upper(hC), uR = from_upper(uC), vR = from_upper(vC);
-      const T etaR = from_upper(etaC), gzyR = from_upper(gzy);
+      const T gzyR = from_upper(gzy);
       Face<T> fy;
-      face<T>(b0, br, gzy, gzyR, etaC, etaR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
+      face<T>(b0, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
       const T yFhB = from_lower(fy.Fh), yFNB = from_lower(fy.FN), yFTB = from_lower(fy.FT);
       const T ysRB = from_lower(fy.srcR);
       const int ygB = from_lower((int)fy.gate);
@@ -276,13 +273,15 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
           qout[0][idx] = h;
           qout[1][idx] = hu;
           qout[2][idx] = hv;
-          const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
-          lam = lb_ > lam ? lb_ : lam;
+          if (!(fl & GS_FLAG_CFL_SKIP)) {
+            const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
+            lam = lb_ > lam ? lb_ : lam;
+          }
         }
       }
 
       // slide the window one row down
-      hC = hN; huC = huN; hvC = hvN; uC = uN; vC = vN; etaC = etaN; gzxC = gzxN;
+      hC = hN; huC = huN; hvC = hvN; uC = uN; vC = vN; gzxC = gzxN;
       bm1 = b0; b0 = bp1; bp1 = bp2;
       xFh = fn.Fh; xFN = fn.FN; xFT = fn.FT; xsR = fn.srcR; xg = fn.gate;
     }
@@ -301,6 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_cfl(DenseK<T> a) {
        k += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = k / a.ny, j = k - i * a.ny;
     const int64_t idx = i * a.pitch + j;
+    if (a.flags && (a.flags[idx] & GS_FLAG_CFL_SKIP)) continue;
     const uint64_t l = lam_bits<T>(q[0][idx], q[1][idx], q[2][idx], a.P);
     lam = l > lam ? l : lam;
   }

@@ -1,0 +1,140 @@
+// gs_util.hip -- device set-up and measurement kernels: constant fill, the seeded
+// fractional-Brownian-motion bed generator used by the synthetic benchmark cases, and a
+// plain streaming copy used to calibrate HBM bandwidth and traffic counters.
+#include "gs_common.h"
+
+namespace gs {
+
+// Philox2x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy as
+// 1, 2, 3", SC'11): counter (c0, c1), key k; returns the first output word.
+__device__ __forceinline__ uint32_t philox2x32(uint32_t c0, uint32_t c1, uint32_t k) {
+  for (int r = 0; r < 10; ++r) {
+    const uint64_t p = (uint64_t)0xD256D193u * (uint64_t)c0;
+    const uint32_t hi = (uint32_t)(p >> 32), lo = (uint32_t)p;
+    c0 = hi ^ k ^ c1;
+    c1 = lo;
+    k += 0x9E3779B9u;
+  }
+  return c0;
+}
+
+// Lattice value in [-1, 1) of octave `oct` at lattice point (ix, iy).
+__device__ __forceinline__ double lattice(uint32_t seed, int oct, int64_t ix, int64_t iy) {
+  const uint32_t c0 = (uint32_t)ix;
+  const uint32_t c1 = (uint32_t)iy ^ ((uint32_t)oct << 24);
+  const uint32_t r = philox2x32(c0, c1, seed + 0x1000193u * (uint32_t)oct);
+  return (double)r * (2.0 / 4294967296.0) - 1.0;
+}
+
+// Value-noise fBm at cell (i, j): sum over octaves k of amp_k * bilinear-smoothstep
+// interpolation of lattice values with spacing L0 / 2^k cells, amp_k = 2^(-H k).
+// Every operation is an IEEE double rounding, mirrored by oracle/generators.py.
+__device__ double fbm_at(int64_t i, int64_t j, uint32_t seed, int octaves, double L0, double gain) {
+  double acc = 0.0, amp = 1.0, L = L0;
+  for (int k = 0; k < octaves; ++k) {
+    const double fx = ((double)i + 0.5) / L;
+    const double fy = ((double)j + 0.5) / L;
+    const double x0 = floor(fx), y0 = floor(fy);
+    const double tx = fx - x0, ty = fy - y0;
+    const double sx = (tx * tx) * (3.0 - 2.0 * tx);
+    const double sy = (ty * ty) * (3.0 - 2.0 * ty);
+    const int64_t ix = (int64_t)x0, iy = (int64_t)y0;
+    const double v00 = lattice(seed, k, ix, iy), v10 = lattice(seed, k, ix + 1, iy);
+    const double v01 = lattice(seed, k, ix, iy + 1), v11 = lattice(seed, k, ix + 1, iy + 1);
+    const double a = v00 + sx * (v10 - v00);
+    const double b = v01 + sx * (v11 - v01);
+    acc = acc + amp * (a + sy * (b - a));
+    amp = amp * gain;
+    L = L * 0.5;
+  }
+  return acc;
+}
+
+template <typename T>
+__global__ void k_gen_bed(T* out, int nx, int ny, int64_t pitch, int64_t i0, int64_t j0, int64_t stride,
+                          uint32_t seed, int octaves, double L0, double gain, double mu, double scale, double slope_x,
+                          double dx, double offset) {
+  const int64_t n = (int64_t)nx * ny;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = k / ny, j = k - (k / ny) * ny;
+    const int64_t gi = i0 + i * stride, gj = j0 + j * stride;
+    const double v = (fbm_at(gi, gj, seed, octaves, L0, gain) - mu) * scale;
+    const double b = slope_x * (((double)gi + 0.5) * dx) + v + offset;
+    out[i * pitch + j] = (T)b;
+  }
+}
+
+template <typename T>
+__global__ void k_fill(T* out, int64_t rows, int64_t cols, int64_t pitch, T value) {
+  const int64_t n = rows * cols;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = k / cols, j = k - i * cols;
+    out[i * pitch + j] = value;
+  }
+}
+
+// Streaming copy, 16 B per lane per iteration.
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += (int64_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+
+static unsigned grid_for(int64_t n, int64_t cap = 65536) {
+  int64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+extern "C" {
+
+int gs_fill(void* dst, int dtype, int64_t rows, int64_t cols, int64_t pitch, double value, void* stream) {
+  if (!dst || rows < 0 || cols < 0 || pitch < cols) return fail(GS_E_ARG, "gs_fill: bad arguments");
+  if (rows * cols == 0) return GS_OK;
+  const unsigned g = grid_for(rows * cols);
+  if (dtype == GS_F32)
+    hipLaunchKernelGGL(k_fill<float>, dim3(g), dim3(256), 0, (hipStream_t)stream, (float*)dst, rows, cols, pitch,
+                       (float)value);
+  else if (dtype == GS_F64)
+    hipLaunchKernelGGL(k_fill<double>, dim3(g), dim3(256), 0, (hipStream_t)stream, (double*)dst, rows, cols, pitch,
+                       value);
+  else
+    return fail(GS_E_DTYPE, "gs_fill: unsupported dtype %d", dtype);
+  GS_HIP(hipGetLastError(), "k_fill launch");
+  return GS_OK;
+}
+
+int gs_gen_bed(void* dst, int dtype, int nx, int ny, int64_t pitch, int64_t i0, int64_t j0, int64_t stride,
+               uint32_t seed, int octaves, double L0, double hurst, double mu, double scale, double slope_x,
+               double dx, double offset, void* stream) {
+  if (!dst || nx < 1 || ny < 1 || pitch < ny || stride < 1 || octaves < 1 || octaves > 30 || !(L0 > 0.0))
+    return fail(GS_E_ARG, "gs_gen_bed: bad arguments");
+  const double gain = pow(2.0, -hurst);
+  const unsigned g = grid_for((int64_t)nx * ny);
+  if (dtype == GS_F32)
+    hipLaunchKernelGGL(k_gen_bed<float>, dim3(g), dim3(256), 0, (hipStream_t)stream, (float*)dst, nx, ny, pitch, i0,
+                       j0, stride, seed, octaves, L0, gain, mu, scale, slope_x, dx, offset);
+  else if (dtype == GS_F64)
+    hipLaunchKernelGGL(k_gen_bed<double>, dim3(g), dim3(256), 0, (hipStream_t)stream, (double*)dst, nx, ny, pitch,
+                       i0, j0, stride, seed, octaves, L0, gain, mu, scale, slope_x, dx, offset);
+  else
+    return fail(GS_E_DTYPE, "gs_gen_bed: unsupported dtype %d", dtype);
+  GS_HIP(hipGetLastError(), "k_gen_bed launch");
+  return GS_OK;
+}
+
+int gs_bench_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!dst || !src || (bytes % 16) != 0 || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+    return fail(GS_E_ARG, "gs_bench_copy: 16-byte aligned pointers and a multiple of 16 bytes required");
+  const int64_t n16 = (int64_t)(bytes / 16);
+  hipLaunchKernelGGL(k_copy, dim3(grid_for(n16, 1 << 20)), dim3(256), 0, (hipStream_t)stream, (const uint4*)src,
+                     (uint4*)dst, n16);
+  GS_HIP(hipGetLastError(), "k_copy launch");
+  return GS_OK;
+}
+
+}  // extern "C"

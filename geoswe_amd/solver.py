@@ -134,21 +134,28 @@ class Solver2D:
     """Two-dimensional shallow-water solver on the dense padded grid (solver.py:1273)."""
 
     def __init__(self, mesh: Mesh2D, cfg: Config, q0: np.ndarray, b: np.ndarray, comm=None, dims=None,
-                 track_max_depth: bool = True, tile_rows: int = 0):
+                 track_max_depth: bool = True, tile_rows: int = 0, device_init=None):
+        """``q0`` (3, nx, ny) and ``b`` (nx, ny) host arrays, as the reference takes them.
+
+        ``device_init=(h0, bed_writer)`` instead builds a state at rest with uniform depth
+        ``h0`` and a bed written on the device by ``bed_writer(ptr, pitch, dtype_code)``;
+        ``q0`` and ``b`` are then None (used for grids too large to stage through the host).
+        """
         if comm is not None and getattr(comm, "size", 1) > 1:
             raise NotImplementedError("multi-GPU domain decomposition is not part of this build; one GPU "
                                       "runs the whole domain")
         cfg.require_production_scheme("Solver2D")
         self.mesh, self.cfg = mesh, cfg
-        q0 = np.asarray(q0)
-        b = np.asarray(b)
-        if tuple(q0.shape) != (3, mesh.nx, mesh.ny):
-            raise ValueError(f"q0 must have shape (3, nx, ny) = (3, {mesh.nx}, {mesh.ny}); got {tuple(q0.shape)}")
-        if tuple(b.shape) != (mesh.nx, mesh.ny):
-            raise ValueError(f"the bed must have shape (nx, ny) = ({mesh.nx}, {mesh.ny}); got {tuple(b.shape)}")
-        for what, arr in (("the bed", b), ("the initial state q0", q0)):
-            if not bool(np.isfinite(arr).all()):
-                raise ValueError(f"Solver2D: {what} holds non-finite values (NaN or inf)")
+        if device_init is None:
+            q0 = np.asarray(q0)
+            b = np.asarray(b)
+            if tuple(q0.shape) != (3, mesh.nx, mesh.ny):
+                raise ValueError(f"q0 must have shape (3, nx, ny) = (3, {mesh.nx}, {mesh.ny}); got {tuple(q0.shape)}")
+            if tuple(b.shape) != (mesh.nx, mesh.ny):
+                raise ValueError(f"the bed must have shape (nx, ny) = ({mesh.nx}, {mesh.ny}); got {tuple(b.shape)}")
+            for what, arr in (("the bed", b), ("the initial state q0", q0)):
+                if not bool(np.isfinite(arr).all()):
+                    raise ValueError(f"Solver2D: {what} holds non-finite values (NaN or inf)")
         if mesh.ngh < 2:
             raise ValueError(f"ngh={mesh.ngh} is too small: the SRM-HLLC stencil reads two ghost layers")
         if mesh.nx < 4 or mesh.ny < 4:
@@ -162,20 +169,34 @@ class Solver2D:
         self.pitch = _pitch(ny)
         self._plane = (nx, self.pitch)
         self.stream = None
-        self._bufs = [[L.DeviceArray(self._plane, self.dtype, zero=True) for _ in range(3)] for _ in range(2)]
-        host = np.zeros((3, nx, self.pitch), self.dtype)
-        host[:, :, :ny] = q0.astype(self.dtype, copy=False)
-        for k in range(3):
-            self._bufs[0][k].upload(host[k])
-            self._bufs[1][k].upload(host[k])
-        bh = np.zeros(self._plane, self.dtype)
-        bh[:, :ny] = b.astype(self.dtype, copy=False)
-        self._bed = L.DeviceArray.from_host(bh)
-        self._b_host = b.astype(self.dtype, copy=True)
+        self._bufs = [[L.DeviceArray(self._plane, self.dtype, zero=device_init is None) for _ in range(3)]
+                      for _ in range(2)]
+        dcode = L.GS_F32 if self.dtype == np.float32 else L.GS_F64
+        if device_init is None:
+            host = np.zeros((3, nx, self.pitch), self.dtype)
+            host[:, :, :ny] = q0.astype(self.dtype, copy=False)
+            for k in range(3):
+                self._bufs[0][k].upload(host[k])
+                self._bufs[1][k].upload(host[k])
+            del host
+            bh = np.zeros(self._plane, self.dtype)
+            bh[:, :ny] = b.astype(self.dtype, copy=False)
+            self._bed = L.DeviceArray.from_host(bh)
+            self._b_host = b.astype(self.dtype, copy=True)
+        else:
+            h0, writer = device_init
+            for s_ in range(2):
+                for k, v in enumerate((float(h0), 0.0, 0.0)):
+                    L.call("gs_fill", C.c_void_p(self._bufs[s_][k].ptr), dcode, nx, self.pitch, self.pitch,
+                           C.c_double(v), None)
+            self._bed = L.DeviceArray(self._plane, self.dtype, zero=True)
+            writer(self._bed.ptr, self.pitch, dcode)
+            L.sync()
+            self._b_host = None
         self._flags_host = None          # (nx, pitch) uint8
         self._flags = None
         self._n_field = None
-        self._ntab = L.DeviceArray((64,), self.dtype, zero=True)
+        self._ntab = L.DeviceArray((L.MAX_CLASSES,), self.dtype, zero=True)
         self._n_classes = 1
         self._manning_uniform = float(cfg.manning_n)
         self.inside_mask = None
@@ -232,7 +253,7 @@ class Solver2D:
         self._flags = L.DeviceArray.from_host(self._flags_host)
 
     def _upload_ntab(self, table):
-        t = np.zeros(64, self.dtype)
+        t = np.zeros(L.MAX_CLASSES, self.dtype)
         t[:len(table)] = np.asarray(table, dtype=np.float64).astype(self.dtype)
         self._ntab.upload(t)
         self._n_classes = max(1, len(table))
@@ -273,6 +294,23 @@ class Solver2D:
         fl |= np.where(self.inside_mask, 0, L.FLAG_INACTIVE).astype(np.uint8)
         self._push_flags()
 
+    def set_cfl_ghost_mask(self, mask):
+        """Leave interior cells out of the CFL wave-speed maximum (solver.py:1611-1633).
+
+        ``mask`` is an unpadded (nx, ny) bool array (True = skipped), or None to clear.
+        """
+        self._ensure_flags()
+        ny = self.mesh.ny
+        fl = self._flags_host[:, :ny]
+        fl &= np.uint8(~L.FLAG_CFL_SKIP & 0xFF)
+        if mask is not None:
+            mask = np.asarray(mask)
+            if mask.shape != (self.mesh.nx, ny):
+                raise ValueError(f"cfl_ghost_mask must have shape ({self.mesh.nx}, {ny}); got {mask.shape}")
+            fl |= np.where(mask.astype(bool), L.FLAG_CFL_SKIP, 0).astype(np.uint8)
+        self._push_flags()
+        self._lam_valid = False
+
     def set_manning_table(self, cls_padded, table):
         """Manning n as a uint8 class per padded cell plus a table (solver.py:1524-1556)."""
         g, nx, ny = self.mesh.ngh, self.mesh.nx, self.mesh.ny
@@ -287,11 +325,14 @@ class Solver2D:
         if int(cls_padded.max()) >= int(table.size):
             raise ValueError(f"set_manning_table: class id {int(cls_padded.max())} out of range for a "
                              f"{int(table.size)}-entry table (OOB device read)")
-        if table.size > 64:
-            raise ValueError("set_manning_table: at most 64 classes fit the fused step's class byte")
         if self.cfg.friction is None:
             warnings.warn("set_manning_table: friction is off (Config.friction is None), so this roughness "
                           "table will not be applied", stacklevel=2)
+        if table.size > L.MAX_CLASSES:
+            # more classes than the flags byte holds: store n = table[cls] per cell instead
+            self._set_n_plane(table[cls_padded[g:g + nx, g:g + ny]].astype(np.float64))
+            self._manning_table = table
+            return
         self._ensure_flags()
         fl = self._flags_host[:, :ny]
         fl &= np.uint8(~L.FLAG_CLASS_MASK & 0xFF)
@@ -314,7 +355,7 @@ class Solver2D:
         if self.cfg.friction is None:
             self.cfg.friction = "manning_implicit"
         vals, inv = np.unique(nh, return_inverse=True)
-        if vals.size <= 64 and self.dtype == np.float32:
+        if vals.size <= L.MAX_CLASSES and self.dtype == np.float32:
             self._ensure_flags()
             fl = self._flags_host[:, :ny]
             fl &= np.uint8(~L.FLAG_CLASS_MASK & 0xFF)
@@ -362,10 +403,12 @@ class Solver2D:
 
     @property
     def b(self) -> np.ndarray:
-        return host_pad_bed(self._b_host, self.mesh.ngh, self.cfg.bc_x, self.cfg.bc_y)
+        bh = self._b_host if self._b_host is not None else self._bed.download()[:, :self.mesh.ny]
+        return host_pad_bed(bh, self.mesh.ngh, self.cfg.bc_x, self.cfg.bc_y)
 
     def depth(self) -> np.ndarray:
-        return self.q_interior[0]
+        """Water depth (nx, ny) as a host array."""
+        return self._bufs[self._cur()][0].download()[:, :self.mesh.ny]
 
     def max_depth(self) -> np.ndarray:
         if self._max_h is None:

@@ -49,7 +49,8 @@ extern "C" {
 #define GS_F64 1
 
 /* per-cell flags byte (dense "flags" plane, flat "flags" array) */
-#define GS_FLAG_CLASS_MASK 0x3F  /* Manning class id (0..63) into the ntab array */
+#define GS_FLAG_CLASS_MASK 0x1F  /* Manning class id (0..31) into the ntab array */
+#define GS_FLAG_CFL_SKIP 0x20    /* left out of the CFL wave-speed maximum (set_cfl_ghost_mask) */
 #define GS_FLAG_INACTIVE 0x40    /* outside the inside mask: zero residual, no rain */
 #define GS_FLAG_STAGE 0x80       /* stage-boundary cell: h = max(eta(t)-b, 0), hu = hv = 0 */
 
@@ -96,7 +97,7 @@ typedef struct GsPhys {
   int32_t quadratic;   /* 1 quadratic-alpha friction root, 0 linearised */
   int32_t keep_h;      /* 1 keep the sub-threshold depth on dry cells, 0 zero it */
   int32_t cfl_l2;      /* CFL velocity norm: 0 max(|u|,|v|), 1 sqrt(u^2+v^2) */
-  int32_t n_classes;   /* entries of the Manning table (1..64) */
+  int32_t n_classes;   /* entries of the Manning table (1..32) */
   int32_t pad_;
 } GsPhys;
 
@@ -237,6 +238,24 @@ int gs_flat_residual(const GsFlat* f, const GsPhys* phys, int which, void* rhs0,
 /* Replaces: cfl_lammax_flat_blk + CompressedStepper.cfl_dt (compressed_solver.py:200-240, 721-748). */
 int gs_flat_cfl(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev,
                 void* stream);
+
+/* ---- set-up and measurement utilities ---- */
+/* Constant fill of a (rows, cols) plane with row pitch `pitch` (elements). No reference
+ * counterpart beyond the host-side np.full/np.zeros of the solver constructors. */
+int gs_fill(void* dst, int dtype, int64_t rows, int64_t cols, int64_t pitch, double value, void* stream);
+/*
+ * Seeded synthetic bed b(i, j) = slope_x*x_I + scale*(fbm(I, J) - mu) + offset on a (nx, ny)
+ * plane, I = i0 + i*stride, J = j0 + j*stride: value-noise fBm over Philox2x32-10 lattice
+ * values, `octaves` octaves of spacing L0/2^k cells and gain 2^-hurst, x_I = (I+0.5)*dx. Replaces the host-side
+ * synthetic-case set-up of the reference's scaling harness
+ * (benchmark/scaling_640m/scaling_bench.py:59-83); oracle/generators.py mirrors it bit for bit.
+ */
+int gs_gen_bed(void* dst, int dtype, int nx, int ny, int64_t pitch, int64_t i0, int64_t j0, int64_t stride,
+               uint32_t seed, int octaves, double L0, double hurst, double mu, double scale, double slope_x,
+               double dx, double offset, void* stream);
+/* Plain streaming copy kernel (16 B per lane), used to calibrate HBM bandwidth and the
+ * rocprofv3 FETCH_SIZE/WRITE_SIZE counters. No reference counterpart. */
+int gs_bench_copy(void* dst, const void* src, size_t bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -24,6 +24,11 @@ Sources followed (paths relative to the reference checkout):
 
 Documented departures (shared with the HIP kernels, which implement the same rule):
 
+* Each face is evaluated with elevations relative to its lower cell's bed (a per-face
+  shift of the vertical datum). The expressions are the reference's in exact arithmetic;
+  in float32 the face depths and the bed-slope source no longer lose |b|-sized ulps (on a
+  77 m slope one ulp of b is 7.6e-6 m, a 0.25 % error in the source of a 3 m cell).
+
 * ``h ** (-4/3)`` in the friction coefficient uses ``pow_m43`` (an inverse-cube-root
   Newton iteration built from IEEE +,-,* only) instead of ``powf(h, -4/3)``, so CPU and
   GPU results are reproducible bit for bit. Its error against the exact power is below
@@ -113,26 +118,41 @@ def kernel_consts(dx: float, dy: float, g: float, h_min: float, dtype):
                 g=T(g), h_min=T(h_min))
 
 
-def pow_m43(h: np.ndarray) -> np.ndarray:
-    """``h ** (-4/3)`` for float32 ``h > 0`` from IEEE +,-,* only (bit-reproducible).
+def icbrt_f32(h: np.ndarray) -> np.ndarray:
+    """``h ** (-1/3)`` in float32 from IEEE +,-,* only (bit-reproducible on the GPU).
 
-    Inverse cube root y = h^(-1/3) by Newton's method on f(y) = y^-3 - h,
-    ``y <- y * (4/3 - (h/3) * y^3)``, three iterations from the exponent-halving seed
-    ``bits(y0) = 0x54a2fa8c - bits(h)/3`` (the magic-constant seed of Moroz et al.,
-    "Fast calculation of cube and inverse cube roots using a magic constant", 2021),
-    then ``h^(-4/3) = (y*y)*(y*y)``. Float64 input returns the exact power (FP64 oracle).
-    Pinned by tests/test_oracle_pow43.py against float64 ``h**(-4/3)`` (< 13 ulp).
+    Newton's method on f(y) = y^-3 - h, ``y <- y * (4/3 - (h/3) * y^3)``, three iterations
+    from the magic-constant seed ``bits(y0) = 0x54a2fa8c - bits(h)/3`` (Moroz et al.,
+    "Fast calculation of cube and inverse cube roots using a magic constant", 2021).
     """
-    h = np.asarray(h)
-    if h.dtype == np.float64:
-        return h ** (-4.0 / 3.0)
     T = np.float32
-    bits = h.astype(T, copy=False).view(np.uint32)
+    h = np.asarray(h).astype(T, copy=False)
+    bits = h.view(np.uint32)
     y = (np.uint32(0x54A2FA8C) - bits // np.uint32(3)).view(T)
     xt = h * T(1.0 / 3.0)
     c43 = T(4.0 / 3.0)
     for _ in range(3):
         y = y * (c43 - xt * ((y * y) * y))
+    return y
+
+
+def pow_m43(h: np.ndarray) -> np.ndarray:
+    """``h ** (-4/3)`` for ``h > 0`` from IEEE +,-,* only, so CPU and GPU agree bit for bit.
+
+    float32: ``y = icbrt_f32(h)``, then ``(y*y)*(y*y)``. float64: the float32 root refined
+    by two more Newton steps in float64, then ``(y*y)*(y*y)``.
+    Pinned by tests/test_oracle_invariants.py against ``h**(-4/3)`` (float32 < 13 ulp,
+    float64 <= 16 ulp).
+    """
+    h = np.asarray(h)
+    if h.dtype == np.float64:
+        y = icbrt_f32(h.astype(np.float32)).astype(np.float64)
+        xt = h * (1.0 / 3.0)
+        for _ in range(2):
+            y = y * (4.0 / 3.0 - xt * ((y * y) * y))
+        y2 = y * y
+        return y2 * y2
+    y = icbrt_f32(h)
     y2 = y * y
     return y2 * y2
 
@@ -197,37 +217,41 @@ def hllc(hL, uNL, uTL, hR, uNR, uTR, g, h_min):
     return Fh, FN, FT
 
 
-def srm_face(bL, bR, gzL, gzR, etaL, etaR, hL, hR, half_d, g, h_min):
+def srm_face(bL, bR, gzL, gzR, hL, hR, half_d, g, h_min):
     """SRM face depths and the two one-sided bed sources of one face (rhs_cuda.py:1042-1115).
 
     Returns (h_Lf, h_Rf, gate, srcL, srcR): ``srcL`` is the lower-index cell's source in
     its own frame (it subtracts ``srcL*inv_d``), ``srcR`` the upper cell's (it adds
-    ``srcR*inv_d``), each with the dry gate on the *other* cell's depth. The face values
-    are the same bits in either cell's frame (IEEE negation is exact), so computing a
-    face once is the reference's per-cell evaluation.
-    Pinned by lake-at-rest (tests/test_oracle_invariants.py) and the reference NumPy
-    residual golden vectors.
+    ``srcR*inv_d``), each with the dry gate on the *other* cell's depth.
+
+    Elevations are measured from the lower cell's bed bL (eta_L - bL = hL,
+    eta_R - bL = hR + (bR - bL), z - bL): the reference's expressions in exact arithmetic,
+    without the cancellation of (z_f - b_c) against float32 ulps of |b|.
+    Pinned by lake-at-rest (tests/test_oracle_invariants.py) and by the reference NumPy
+    residual golden vectors (tests/test_oracle_golden.py).
     """
     T = type(g)
     z = T(0)
-    zL = bL + half_d * gzL
-    zR = bR - half_d * gzR
-    zf0 = np.where(bL > bR, bL, bR)
+    db = bR - bL
+    zL = half_d * gzL
+    zR = db - half_d * gzR
+    zf0 = np.where(db > z, db, z)
     dzc = zR - zL
-    dz = (bR - bL) - dzc
-    detaL = np.maximum(z, np.minimum(dz, etaR - etaL))
-    detaR = np.maximum(z, np.minimum(-dz, etaL - etaR))
-    hLf = np.maximum(z, (etaL + detaL) - zf0)
+    dz = db - dzc
+    etaR = hR + db
+    detaL = np.maximum(z, np.minimum(dz, etaR - hL))
+    detaR = np.maximum(z, np.minimum(-dz, hL - etaR))
+    hLf = np.maximum(z, (hL + detaL) - zf0)
     hRf = np.maximum(z, (etaR + detaR) - zf0)
     gate = ~((hL < h_min) & (hR < h_min))
     # owner L: gate on the neighbour R being dry
-    headL = zf0 - etaL
+    headL = zf0 - hL
     dzL = np.where(hR < h_min, np.maximum(z, headL), np.maximum(z, np.minimum(dzc, headL)))
-    srcL = ((T(0.5) * g) * (hLf + hL)) * ((zf0 - dzL) - bL)
+    srcL = ((T(0.5) * g) * (hLf + hL)) * (zf0 - dzL)
     # owner R (its left face): dz_clip in R's frame is zL - zR
     headR = zf0 - etaR
     dzR = np.where(hL < h_min, np.maximum(z, headR), np.maximum(z, np.minimum(zL - zR, headR)))
-    srcR = ((T(0.5) * g) * (hRf + hR)) * ((zf0 - dzR) - bR)
+    srcR = ((T(0.5) * g) * (hRf + hR)) * ((zf0 - dzR) - db)
     return hLf, hRf, gate, srcL, srcR
 
 
@@ -263,7 +287,6 @@ def residual(q: np.ndarray, b: np.ndarray, dx: float, dy: float, g: float = G,
     h, hu, hv = q[0], q[1], q[2]
     nxp, nyp = h.shape
     u, v = primitives(h, hu, hv, hm)
-    eta = h + b
     gzx = np.zeros_like(b)
     gzy = np.zeros_like(b)
     gzx[1:-1, :] = (T(0.5) * (b[2:, :] - b[:-2, :])) * ix
@@ -271,12 +294,11 @@ def residual(q: np.ndarray, b: np.ndarray, dx: float, dy: float, g: float = G,
     hdx = T(0.5) * c["dxk"]
     hdy = T(0.5) * c["dyk"]
     # x faces between rows i and i+1 (shape nxp-1, nyp)
-    xhL, xhR, xg, xsL, xsR = srm_face(b[:-1], b[1:], gzx[:-1], gzx[1:], eta[:-1], eta[1:],
-                                      h[:-1], h[1:], hdx, gk, hm)
+    xhL, xhR, xg, xsL, xsR = srm_face(b[:-1], b[1:], gzx[:-1], gzx[1:], h[:-1], h[1:], hdx, gk, hm)
     xFh, xFN, xFT = hllc(xhL, u[:-1], v[:-1], xhR, u[1:], v[1:], gk, hm)
     # y faces between columns j and j+1 (shape nxp, nyp-1): normal v, transverse u
-    yhL, yhR, yg, ysL, ysR = srm_face(b[:, :-1], b[:, 1:], gzy[:, :-1], gzy[:, 1:], eta[:, :-1],
-                                      eta[:, 1:], h[:, :-1], h[:, 1:], hdy, gk, hm)
+    yhL, yhR, yg, ysL, ysR = srm_face(b[:, :-1], b[:, 1:], gzy[:, :-1], gzy[:, 1:], h[:, :-1], h[:, 1:],
+                                      hdy, gk, hm)
     yFh, yFN, yFT = hllc(yhL, v[:, :-1], u[:, :-1], yhR, v[:, 1:], u[:, 1:], gk, hm)
 
     W = (slice(2, nxp - 2), slice(2, nyp - 2))          # the kernel window
@@ -529,8 +551,10 @@ class DenseOracle:
         frame = (rate[idx] / 3.6e6).astype(self.dtype)
         return frame, float(frame.max())
 
+    cfl_skip = None       # (nx, ny) bool: cells left out of the CFL (set_cfl_ghost_mask)
+
     def cfl_dt(self) -> float:
-        return dt_from_lam(lam_max(self.interior, self.ph), self.ph, self.dmin)
+        return dt_from_lam(lam_max(self.interior, self.ph, skip=self.cfl_skip), self.ph, self.dmin)
 
     def step(self, dt: float) -> None:
         c, ph = self.case, self.ph

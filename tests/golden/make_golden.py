@@ -8,6 +8,8 @@ Imports GeoSWE from /root/reference/src with GEOSWE_BACKEND=numpy and records, a
 * rhs_bowl64_f64.npz        -- the same at the 64x64 bowl + dam state of
   tests/test_gpu_compressed_equiv.py:37-44 (extrapolate BCs, ngh=4), plus one full step
   with Manning friction and rain.
+* ritter_srm_f64.npz        -- dry-bed dam break (800 x 4 strip, dx 0.025, h_min 1e-14) run to
+  t = 1 s by Solver2D.run with the production scheme.
 * c1_traj_f64.npz           -- the 256x256 circular dam break (walls, no friction), h at
   steps 1, 10 and 100 and the dt sequence, from Solver2D.step(cfl_dt()).
 * flatmesh_small.npz        -- CompressedMesh2D flat order and E/W/N/S neighbour table on a
@@ -100,6 +102,19 @@ def main():
             keep[f"hv{k}"] = s.q_interior[2].copy()
     el = time.perf_counter() - t0
     np.savez_compressed(OUT / "c1_traj_f64.npz", dts=np.array(dts), seconds_100_steps=el, **keep)
+
+    # 3b. dry-bed dam break (Ritter) through the production SRM-HLLC scheme, float64
+    nx, dx = 800, 0.025
+    hR = np.zeros((nx, 4))
+    hR[: nx // 2] = 1.0
+    qR = np.zeros((3, nx, 4))
+    qR[0] = hR
+    mesh = Mesh2D(nx=nx, ny=4, dx=dx, dy=dx, ngh=4)
+    cfg = Config(bc_x="extrapolate", bc_y="wall", dtype="float64", h_min=1e-14, cfl=0.4)
+    s = Solver2D(mesh, cfg, qR, np.zeros((nx, 4)))
+    n = s.run(t_end=1.0)
+    np.savez_compressed(OUT / "ritter_srm_f64.npz", h=s.q_interior[0].copy(), hu=s.q_interior[1].copy(),
+                        steps=n, t=s.t, dx=dx)
 
     # 4. flat order and neighbours on a small irregular mask (CompressedSWE ring=2)
     from scipy.ndimage import binary_dilation

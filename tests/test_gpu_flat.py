@@ -61,12 +61,16 @@ def test_flat_run_matches_dense(gpu, masked):
     from geoswe_amd import CompressedSolver
     q0, bed = rough_state(64, 80, seed=17, depth=0.7, rough=0.9, vel=0.3)
     mk = lambda: dense(q0, bed, friction="manning", manning_n=0.03, rainfall=60.0 / 3.6e6)
+    if masked:
+        # inactive cells at rest: the dense carry leaves them unchanged, as the flat ring
+        m = irregular_mask(64, 80)
+        q0 = q0 * m
     s = mk()
     c_src = mk()
     if masked:
-        m = irregular_mask(64, 80)
-        s.set_inside_mask(m)
-        c_src.set_inside_mask(m)
+        for x in (s, c_src):
+            x.set_inside_mask(m)
+            x.set_cfl_ghost_mask(~m)      # the dense CFL otherwise includes inactive cells
     c = CompressedSolver.from_dense(c_src, cfl_linf=True)
     nd = s.run(4.0)
     nc = c.run(4.0)
@@ -93,13 +97,15 @@ def test_flat_classes_gridded_rain_stage(gpu):
     mesh = Mesh2D(nx, ny, 2.0, 2.0)
     sb = StageBoundary.from_mask(sea, mesh, bed, np.array([0.0, 5.0]), np.array([0.6, 1.1]))
     mk = lambda: dense(q0, bed, friction="manning", rainfall_forcing=RainfallForcing(np.array([0.0, 1.0]), frames),
-                       stage_boundary=sb)
+                       stage_boundary=sb)   # reads q0 when called
     n = np.choose(rng.integers(0, 8, (nx, ny)), np.linspace(0.02, 0.12, 8))
-    s = mk()
-    s.set_manning(n)
     m = irregular_mask(nx, ny, seed=8)
     m[0, :] = True
+    q0 = q0 * m                      # inactive cells at rest
+    s = mk()
+    s.set_manning(n)
     s.set_inside_mask(m)
+    s.set_cfl_ghost_mask(~m)
     c = CompressedSolver.from_dense(s, cfl_linf=True)
     s.run(2.5)
     c.run(2.5)
@@ -114,5 +120,6 @@ def test_flat_classes_gridded_rain_stage(gpu):
                        stage={"cells": np.argwhere(sea), "time_s": np.array([0.0, 5.0]),
                               "stage_m": np.array([0.6, 1.1])})
     o = O.DenseOracle(case, O.Physics(friction=True), np.float32)
+    o.cfl_skip = ~m
     o.run(2.5)
     assert np.array_equal(s.q_interior, o.interior)
