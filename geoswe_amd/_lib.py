@@ -97,8 +97,9 @@ _SIGS = {
     "gs_gen_bed": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_uint32,
                              C.c_int,
                              C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
-                             _P]),
+                             C.c_int, _P]),
     "gs_bench_copy": (C.c_int, [_P, _P, C.c_size_t, _P]),
+    "gs_gen_voronoi": (C.c_int, [_P, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, _P, _P, _P, C.c_int, _P]),
 }
 
 _LIB = None

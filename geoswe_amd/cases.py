@@ -39,16 +39,17 @@ class FbmBed:
         buf = L.DeviceArray((sx, sy), np.float64)
         L.call("gs_gen_bed", C.c_void_p(buf.ptr), L.GS_F64, sx, sy, sy, 0, 0, stride, C.c_uint32(self.seed),
                self.octaves, C.c_double(self.L0), C.c_double(self.hurst), C.c_double(0.0), C.c_double(1.0),
-               C.c_double(0.0), C.c_double(self.dx), C.c_double(0.0), None)
+               C.c_double(0.0), C.c_double(self.dx), C.c_double(0.0), 0, None)
         raw = buf.download()
         self.mu = float(raw.mean())
         self.scale = float(self.amplitude / np.abs(raw - self.mu).max())
         return self
 
-    def write(self, ptr: int, pitch: int, dcode: int, nx: int, ny: int, i0: int = 0, j0: int = 0) -> None:
+    def write(self, ptr: int, pitch: int, dcode: int, nx: int, ny: int, i0: int = 0, j0: int = 0,
+              round_f32: bool = False) -> None:
         L.call("gs_gen_bed", C.c_void_p(ptr), dcode, nx, ny, pitch, i0, j0, 1, C.c_uint32(self.seed), self.octaves,
                C.c_double(self.L0), C.c_double(self.hurst), C.c_double(self.mu), C.c_double(self.scale),
-               C.c_double(self.slope_x), C.c_double(self.dx), C.c_double(self.offset), None)
+               C.c_double(self.slope_x), C.c_double(self.dx), C.c_double(self.offset), int(round_f32), None)
 
     def host_crop(self, i0, j0, nx, ny, dtype=np.float32) -> np.ndarray:
         """The same field on the host, through the device generator (small crops)."""
@@ -69,14 +70,136 @@ def dam_break_c1(n: int = 256, dtype: str = "float32"):
 
 
 def rain_on_slope_c2(nx: int = 25600, ny: int = 25000, dtype: str = "float32", track_max_depth: bool = False,
-                     tile_rows: int = 0, bed: FbmBed = None):
-    """Solver2D of the rain-on-slope case, state and bed built on the device."""
+                     tile_rows: int = 0, bed: FbmBed = None, f32_inputs: bool = False):
+    """Solver2D of the rain-on-slope case, state and bed built on the device.
+
+    ``f32_inputs`` rounds every input (bed, h0, n, rain rate) to float32 first and takes the
+    float32 run's wet/dry thresholds (h_min = h_min_cfl = 1e-6 instead of the float64
+    default 1e-10): a float64 run on exactly the inputs and parameters of the float32 run,
+    which isolates the arithmetic error."""
     from .solver import Solver2D
     bed = bed or FbmBed(seed=C2_SEED).normalise(nx, ny)
+    r = (lambda x: float(np.float32(x))) if f32_inputs else (lambda x: x)
+    thr = dict(h_min=1.0e-6, h_min_cfl=1.0e-6) if f32_inputs else {}
     mesh = Mesh2D(nx, ny, 3.0, 3.0, ngh=4)
-    cfg = Config(bc_x="fall", bc_y="fall", cfl=0.5, dtype=dtype, friction="manning", manning_n=0.03,
-                 rainfall=50.0 / 3.6e6)
+    cfg = Config(bc_x="fall", bc_y="fall", cfl=0.5, dtype=dtype, friction="manning", manning_n=r(0.03),
+                 rainfall=r(50.0 / 3.6e6), **thr)
     s = Solver2D(mesh, cfg, None, None, track_max_depth=track_max_depth, tile_rows=tile_rows,
-                 device_init=(0.1, lambda ptr, pitch, dcode: bed.write(ptr, pitch, dcode, nx, ny)))
+                 device_init=(r(0.1), lambda ptr, pitch, dcode: bed.write(ptr, pitch, dcode, nx, ny,
+                                                                          round_f32=f32_inputs)))
     s.bed_spec = bed
     return s
+
+
+C3_SEED = 20261203
+TIDE_PERIOD_S = 44712.0     # M2 tide, 12.42 h
+
+
+@dataclass
+class CoastalCase:
+    """Host arrays of the compressed coastal workload (unpadded (n, n) fields)."""
+    mesh: Mesh2D
+    cfg: Config
+    q0: np.ndarray            # (3, n, n) float32
+    bed: np.ndarray           # (n, n) float32
+    active: np.ndarray        # (n, n) bool: land above -2 m plus the nearshore band
+    cls: np.ndarray           # (n, n) uint8 Manning class
+    ntab: np.ndarray          # (8,) Manning n per class
+    sea_edge: np.ndarray      # (n, n) bool: stage cells (active box-edge cells below -2 m)
+
+    def dense(self, track_max_depth: bool = False):
+        """Solver2D with the mask, classes and a CFL that skips inactive cells (so that the
+        dense and flat layouts take the same steps)."""
+        from .solver import Solver2D
+        s = Solver2D(self.mesh, self.cfg, self.q0, self.bed, track_max_depth=track_max_depth)
+        g = self.mesh.ngh
+        cls_pad = np.zeros((self.mesh.nx + 2 * g, self.mesh.ny + 2 * g), np.uint8)
+        cls_pad[g:-g, g:-g] = self.cls
+        s.set_manning_table(cls_pad, self.ntab)
+        s.set_inside_mask(self.active)
+        s.set_cfl_ghost_mask(~self.active)
+        return s
+
+    def flat(self, track_max_depth: bool = False):
+        from .compressed import CompressedSolver
+        return CompressedSolver.from_dense(self.dense(), cfl_linf=True, track_max_depth=track_max_depth)
+
+
+def _taxicab_dilate(mask: np.ndarray, r: int) -> np.ndarray:
+    """Cross (4-neighbour) dilation by r iterations = taxicab distance <= r."""
+    from scipy.ndimage import distance_transform_cdt
+    if not mask.any():
+        return mask.copy()
+    return distance_transform_cdt(~mask, metric="taxicab") <= r
+
+
+def coastal_c3(n: int = 16384, seed: int = C3_SEED, band: int = 64, active_share: float = 0.45,
+               n_sites: int = 256, rain_frames: int = 3, dx: float = 3.0) -> CoastalCase:
+    """The seeded compressed coastal case.
+
+    Bed: value-noise fBm (Hurst 0.8) generated on the device, mapped piecewise-linearly so
+    that the -2 m contour sits at the quantile that makes land (b > -2 m) plus its
+    `band`-cell nearshore band cover ~`active_share` of the box, with elevations spanning
+    -10 .. 60 m. Manning: 8 classes n = linspace(0.02, 0.12, 8) by Voronoi over `n_sites`
+    seeded sites (device). Rain: a smooth correlated field scaled to 0..80 mm/h, a new frame
+    every 300 s. Tide: eta(t) = sin(2 pi t / 12.42 h) on the active box-edge cells below
+    -2 m. Initial state: still water at eta = 0 where b < 0, dry elsewhere. Free-outflow
+    ('fall') box edges, h_min 1e-6, float32.
+    """
+    from .forcing import RainfallForcing, StageBoundary
+    rng = np.random.default_rng(seed)
+    spec = FbmBed(seed=seed, octaves=max(4, int(np.log2(n)) - 1), L0=max(64.0, n / 4.0), slope_x=0.0, dx=dx)
+    raw_dev = L.DeviceArray((n, n), np.float32)
+    spec.write(raw_dev.ptr, n, L.GS_F32, n, n)
+    raw = raw_dev.download()
+    del raw_dev
+    # choose the land quantile on a coarse copy so that land + band ~ active_share
+    f = max(1, n // 1024)
+    coarse = raw[::f, ::f]
+    lo, hi = 0.0, 1.0
+    for _ in range(18):
+        mid = 0.5 * (lo + hi)
+        thr = np.quantile(coarse, 1.0 - mid)
+        share = _taxicab_dilate(coarse > thr, max(1, band // f)).mean()
+        lo, hi = (mid, hi) if share < active_share else (lo, mid)
+    land_frac = 0.5 * (lo + hi)
+    r_star = float(np.quantile(raw[::max(1, n // 2048), ::max(1, n // 2048)], 1.0 - land_frac))
+    rmin, rmax = float(raw.min()), float(raw.max())
+    below = raw < r_star
+    bed = np.where(below, -2.0 + (raw - r_star) * (8.0 / max(r_star - rmin, 1e-12)),
+                   -2.0 + (raw - r_star) * (62.0 / max(rmax - r_star, 1e-12))).astype(np.float32)
+    del raw, below
+    land = bed > -2.0
+    active = _taxicab_dilate(land, band)
+    # Manning classes by Voronoi (device)
+    sx = rng.uniform(0, n, n_sites).astype(np.float32)
+    sy = rng.uniform(0, n, n_sites).astype(np.float32)
+    sc = rng.integers(0, 8, n_sites).astype(np.uint8)
+    dsx, dsy, dsc = (L.DeviceArray.from_host(a) for a in (sx, sy, sc))
+    dcls = L.DeviceArray((n, n), np.uint8)
+    L.call("gs_gen_voronoi", C.c_void_p(dcls.ptr), n, n, n, 0, 0, C.c_void_p(dsx.ptr), C.c_void_p(dsy.ptr),
+           C.c_void_p(dsc.ptr), n_sites, None)
+    cls = dcls.download()
+    ntab = np.linspace(0.02, 0.12, 8)
+    # rain frames (mm/h), smooth correlated fields in [0, 80]
+    frames = []
+    for k in range(rain_frames):
+        rs = FbmBed(seed=seed + 1000 + k, octaves=4, L0=max(32.0, n / 2.0), hurst=0.8, slope_x=0.0)
+        d = L.DeviceArray((n, n), np.float32)
+        rs.write(d.ptr, n, L.GS_F32, n, n)
+        fr = d.download()
+        a, b = float(fr.min()), float(fr.max())
+        frames.append(np.clip((fr - a) * (80.0 / max(b - a, 1e-12)), 0.0, 80.0).astype(np.float32))
+    rain = RainfallForcing(time_s=np.arange(rain_frames) * 300.0, rate_mm_h=np.stack(frames))
+    # tide on the active box-edge cells below -2 m
+    edge = np.zeros((n, n), bool)
+    edge[0, :] = edge[-1, :] = edge[:, 0] = edge[:, -1] = True
+    sea_edge = edge & active & (bed < -2.0)
+    mesh = Mesh2D(n, n, dx, dx, ngh=4)
+    ts = np.linspace(0.0, 2.0 * TIDE_PERIOD_S, 4001)
+    stage = StageBoundary.from_mask(sea_edge, mesh, bed, ts, np.sin(2.0 * np.pi * ts / TIDE_PERIOD_S))
+    cfg = Config(bc_x="fall", bc_y="fall", cfl=0.5, dtype="float32", friction="manning",
+                 rainfall_forcing=rain, stage_boundary=stage)
+    h0 = np.maximum(-bed, 0.0).astype(np.float32)
+    q0 = np.stack([h0, np.zeros_like(h0), np.zeros_like(h0)])
+    return CoastalCase(mesh=mesh, cfg=cfg, q0=q0, bed=bed, active=active, cls=cls, ntab=ntab, sea_edge=sea_edge)

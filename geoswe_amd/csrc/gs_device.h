@@ -43,8 +43,9 @@ __device__ __forceinline__ void hllc(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g
   const T aR = vsqrt(g * (hR > z ? hR : z));
   const T u_st = (T(0.5) * (uNL + uNR) + aL) - aR;
   const T h_rt = T(0.5) * (aL + aR) + T(0.25) * (uNL - uNR);
-  const T h_st = (h_rt > z) ? (h_rt * h_rt) / g : z;
-  const T a_st = vsqrt(g * (h_st > z ? h_st : z));
+  // star celerity: the reference forms h* = h_rt^2/g and a* = sqrt(g h*), which is h_rt
+  // itself in exact arithmetic; taking it directly saves a division and a square root
+  const T a_st = (h_rt > z) ? h_rt : z;
   const T SL_w = vmin(uNL - aL, u_st - a_st);
   const T SR_w = vmax(uNR + aR, u_st + a_st);
   const bool dryL = (hL <= h_min);
@@ -207,14 +208,16 @@ __device__ __forceinline__ void update(T q0, T q1, T q2, T rr0, T r1, T r2, T dt
 template <typename T>
 __device__ __forceinline__ uint64_t lam_bits(T h, T hu, T hv, const Phys<T>& P) {
   if (h < P.h_cfl) return 0ull;
-  const T u = hu / h;
-  const T v = hv / h;
   T vel;
   if (P.cfl_l2) {
+    const T u = hu / h;
+    const T v = hv / h;
     vel = vsqrt(u * u + v * v);
   } else {
-    const T au = vabs(u), av = vabs(v);
-    vel = (au > av) ? au : av;
+    // max(|hu/h|, |hv/h|) = max(|hu|, |hv|)/h bit for bit: correctly rounded division by
+    // the same positive h is monotonic, so one division serves both components
+    const T au = vabs(hu), av = vabs(hv);
+    vel = ((au > av) ? au : av) / h;
   }
   const T lam = vel + vsqrt(P.g * h);
   const double l = (double)lam;

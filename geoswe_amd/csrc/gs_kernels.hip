@@ -110,26 +110,39 @@ __device__ __forceinline__ int map_bed(int i, int n, int bc) {
   return i < 0 ? 0 : n - 1;
 }
 
+// Column part of the ghost map, fixed for a lane over the whole sweep.
+struct ColMap {
+  int mj, mjb;     // state column, bed column
+  bool zero, flip;
+};
+__device__ __forceinline__ ColMap col_map(int c, int ny, int bc) {
+  ColMap m;
+  m.zero = false;
+  m.flip = false;
+  m.mj = map_state(c, ny, bc, m.zero, m.flip);
+  m.mjb = map_bed(c, ny, bc);
+  return m;
+}
+
+// h, hu, hv of cell (i, column map cm); the row map is uniform across the wavefront.
 template <typename T>
-__device__ __forceinline__ void load_q(const DenseK<T>& a, T* const* q, int i, int c, T& h, T& hu,
+__device__ __forceinline__ void load_q(const DenseK<T>& a, T* const* q, int i, const ColMap& cm, T& h, T& hu,
                                        T& hv) {
-  bool zx = false, fx = false, zy = false, fy = false;
+  bool zx = false, fx = false;
   const int mi = map_state(i, a.nx, a.bc_x, zx, fx);
-  const int mj = map_state(c, a.ny, a.bc_y, zy, fy);
-  const int64_t idx = (int64_t)mi * a.pitch + mj;
+  const int64_t idx = (int64_t)mi * a.pitch + cm.mj;
   h = q[0][idx];
   hu = q[1][idx];
   hv = q[2][idx];
   if (fx) hu = -hu;
-  if (fy) hv = -hv;
-  if (zx || zy) { h = T(0); hu = T(0); hv = T(0); }
+  if (cm.flip) hv = -hv;
+  if (zx || cm.zero) { h = T(0); hu = T(0); hv = T(0); }
 }
 
 template <typename T>
-__device__ __forceinline__ T load_b(const DenseK<T>& a, int i, int c) {
+__device__ __forceinline__ T load_b(const DenseK<T>& a, int i, const ColMap& cm) {
   const int mi = map_bed(i, a.nx, a.bc_x);
-  const int mj = map_bed(c, a.ny, a.bc_y);
-  return a.bed[(int64_t)mi * a.pitch + mj];
+  return a.bed[(int64_t)mi * a.pitch + cm.mjb];
 }
 
 template <typename T> __device__ __forceinline__ T from_upper(T x) { return __shfl_down(x, 1, 64); }
@@ -183,29 +196,32 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
     const int i_end = min(i0 + a.tile_rows, a.nx);
     const bool col_ok = (lane >= 2) && (lane < 2 + kOut) && (c < a.ny);
 
-    T bm2 = load_b(a, i0 - 2, cl), bm1 = load_b(a, i0 - 1, cl), b0 = load_b(a, i0, cl),
-      bp1 = load_b(a, i0 + 1, cl);
+    const ColMap cm = col_map(cl, a.ny, a.bc_y);
+    T bm2 = load_b(a, i0 - 2, cm), bm1 = load_b(a, i0 - 1, cm), b0 = load_b(a, i0, cm),
+      bp1 = load_b(a, i0 + 1, cm);
     T hP, huP, hvP, uP, vP;
-    load_q(a, qin, i0 - 1, cl, hP, huP, hvP);
+    load_q(a, qin, i0 - 1, cm, hP, huP, hvP);
     prims(hP, huP, hvP, hm, uP, vP);
     const T gzxP = (T(0.5) * (b0 - bm2)) * ix;
     T hC, huC, hvC, uC, vC;
-    load_q(a, qin, i0, cl, hC, huC, hvC);
+    load_q(a, qin, i0, cm, hC, huC, hvC);
     prims(hC, huC, hvC, hm, uC, vC);
     T gzxC = (T(0.5) * (bp1 - bm1)) * ix;
     Face<T> fx;
     face<T>(bm1, b0, gzxP, gzxC, hP, hC, uP, vP, uC, vC, hdx, g, hm, fx);
-    T xFh = fx.Fh, xFN = fx.FN, xFT = fx.FT, xsR = fx.srcR;
+    // the x-face below the current row, as the products the current cell adds
+    T xFh = fx.Fh * ix, xFN = fx.FN * ix, xFT = fx.FT * ix, xsR = fx.srcR * ix;
     bool xg = fx.gate;
 
     for (int i = i0; i < i_end; ++i) {
-      const T bp2 = load_b(a, i + 2, cl);
+      const T bp2 = load_b(a, i + 2, cm);
       T hN, huN, hvN, uN, vN;
-      load_q(a, qin, i + 1, cl, hN, huN, hvN);
+      load_q(a, qin, i + 1, cm, hN, huN, hvN);
       prims(hN, huN, hvN, hm, uN, vN);
       const T gzxN = (T(0.5) * (bp2 - b0)) * ix;
       Face<T> fn;
       face<T>(b0, bp1, gzxC, gzxN, hC, hN, uC, vC, uN, vN, hdx, g, hm, fn);
+      const T nFh = fn.Fh * ix, nFN = fn.FN * ix, nFT = fn.FT * ix;
 
       // y faces of row i: lane computes its upper face (c, c+1)
       const T bl = from_lower(b0), br = from_upper(b0);
@@ -214,34 +230,35 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
       const T gzyR = from_upper(gzy);
       Face<T> fy;
       face<T>(b0, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
-      const T yFhB = from_lower(fy.Fh), yFNB = from_lower(fy.FN), yFTB = from_lower(fy.FT);
-      const T ysRB = from_lower(fy.srcR);
+      const T yFh = fy.Fh * iy, yFN = fy.FN * iy, yFT = fy.FT * iy;
+      const T yFhB = from_lower(yFh), yFNB = from_lower(yFN), yFTB = from_lower(yFT);
+      const T ysRB = from_lower(fy.srcR * iy);
       const int ygB = from_lower((int)fy.gate);
 
       T rh = T(0), rhu = T(0), rhv = T(0);
       if (fn.gate) {
-        rh = rh - fn.Fh * ix;
-        rhu = rhu - fn.FN * ix;
-        rhv = rhv - fn.FT * ix;
+        rh = rh - nFh;
+        rhu = rhu - nFN;
+        rhv = rhv - nFT;
         rhu = rhu - fn.srcL * ix;
       }
       if (xg) {
-        rh = rh + xFh * ix;
-        rhu = rhu + xFN * ix;
-        rhv = rhv + xFT * ix;
-        rhu = rhu + xsR * ix;
+        rh = rh + xFh;
+        rhu = rhu + xFN;
+        rhv = rhv + xFT;
+        rhu = rhu + xsR;
       }
       if (fy.gate) {
-        rh = rh - fy.Fh * iy;
-        rhu = rhu - fy.FT * iy;
-        rhv = rhv - fy.FN * iy;
+        rh = rh - yFh;
+        rhu = rhu - yFT;
+        rhv = rhv - yFN;
         rhv = rhv - fy.srcL * iy;
       }
       if (ygB) {
-        rh = rh + yFhB * iy;
-        rhu = rhu + yFTB * iy;
-        rhv = rhv + yFNB * iy;
-        rhv = rhv + ysRB * iy;
+        rh = rh + yFhB;
+        rhu = rhu + yFTB;
+        rhv = rhv + yFNB;
+        rhv = rhv + ysRB;
       }
 
       if (col_ok) {
@@ -283,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
       // slide the window one row down
       hC = hN; huC = huN; hvC = hvN; uC = uN; vC = vN; gzxC = gzxN;
       bm1 = b0; b0 = bp1; bp1 = bp2;
-      xFh = fn.Fh; xFN = fn.FN; xFT = fn.FT; xsR = fn.srcR; xg = fn.gate;
+      xFh = nFh; xFN = nFN; xFT = nFT; xsR = fn.srcR * ix; xg = fn.gate;
     }
   }
   if (MODE == 1) grid_reduce_and_plan(lam, a.clock, a.plan, true);

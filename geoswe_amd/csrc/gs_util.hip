@@ -53,14 +53,14 @@ __device__ double fbm_at(int64_t i, int64_t j, uint32_t seed, int octaves, doubl
 template <typename T>
 __global__ void k_gen_bed(T* out, int nx, int ny, int64_t pitch, int64_t i0, int64_t j0, int64_t stride,
                           uint32_t seed, int octaves, double L0, double gain, double mu, double scale, double slope_x,
-                          double dx, double offset) {
+                          double dx, double offset, int round_f32) {
   const int64_t n = (int64_t)nx * ny;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = k / ny, j = k - (k / ny) * ny;
     const int64_t gi = i0 + i * stride, gj = j0 + j * stride;
     const double v = (fbm_at(gi, gj, seed, octaves, L0, gain) - mu) * scale;
     const double b = slope_x * (((double)gi + 0.5) * dx) + v + offset;
-    out[i * pitch + j] = (T)b;
+    out[i * pitch + j] = round_f32 ? (T)(float)b : (T)b;
   }
 }
 
@@ -70,6 +70,35 @@ __global__ void k_fill(T* out, int64_t rows, int64_t cols, int64_t pitch, T valu
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = k / cols, j = k - i * cols;
     out[i * pitch + j] = value;
+  }
+}
+
+// Voronoi class field: each cell takes the class of the nearest site (squared distance in
+// cell units, ties to the lowest site index), centres at (i0 + i + 0.5, j0 + j + 0.5).
+__global__ __launch_bounds__(256) void k_gen_voronoi(uint8_t* out, int nx, int ny, int64_t pitch, int64_t i0,
+                                                     int64_t j0, const float* __restrict__ sx,
+                                                     const float* __restrict__ sy, const uint8_t* __restrict__ scls,
+                                                     int nsites) {
+  __shared__ float s_x[1024], s_y[1024];
+  __shared__ uint8_t s_c[1024];
+  for (int t = threadIdx.x; t < nsites; t += blockDim.x) {
+    s_x[t] = sx[t];
+    s_y[t] = sy[t];
+    s_c[t] = scls[t];
+  }
+  __syncthreads();
+  const int64_t n = (int64_t)nx * ny;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = k / ny, j = k - (k / ny) * ny;
+    const float x = (float)(i0 + i) + 0.5f, y = (float)(j0 + j) + 0.5f;
+    float best = INFINITY;
+    uint8_t bc = 0;
+    for (int t = 0; t < nsites; ++t) {
+      const float dx = x - s_x[t], dy = y - s_y[t];
+      const float d = dx * dx + dy * dy;
+      if (d < best) { best = d; bc = s_c[t]; }
+    }
+    out[i * pitch + j] = bc;
   }
 }
 
@@ -110,20 +139,30 @@ int gs_fill(void* dst, int dtype, int64_t rows, int64_t cols, int64_t pitch, dou
 
 int gs_gen_bed(void* dst, int dtype, int nx, int ny, int64_t pitch, int64_t i0, int64_t j0, int64_t stride,
                uint32_t seed, int octaves, double L0, double hurst, double mu, double scale, double slope_x,
-               double dx, double offset, void* stream) {
+               double dx, double offset, int round_f32, void* stream) {
   if (!dst || nx < 1 || ny < 1 || pitch < ny || stride < 1 || octaves < 1 || octaves > 30 || !(L0 > 0.0))
     return fail(GS_E_ARG, "gs_gen_bed: bad arguments");
   const double gain = pow(2.0, -hurst);
   const unsigned g = grid_for((int64_t)nx * ny);
   if (dtype == GS_F32)
     hipLaunchKernelGGL(k_gen_bed<float>, dim3(g), dim3(256), 0, (hipStream_t)stream, (float*)dst, nx, ny, pitch, i0,
-                       j0, stride, seed, octaves, L0, gain, mu, scale, slope_x, dx, offset);
+                       j0, stride, seed, octaves, L0, gain, mu, scale, slope_x, dx, offset, round_f32);
   else if (dtype == GS_F64)
     hipLaunchKernelGGL(k_gen_bed<double>, dim3(g), dim3(256), 0, (hipStream_t)stream, (double*)dst, nx, ny, pitch,
-                       i0, j0, stride, seed, octaves, L0, gain, mu, scale, slope_x, dx, offset);
+                       i0, j0, stride, seed, octaves, L0, gain, mu, scale, slope_x, dx, offset, round_f32);
   else
     return fail(GS_E_DTYPE, "gs_gen_bed: unsupported dtype %d", dtype);
   GS_HIP(hipGetLastError(), "k_gen_bed launch");
+  return GS_OK;
+}
+
+int gs_gen_voronoi(uint8_t* dst, int nx, int ny, int64_t pitch, int64_t i0, int64_t j0, const float* site_x,
+                   const float* site_y, const uint8_t* site_cls, int nsites, void* stream) {
+  if (!dst || nx < 1 || ny < 1 || pitch < ny || !site_x || !site_y || !site_cls || nsites < 1 || nsites > 1024)
+    return fail(GS_E_ARG, "gs_gen_voronoi: bad arguments (1..1024 sites)");
+  hipLaunchKernelGGL(k_gen_voronoi, dim3(grid_for((int64_t)nx * ny)), dim3(256), 0, (hipStream_t)stream, dst, nx, ny,
+                     pitch, i0, j0, site_x, site_y, site_cls, nsites);
+  GS_HIP(hipGetLastError(), "k_gen_voronoi launch");
   return GS_OK;
 }
 

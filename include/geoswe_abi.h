@@ -246,13 +246,22 @@ int gs_fill(void* dst, int dtype, int64_t rows, int64_t cols, int64_t pitch, dou
 /*
  * Seeded synthetic bed b(i, j) = slope_x*x_I + scale*(fbm(I, J) - mu) + offset on a (nx, ny)
  * plane, I = i0 + i*stride, J = j0 + j*stride: value-noise fBm over Philox2x32-10 lattice
- * values, `octaves` octaves of spacing L0/2^k cells and gain 2^-hurst, x_I = (I+0.5)*dx. Replaces the host-side
+ * values, `octaves` octaves of spacing L0/2^k cells and gain 2^-hurst, x_I = (I+0.5)*dx.
+ * With round_f32 the float64 value is rounded to float32 first (a float64 run on the
+ * float32 run's exact inputs). Replaces the host-side
  * synthetic-case set-up of the reference's scaling harness
  * (benchmark/scaling_640m/scaling_bench.py:59-83); oracle/generators.py mirrors it bit for bit.
  */
 int gs_gen_bed(void* dst, int dtype, int nx, int ny, int64_t pitch, int64_t i0, int64_t j0, int64_t stride,
                uint32_t seed, int octaves, double L0, double hurst, double mu, double scale, double slope_x,
-               double dx, double offset, void* stream);
+               double dx, double offset, int round_f32, void* stream);
+/* Voronoi class field for synthetic Manning land cover: each cell of the (nx, ny) plane
+ * takes the class of the nearest of `nsites` sites (float32 squared distance in cell units
+ * from (i0+i+0.5, j0+j+0.5), ties to the lowest index). Replaces the host-side land-cover
+ * preparation of the reference's cases (data_prep.py NLCD -> Manning), for synthetic
+ * benchmark inputs; oracle/generators.py mirrors it bit for bit. */
+int gs_gen_voronoi(uint8_t* dst, int nx, int ny, int64_t pitch, int64_t i0, int64_t j0, const float* site_x,
+                   const float* site_y, const uint8_t* site_cls, int nsites, void* stream);
 /* Plain streaming copy kernel (16 B per lane), used to calibrate HBM bandwidth and the
  * rocprofv3 FETCH_SIZE/WRITE_SIZE counters. No reference counterpart. */
 int gs_bench_copy(void* dst, const void* src, size_t bytes, void* stream);

@@ -61,10 +61,30 @@ def fbm(gi, gj, seed: int, octaves: int, L0: float, hurst: float):
     return acc
 
 
-def bed_crop(i0, j0, nx, ny, seed, octaves, L0, hurst, mu, scale, slope_x, dx, offset, dtype=np.float32, stride=1):
+def bed_crop(i0, j0, nx, ny, seed, octaves, L0, hurst, mu, scale, slope_x, dx, offset, dtype=np.float32, stride=1,
+             round_f32=False):
     """b = slope_x*x + scale*(fbm - mu) + offset on the crop [i0, i0+nx) x [j0, j0+ny)."""
     gi = i0 + np.arange(nx, dtype=np.int64)[:, None] * stride
     gj = j0 + np.arange(ny, dtype=np.int64)[None, :] * stride
     v = (fbm(gi, gj, seed, octaves, L0, hurst) - mu) * scale
     b = slope_x * ((gi.astype(np.float64) + 0.5) * dx) + v + offset
+    if round_f32:
+        b = b.astype(np.float32)
     return b.astype(dtype)
+
+
+def voronoi_crop(i0, j0, nx, ny, sx, sy, scls):
+    """Class of the nearest site per cell (float32 squared distance, ties to the lowest
+    index), the CPU mirror of gs_gen_voronoi."""
+    x = (np.arange(nx, dtype=np.int64) + i0).astype(np.float32) + np.float32(0.5)
+    y = (np.arange(ny, dtype=np.int64) + j0).astype(np.float32) + np.float32(0.5)
+    best = np.full((nx, ny), np.inf, np.float32)
+    cls = np.zeros((nx, ny), np.uint8)
+    for t in range(len(sx)):
+        dx = x[:, None] - np.float32(sx[t])
+        dy = y[None, :] - np.float32(sy[t])
+        d = dx * dx + dy * dy
+        m = d < best
+        best = np.where(m, d, best)
+        cls = np.where(m, np.uint8(scls[t]), cls)
+    return cls

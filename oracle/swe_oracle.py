@@ -34,6 +34,9 @@ Documented departures (shared with the HIP kernels, which implement the same rul
   GPU results are reproducible bit for bit. Its error against the exact power is below
   13 ulp, the same order as ``powf(h, (float)(-4/3))`` itself (which carries the
   rounding of the exponent).
+* HLLC star celerity: the reference's ``a* = sqrt(g * (h_rt^2 / g))`` is taken as ``h_rt``
+  (for ``h_rt > 0``, else 0), its exact value; it differs from the reference's rounding by
+  at most a few ulp and saves a division and a square root per face.
 * The CFL reduction keeps NaN (IEEE-bit maximum), as the reference flat kernels do
   (compressed_solver.py:200-240); the dense reference kernel drops a NaN lambda.
 """
@@ -175,8 +178,8 @@ def hllc(hL, uNL, uTL, hR, uNR, uTR, g, h_min):
     aR = np.sqrt(g * np.where(hR > z, hR, z))
     u_st = (T(0.5) * (uNL + uNR) + aL) - aR
     h_rt = T(0.5) * (aL + aR) + T(0.25) * (uNL - uNR)
-    h_st = np.where(h_rt > z, (h_rt * h_rt) / g, z)
-    a_st = np.sqrt(g * np.where(h_st > z, h_st, z))
+    # star celerity sqrt(g * h_rt^2/g) taken as h_rt itself (exact arithmetic identity)
+    a_st = np.where(h_rt > z, h_rt, z)
     SL_w = np.minimum(uNL - aL, u_st - a_st)
     SR_w = np.maximum(uNR + aR, u_st + a_st)
     dryL = hL <= h_min
