@@ -101,6 +101,87 @@ def copy_calibration(nbytes: int, reps: int):
                       "GBps_read_plus_write": 2 * nbytes / (per * 1e-3) / 1e9}), flush=True)
 
 
+def coastal(args):
+    """The compressed coastal case: flat and dense layouts stepped side by side, depth and
+    residual compared bit for bit at steps 1, 500 and the last; both timed with HIP events.
+    Prints one JSON line for the flat layout (the production layout of this case)."""
+    from geoswe_amd import _lib as L
+    from geoswe_amd.cases import coastal_c3
+    n = args.n
+    t0 = time.perf_counter()
+    log(f"building the coastal case {n}x{n}")
+    case = coastal_c3(n=n)
+    d = case.dense(track_max_depth=False)
+    f = case.flat(track_max_depth=False)
+    fm = f.fm
+    act = fm.active
+    ij = fm.ij[act] - case.mesh.ngh
+    log(f"built in {time.perf_counter() - t0:.0f} s: {fm.n_active / 1e6:.1f} M active cells "
+        f"({fm.n_active / n / n:.1%} of the box), {fm.n_stored / 1e6:.1f} M stored, {len(fm.run_k0)} runs")
+
+    def compare(tag):
+        hd = d.depth()[ij[:, 0], ij[:, 1]]
+        hf = f.q_flat()[0][act]
+        same_h = bool(np.array_equal(hd, hf))
+        rd = d.residual()
+        rf = f.residual()
+        same_r = bool(np.array_equal(rf[:, act], rd[:, ij[:, 0], ij[:, 1]]))
+        del rd, rf
+        same_clock = d.nsteps == f.nsteps and d.t == f.t
+        log(f"{tag}: steps {d.nsteps}/{f.nsteps}, t {d.t:.6f}/{f.t:.6f} s, depth equal {same_h}, "
+            f"residual equal {same_r}, wet cells {int((hf > 1e-6).sum())}")
+        return same_h and same_r and same_clock
+
+    ok = compare("t=0")
+    marks = [1, 500, args.steps] if args.steps > 500 else [1, args.steps]
+    tf = td = 0.0
+    done = 0
+    for m in marks:
+        k = m - done
+        wf, ef = timed_steps(f, k)
+        wd, ed = timed_steps(d, k)
+        if m > 1:
+            tf += ef
+            td += ed
+        done = m
+        ok = compare(f"step {m}") and ok
+    timed = done - 1
+    for s_ in (d, f):
+        if s_._clockv().status & L.ST_NONFINITE:
+            raise FloatingPointError("non-finite state in the coastal run")
+    ms_f, ms_d = tf / timed, td / timed
+    nact = fm.n_active
+    bpc = 33.0      # q 12 + bed 4 + flags 1 + rain frame 4 + q' 12 per active cell
+    achieved = bpc * nact / (ms_f * 1e-3) / 1e9
+    log(f"flat {ms_f:.3f} ms/step ({nact / (ms_f * 1e-3) / 1e9:.1f} G active cell-updates/s), dense "
+        f"{ms_d:.3f} ms/step over the whole box; {achieved:.0f} GB/s algorithmic (flat)")
+    out = {
+        "metric": "active cell-updates/s (compressed coastal step, flat layout, 1 MI355X)",
+        "value": nact / (ms_f * 1e-3),
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": timed,
+        "warmup": 1,
+        "ms_per_step": ms_f,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded fBm DEM, Voronoi Manning classes, fBm rain frames, M2 tide)",
+        "config": {"workload": f"coastal {n}x{n}, {nact} active cells ({nact / n / n:.1%}), 8 Manning classes, "
+                               f"gridded rain every 300 s, tidal stage on the sea edge, fall edges, CFL 0.5 (L-inf)",
+                   "layout": "flat", "active_cells": int(nact), "stored_cells": int(fm.n_stored)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gs::k_flat_step<float, 1>",
+                     "kernel_ms": ms_f, "bytes_per_cell": bpc},
+        "dense_ms_per_step": ms_d,
+        "dense_equals_flat": ok,
+    }
+    print(json.dumps(out), flush=True)
+    if not ok:
+        sys.exit(1)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,13 +197,19 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead of the step, time the plain copy kernel over this many bytes (HBM calibration)")
     ap.add_argument("--copy-reps", type=int, default=10)
+    ap.add_argument("--workload", choices=["rain-on-slope", "coastal"], default="rain-on-slope",
+                    help="coastal: the compressed coastal case (flat and dense layouts, bitwise cross-check)")
+    ap.add_argument("--n", type=int, default=16384, help="coastal box side")
     args = ap.parse_args()
     if args.copy_bytes:
         return copy_calibration(args.copy_bytes, args.copy_reps)
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build (the domain "
-              f"would be split into strips with a 2-cell halo exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build (a multi-GPU run "
+              f"would split the domain into strips along y with a 2-cell halo exchange per step)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "coastal":
+        return coastal(args)
 
     from geoswe_amd import _lib as L
     from geoswe_amd.cases import FbmBed, C2_SEED, rain_on_slope_c2

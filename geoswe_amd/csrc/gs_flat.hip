@@ -9,8 +9,8 @@
 // neighbour table. Cells in no run are the frozen ring: never written, identical in both
 // state buffers.
 //
-// A wavefront covers 64 consecutive flat ids and writes the middle 60 (the chunk stride
-// of the run index). Each lane evaluates its two x-faces with gathered neighbours and its
+// A wavefront pass covers 64 consecutive flat ids and writes the middle 60 (the chunk
+// stride of the run index). Each lane evaluates its two x-faces with gathered neighbours and its
 // upper y-face with the lane above; the lower y-face comes from the lane below by a
 // cross-lane shuffle. The face arithmetic is gs_device.h's, shared with the dense sweep,
 // so the residual is bitwise the dense one.
@@ -24,7 +24,8 @@ constexpr int kFBlock = 256;
 template <typename T>
 struct FlatK {
   int64_t n;
-  int n_runs, nyp, ngh, nchunks;
+  int n_runs, nyp, ngh;
+  int64_t nchunks;
   T* buf[2][3];
   const T* bed;
   const uint8_t* flags;
@@ -82,10 +83,16 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
   const T* frame = (MODE == 1 && a.rain_mode == 2) ? a.rain_frames + (int64_t)ridx * a.frame_stride : nullptr;
 
   const int lane = threadIdx.x & 63;
-  const int64_t chunk = (int64_t)blockIdx.x * (kFBlock / 64) + (threadIdx.x >> 6);
+  // Resident grid, each wavefront striding over chunks. Workgroups are dealt round-robin to
+  // the 8 XCDs; the remap gives each XCD a contiguous range of wavefronts, hence of chunks
+  // in every pass, so the rows i+-1 a chunk gathers are mostly in that XCD's L2.
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned qd = nb / 8u, rm = nb % 8u, xcd = b % 8u, kk = b / 8u;
+  const unsigned lb = (xcd < rm) ? xcd * (qd + 1u) + kk : rm * (qd + 1u) + (xcd - rm) * qd + kk;
+  const int64_t nwaves = (int64_t)nb * (kFBlock / 64);
   uint64_t lam = 0ull;
 
-  if (chunk < a.nchunks) {
+  for (int64_t chunk = (int64_t)lb * (kFBlock / 64) + (threadIdx.x >> 6); chunk < a.nchunks; chunk += nwaves) {
     const int64_t k = chunk * kFOut + lane - 2;
     const int64_t kl = k < 0 ? 0 : (k >= a.n ? a.n - 1 : k);
     // this lane's cell (needed by every lane for the y-face shuffles)
@@ -179,7 +186,10 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
         qout[0][k] = h;
         qout[1][k] = hu;
         qout[2][k] = hv;
-        if (!(fg & GS_FLAG_CFL_SKIP)) lam = lam_bits<T>(h, hu, hv, P);
+        if (!(fg & GS_FLAG_CFL_SKIP)) {
+          const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
+          lam = lb_ > lam ? lb_ : lam;
+        }
       }
     }
   }
@@ -229,7 +239,7 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   a.n_runs = f->n_runs;
   a.nyp = f->nyp;
   a.ngh = f->ngh;
-  a.nchunks = (int)((f->n_stored + kFOut - 1) / kFOut);
+  a.nchunks = (f->n_stored + kFOut - 1) / kFOut;
   for (int s = 0; s < 2; ++s)
     for (int k = 0; k < 3; ++k) a.buf[s][k] = (T*)f->q[s][k];
   a.bed = (const T*)f->bed;
@@ -254,13 +264,31 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   return GS_OK;
 }
 
+// Workgroups of k_flat_step that are resident at once on the current device (cached per
+// device and instantiation); the grid never exceeds the work.
+template <typename T, int MODE>
+static unsigned flat_grid(int64_t nchunks) {
+  static int cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) dev = -1;
+  int per = (dev >= 0 && dev < 64) ? cache[dev] : 0;
+  if (per <= 0) {
+    int occ = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_flat_step<T, MODE>, kFBlock, 0) != hipSuccess) occ = 4;
+    if (dev < 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
+    per = (occ > 0 ? occ : 1) * (cus > 0 ? cus : 1);
+    if (dev >= 0 && dev < 64) cache[dev] = per;
+  }
+  const int64_t need = (nchunks + 3) / 4;
+  return (unsigned)(need < per ? (need > 0 ? need : 1) : per);
+}
+
 template <typename T>
 static int flat_step_t(const GsFlat* f, const GsPhys* p, GsForcing fo, GsClock* clk, hipStream_t s) {
   FlatK<T> a;
   int rc = build_flat<T>(f, p, fo, clk, a, true);
   if (rc) return rc;
-  const unsigned blocks = (unsigned)((a.nchunks + 3) / 4);
-  hipLaunchKernelGGL((k_flat_step<T, 1>), dim3(blocks), dim3(kFBlock), 0, s, a);
+  hipLaunchKernelGGL((k_flat_step<T, 1>), dim3(flat_grid<T, 1>(a.nchunks)), dim3(kFBlock), 0, s, a);
   GS_HIP(hipGetLastError(), "k_flat_step launch");
   return GS_OK;
 }
@@ -279,8 +307,7 @@ static int flat_residual_t(const GsFlat* f, const GsPhys* p, int which, void* r0
   a.rhs[0] = (T*)r0;
   a.rhs[1] = (T*)r1;
   a.rhs[2] = (T*)r2;
-  const unsigned blocks = (unsigned)((a.nchunks + 3) / 4);
-  hipLaunchKernelGGL((k_flat_step<T, 0>), dim3(blocks), dim3(kFBlock), 0, s, a);
+  hipLaunchKernelGGL((k_flat_step<T, 0>), dim3(flat_grid<T, 0>(a.nchunks)), dim3(kFBlock), 0, s, a);
   GS_HIP(hipGetLastError(), "k_flat_step(residual) launch");
   return GS_OK;
 }
