@@ -93,6 +93,14 @@ _SIGS = {
     "gs_flat_step": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, _P, _P]),
     "gs_flat_residual": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), C.c_int, _P, _P, _P, _P]),
     "gs_flat_cfl": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, _P, _P]),
+    "gs_ens_create": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), C.POINTER(GsForcing), C.POINTER(_P), C.c_int,
+                                C.POINTER(_P)]),
+    "gs_ens_destroy": (C.c_int, [_P]),
+    "gs_ens_set_launch": (C.c_int, [_P, C.c_int, C.c_int]),
+    "gs_ens_get_launch": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+    "gs_ens_step": (C.c_int, [_P, _P]),
+    "gs_ens_run": (C.c_int, [_P, C.c_int64, C.c_int, _P]),
+    "gs_ens_autotune": (C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_double)]),
     "gs_fill": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_double, _P]),
     "gs_gen_bed": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_uint32,
                              C.c_int,

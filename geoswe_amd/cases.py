@@ -203,3 +203,40 @@ def coastal_c3(n: int = 16384, seed: int = C3_SEED, band: int = 64, active_share
     h0 = np.maximum(-bed, 0.0).astype(np.float32)
     q0 = np.stack([h0, np.zeros_like(h0), np.zeros_like(h0)])
     return CoastalCase(mesh=mesh, cfg=cfg, q0=q0, bed=bed, active=active, cls=cls, ntab=ntab, sea_edge=sea_edge)
+
+
+C4_SEED = 20261204
+
+
+def ensemble_sides(n_members: int = 64, seed: int = C4_SEED, lo_exp: float = 8.0, span_exp: float = 4.0,
+                   quantum: int = 64) -> list:
+    """Member box sides L_k = 2^(lo_exp + span_exp*u_k) (u_k seeded uniform), rounded to a
+    multiple of `quantum`: log-uniform between 2^lo_exp and 2^(lo_exp+span_exp)."""
+    u = np.random.default_rng(seed).uniform(0.0, 1.0, n_members)
+    return [int(max(quantum, round(2.0 ** (lo_exp + span_exp * x) / quantum) * quantum)) for x in u]
+
+
+def ensemble_c4(n_members: int = 64, seed: int = C4_SEED, sides: list = None, rain_frames: int = 3,
+                log=None) -> list:
+    """CoastalCase members of the batched-ensemble workload: member k is the coastal
+    generator at side L_k with seed C3_SEED + k and a nearshore band scaled with the box
+    (64 cells at 16384), its rain frames scaled by U[0.5, 2] and each class's Manning n by
+    1 + U[-0.3, 0.3] (seeded)."""
+    from .forcing import RainfallForcing
+    sides = sides or ensemble_sides(n_members, seed)
+    rng = np.random.default_rng(seed + 1)
+    out = []
+    for k, n in enumerate(sides):
+        band = max(4, int(round(64 * n / 16384)))
+        case = coastal_c3(n=n, seed=C3_SEED + k, band=band, n_sites=max(16, min(256, n // 16)),
+                          rain_frames=rain_frames)
+        rs = float(rng.uniform(0.5, 2.0))
+        nf = 1.0 + rng.uniform(-0.3, 0.3, len(case.ntab))
+        rf = case.cfg.rainfall_forcing
+        case.cfg.rainfall_forcing = RainfallForcing(time_s=rf.time_s,
+                                                    rate_mm_h=(np.asarray(rf.rate_mm_h) * rs).astype(np.float32))
+        case.ntab = case.ntab * nf
+        out.append(case)
+        if log:
+            log(f"member {k}: {n}x{n}, active {case.active.mean():.1%}, rain x{rs:.2f}")
+    return out

@@ -12,16 +12,20 @@
 
 #include "../../include/geoswe_abi.h"
 
+// Host and device: the same arithmetic also builds for the CPU (host-side checks of the
+// hand-written adjoint against finite differences of these functions).
+#define GS_HD __host__ __device__ __forceinline__
+
 namespace gs {
 
-template <typename T> __device__ __forceinline__ T vmax(T a, T b) { return fmax(a, b); }
-template <typename T> __device__ __forceinline__ T vmin(T a, T b) { return fmin(a, b); }
-template <> __device__ __forceinline__ float vmax<float>(float a, float b) { return fmaxf(a, b); }
-template <> __device__ __forceinline__ float vmin<float>(float a, float b) { return fminf(a, b); }
-__device__ __forceinline__ float vsqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double vsqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float vabs(float x) { return fabsf(x); }
-__device__ __forceinline__ double vabs(double x) { return fabs(x); }
+template <typename T> GS_HD T vmax(T a, T b) { return fmax(a, b); }
+template <typename T> GS_HD T vmin(T a, T b) { return fmin(a, b); }
+template <> GS_HD float vmax<float>(float a, float b) { return fmaxf(a, b); }
+template <> GS_HD float vmin<float>(float a, float b) { return fminf(a, b); }
+GS_HD float vsqrt(float x) { return sqrtf(x); }
+GS_HD double vsqrt(double x) { return sqrt(x); }
+GS_HD float vabs(float x) { return fabsf(x); }
+GS_HD double vabs(double x) { return fabs(x); }
 
 // Per-face SRM reconstruction + HLLC flux + the two one-sided bed sources.
 // L is the lower-index cell, R the upper one, along the face normal.
@@ -36,7 +40,7 @@ struct Face {
 // evaluated for the star state; the selected value is computed with the reference's
 // expressions, so the result is the reference's bit pattern.
 template <typename T>
-__device__ __forceinline__ void hllc(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min,
+GS_HD void hllc(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min,
                                      T& Fh, T& FN, T& FT) {
   const T z = T(0);
   const T aL = vsqrt(g * (hL > z ? hL : z));
@@ -98,7 +102,7 @@ __device__ __forceinline__ void hllc(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g
 // is several micrometres, which the absolute-elevation form adds to every face depth
 // and to the bed-slope source.
 template <typename T>
-__device__ __forceinline__ void face(T bL, T bR, T gzL, T gzR, T hL, T hR, T uNL, T uTL, T uNR, T uTR,
+GS_HD void face(T bL, T bR, T gzL, T gzR, T hL, T hR, T uNL, T uTL, T uNR, T uTR,
                                      T half_d, T g, T h_min, Face<T>& f) {
   const T z = T(0);
   f.gate = !(hL < h_min && hR < h_min);
@@ -124,7 +128,7 @@ __device__ __forceinline__ void face(T bL, T bR, T gzL, T gzR, T hL, T hR, T uNL
 
 // Primitive velocities with the dry gate (rhs_cuda.py:931-938).
 template <typename T>
-__device__ __forceinline__ void prims(T h, T hu, T hv, T h_min, T& u, T& v) {
+GS_HD void prims(T h, T hu, T hv, T h_min, T& u, T& v) {
   const bool wet = h > h_min;
   const T hs = wet ? h : h_min;
   u = wet ? hu / hs : T(0);
@@ -134,9 +138,9 @@ __device__ __forceinline__ void prims(T h, T hu, T hv, T h_min, T& u, T& v) {
 // h^(-1/3) from IEEE +,-,* only: Newton on f(y) = y^-3 - h, y <- y*(4/3 - (h/3)*y^3),
 // three iterations from the magic-constant seed of Moroz et al. (2021),
 // bits(y0) = 0x54a2fa8c - bits(h)/3.
-__device__ __forceinline__ float icbrt(float h) {
-  const uint32_t bits = __float_as_uint(h);
-  float y = __uint_as_float(0x54A2FA8Cu - bits / 3u);
+GS_HD float icbrt(float h) {
+  const uint32_t bits = __builtin_bit_cast(uint32_t, h);
+  float y = __builtin_bit_cast(float, 0x54A2FA8Cu - bits / 3u);
   const float xt = h * (float)(1.0 / 3.0);
   const float c43 = (float)(4.0 / 3.0);
   y = y * (c43 - xt * ((y * y) * y));
@@ -145,13 +149,13 @@ __device__ __forceinline__ float icbrt(float h) {
   return y;
 }
 // h^(-4/3) = (y*y)*(y*y); bit-reproducible on the CPU (numpy float32 ops).
-__device__ __forceinline__ float pow_m43(float h) {
+GS_HD float pow_m43(float h) {
   const float y = icbrt(h);
   const float y2 = y * y;
   return y2 * y2;
 }
 // float64: the float32 root refined by two more Newton steps in float64.
-__device__ __forceinline__ double pow_m43(double h) {
+GS_HD double pow_m43(double h) {
   double y = (double)icbrt((float)h);
   const double xt = h * (1.0 / 3.0);
   const double c43 = 4.0 / 3.0;
@@ -171,7 +175,7 @@ struct Phys {
 // Fused update tail, rhs_cuda.py:1757-1796 (flat form compressed_rhs.py:716-748):
 // Euler update, wet/dry rule, point-implicit Manning friction with the velocity cap.
 template <typename T>
-__device__ __forceinline__ void update(T q0, T q1, T q2, T rr0, T r1, T r2, T dt, T n,
+GS_HD void update(T q0, T q1, T q2, T rr0, T r1, T r2, T dt, T n,
                                        const Phys<T>& P, T& h, T& hu, T& hv) {
   h = q0 + dt * rr0;
   hu = q1 + dt * r1;

@@ -14,6 +14,8 @@
 // upper y-face with the lane above; the lower y-face comes from the lane below by a
 // cross-lane shuffle. The face arithmetic is gs_device.h's, shared with the dense sweep,
 // so the residual is bitwise the dense one.
+#include <vector>
+
 #include "gs_common.h"
 
 namespace gs {
@@ -52,147 +54,177 @@ struct FlatK {
 template <typename T> __device__ __forceinline__ T fl_upper(T x) { return __shfl_down(x, 1, 64); }
 template <typename T> __device__ __forceinline__ T fl_lower(T x) { return __shfl_up(x, 1, 64); }
 
+// Per-launch inputs of a step that come from the clock (or the residual selector).
+template <typename T>
+struct FlatCtx {
+  T* const* qin;
+  T* const* qout;
+  T dt, rain_u, eta;
+  const T* frame;   // gridded rain frame of this step (rain_mode 2)
+  const T* ntab;    // Manning n per class (LDS copy or the member's table)
+};
+
+// Reads the clock into a step context; false when the clock says the run is done.
 template <typename T, int MODE>
-__global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
+__device__ __forceinline__ bool flat_ctx(const FlatK<T>& a, FlatCtx<T>& x) {
   int par = 0;
-  T dt = T(0), rain_u = T(0), eta = T(0);
-  int ridx = 0;
+  x.dt = T(0);
+  x.rain_u = T(0);
+  x.eta = T(0);
+  x.frame = nullptr;
   if (MODE == 1) {
     const GsClock* c = a.clock;
-    if (c->status & GS_ST_DONE) return;
+    if (c->status & GS_ST_DONE) return false;
     par = (int)(c->nsteps & 1);
-    ridx = c->rain_idx;
+    const int ridx = c->rain_idx;
     if (sizeof(T) == 4) {
-      dt = (T)c->dt32;
-      rain_u = (T)c->rain32;
-      eta = (T)c->eta32;
+      x.dt = (T)c->dt32;
+      x.rain_u = (T)c->rain32;
+      x.eta = (T)c->eta32;
     } else {
-      dt = (T)c->dt;
-      rain_u = a.rain_mode == 1 ? (T)a.plan.rain_rate[ridx] : T(0);
-      eta = (T)c->eta64;
+      x.dt = (T)c->dt;
+      x.rain_u = a.rain_mode == 1 ? (T)a.plan.rain_rate[ridx] : T(0);
+      x.eta = (T)c->eta64;
     }
+    if (a.rain_mode == 2) x.frame = a.rain_frames + (int64_t)ridx * a.frame_stride;
   }
+  x.qin = MODE == 1 ? a.buf[par] : a.buf[a.which];
+  x.qout = a.buf[par ^ 1];
+  return true;
+}
+
+// One wavefront pass over flat ids chunk*60-2 .. chunk*60+61: the residual (MODE 0) or the
+// fused update (MODE 1) of the 60 middle ids, folding their CFL speed bits into lam.
+// Every lane of the wavefront must call it (cross-lane shuffles).
+template <typename T, int MODE>
+__device__ __forceinline__ void flat_chunk(const FlatK<T>& a, const FlatCtx<T>& x, int64_t chunk, int lane,
+                                           uint64_t& lam) {
+  const Phys<T>& P = a.P;
+  const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
+  T* const* qin = x.qin;
+  const int64_t k = chunk * kFOut + lane - 2;
+  const int64_t kl = k < 0 ? 0 : (k >= a.n ? a.n - 1 : k);
+  // this lane's cell (needed by every lane for the y-face shuffles)
+  const T hC = qin[0][kl], huC = qin[1][kl], hvC = qin[2][kl];
+  const T bC = a.bed[kl];
+  T uC, vC;
+  prims(hC, huC, hvC, hm, uC, vC);
+  const T bl = fl_lower(bC), br = fl_upper(bC);
+  const T gzy = (T(0.5) * (br - bl)) * iy;
+  // upper y-face (k, k+1)
+  const T hR = fl_upper(hC), uR = fl_upper(uC), vR = fl_upper(vC);
+  const T gzyR = fl_upper(gzy);
+  Face<T> fy;
+  face<T>(bC, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
+  const T yFhB = fl_lower(fy.Fh), yFNB = fl_lower(fy.FN), yFTB = fl_lower(fy.FT);
+  const T ysRB = fl_lower(fy.srcR);
+  const int ygB = fl_lower((int)fy.gate);
+
+  // run of this cell
+  int r = -1;
+  if (lane >= 2 && lane < 2 + kFOut && k < a.n) {
+    int rr = a.chunk_run[chunk];
+    while (rr < a.n_runs && a.run_k0[rr] + a.run_len[rr] <= k) ++rr;
+    if (rr < a.n_runs && a.run_k0[rr] <= k) r = rr;
+  }
+  if (r < 0) return;
+  const int32_t* d = a.run_d + 4 * (int64_t)r;
+  const int64_t kE = k + d[0], kW = k + d[1], kEE = k + d[2], kWW = k + d[3];
+  const T bE = a.bed[kE], bW = a.bed[kW], bEE = a.bed[kEE], bWW = a.bed[kWW];
+  const T hE = qin[0][kE], huE = qin[1][kE], hvE = qin[2][kE];
+  const T hW = qin[0][kW], huW = qin[1][kW], hvW = qin[2][kW];
+  T uE, vE, uW, vW;
+  prims(hE, huE, hvE, hm, uE, vE);
+  prims(hW, huW, hvW, hm, uW, vW);
+  const T gzxC = (T(0.5) * (bE - bW)) * ix;
+  const T gzxE = (T(0.5) * (bEE - bC)) * ix;
+  const T gzxW = (T(0.5) * (bC - bWW)) * ix;
+  Face<T> fr, fl;
+  face<T>(bC, bE, gzxC, gzxE, hC, hE, uC, vC, uE, vE, hdx, g, hm, fr);
+  face<T>(bW, bC, gzxW, gzxC, hW, hC, uW, vW, uC, vC, hdx, g, hm, fl);
+  T rh = T(0), rhu = T(0), rhv = T(0);
+  if (fr.gate) {
+    rh = rh - fr.Fh * ix;
+    rhu = rhu - fr.FN * ix;
+    rhv = rhv - fr.FT * ix;
+    rhu = rhu - fr.srcL * ix;
+  }
+  if (fl.gate) {
+    rh = rh + fl.Fh * ix;
+    rhu = rhu + fl.FN * ix;
+    rhv = rhv + fl.FT * ix;
+    rhu = rhu + fl.srcR * ix;
+  }
+  if (fy.gate) {
+    rh = rh - fy.Fh * iy;
+    rhu = rhu - fy.FT * iy;
+    rhv = rhv - fy.FN * iy;
+    rhv = rhv - fy.srcL * iy;
+  }
+  if (ygB) {
+    rh = rh + yFhB * iy;
+    rhu = rhu + yFTB * iy;
+    rhv = rhv + yFNB * iy;
+    rhv = rhv + ysRB * iy;
+  }
+  if (MODE == 0) {
+    a.rhs[0][k] = rh;
+    a.rhs[1][k] = rhu;
+    a.rhs[2][k] = rhv;
+    return;
+  }
+  const uint8_t fg = a.flags ? a.flags[k] : (uint8_t)0;
+  T rr0 = rh;
+  if (a.rain_mode == 1) {
+    rr0 = rr0 + x.rain_u;
+  } else if (a.rain_mode == 2) {
+    const int64_t ii = (int64_t)a.run_i[r] - a.ngh;
+    const int64_t jj = (int64_t)a.run_j0[r] + (k - a.run_k0[r]) - a.ngh;
+    rr0 = rr0 + x.frame[ii * a.rain_pitch + jj];
+  }
+  T n = T(0);
+  if (P.friction) n = a.n_field ? a.n_field[k] : x.ntab[fg & GS_FLAG_CLASS_MASK];
+  T h, hu, hv;
+  update<T>(hC, huC, hvC, rr0, rhu, rhv, x.dt, n, P, h, hu, hv);
+  if (fg & GS_FLAG_STAGE) {
+    const T hs = x.eta - bC;
+    h = hs > T(0) ? hs : T(0);
+    hu = T(0);
+    hv = T(0);
+  }
+  if (a.max_h && h > a.max_h[k]) a.max_h[k] = h;
+  x.qout[0][k] = h;
+  x.qout[1][k] = hu;
+  x.qout[2][k] = hv;
+  if (!(fg & GS_FLAG_CFL_SKIP)) {
+    const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
+    lam = lb_ > lam ? lb_ : lam;
+  }
+}
+
+// Workgroup b's place in an XCD-major order: workgroups are dealt round-robin to the 8
+// XCDs, so giving each XCD a contiguous range of wavefronts (hence of chunks in every
+// pass) keeps the rows i+-1 that a chunk gathers mostly in that XCD's L2.
+__device__ __forceinline__ unsigned xcd_rank(unsigned b, unsigned nb) {
+  const unsigned qd = nb / 8u, rm = nb % 8u, xcd = b % 8u, kk = b / 8u;
+  return (xcd < rm) ? xcd * (qd + 1u) + kk : rm * (qd + 1u) + (xcd - rm) * qd + kk;
+}
+
+// One step of one flat domain on a resident grid, each wavefront striding over chunks.
+template <typename T, int MODE>
+__global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
+  FlatCtx<T> x;
+  if (!flat_ctx<T, MODE>(a, x)) return;   // the whole grid sees the same clock
   __shared__ T s_ntab[32];
   if (MODE == 1 && threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
   __syncthreads();
-
-  T* const* qin = MODE == 1 ? a.buf[par] : a.buf[a.which];
-  T* const* qout = a.buf[par ^ 1];
-  const Phys<T>& P = a.P;
-  const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
-  const T* frame = (MODE == 1 && a.rain_mode == 2) ? a.rain_frames + (int64_t)ridx * a.frame_stride : nullptr;
-
+  x.ntab = s_ntab;
   const int lane = threadIdx.x & 63;
-  // Resident grid, each wavefront striding over chunks. Workgroups are dealt round-robin to
-  // the 8 XCDs; the remap gives each XCD a contiguous range of wavefronts, hence of chunks
-  // in every pass, so the rows i+-1 a chunk gathers are mostly in that XCD's L2.
-  const unsigned nb = gridDim.x, b = blockIdx.x;
-  const unsigned qd = nb / 8u, rm = nb % 8u, xcd = b % 8u, kk = b / 8u;
-  const unsigned lb = (xcd < rm) ? xcd * (qd + 1u) + kk : rm * (qd + 1u) + (xcd - rm) * qd + kk;
-  const int64_t nwaves = (int64_t)nb * (kFBlock / 64);
+  const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
   uint64_t lam = 0ull;
-
-  for (int64_t chunk = (int64_t)lb * (kFBlock / 64) + (threadIdx.x >> 6); chunk < a.nchunks; chunk += nwaves) {
-    const int64_t k = chunk * kFOut + lane - 2;
-    const int64_t kl = k < 0 ? 0 : (k >= a.n ? a.n - 1 : k);
-    // this lane's cell (needed by every lane for the y-face shuffles)
-    const T hC = qin[0][kl], huC = qin[1][kl], hvC = qin[2][kl];
-    const T bC = a.bed[kl];
-    T uC, vC;
-    prims(hC, huC, hvC, hm, uC, vC);
-    const T bl = fl_lower(bC), br = fl_upper(bC);
-    const T gzy = (T(0.5) * (br - bl)) * iy;
-    // upper y-face (k, k+1)
-    const T hR = fl_upper(hC), uR = fl_upper(uC), vR = fl_upper(vC);
-    const T gzyR = fl_upper(gzy);
-    Face<T> fy;
-    face<T>(bC, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
-    const T yFhB = fl_lower(fy.Fh), yFNB = fl_lower(fy.FN), yFTB = fl_lower(fy.FT);
-    const T ysRB = fl_lower(fy.srcR);
-    const int ygB = fl_lower((int)fy.gate);
-
-    // run of this cell
-    int r = -1;
-    if (lane >= 2 && lane < 2 + kFOut && k < a.n) {
-      int rr = a.chunk_run[chunk];
-      while (rr < a.n_runs && a.run_k0[rr] + a.run_len[rr] <= k) ++rr;
-      if (rr < a.n_runs && a.run_k0[rr] <= k) r = rr;
-    }
-    if (r >= 0) {
-      const int32_t* d = a.run_d + 4 * (int64_t)r;
-      const int64_t kE = k + d[0], kW = k + d[1], kEE = k + d[2], kWW = k + d[3];
-      const T bE = a.bed[kE], bW = a.bed[kW], bEE = a.bed[kEE], bWW = a.bed[kWW];
-      const T hE = qin[0][kE], huE = qin[1][kE], hvE = qin[2][kE];
-      const T hW = qin[0][kW], huW = qin[1][kW], hvW = qin[2][kW];
-      T uE, vE, uW, vW;
-      prims(hE, huE, hvE, hm, uE, vE);
-      prims(hW, huW, hvW, hm, uW, vW);
-      const T gzxC = (T(0.5) * (bE - bW)) * ix;
-      const T gzxE = (T(0.5) * (bEE - bC)) * ix;
-      const T gzxW = (T(0.5) * (bC - bWW)) * ix;
-      Face<T> fr, fl;
-      face<T>(bC, bE, gzxC, gzxE, hC, hE, uC, vC, uE, vE, hdx, g, hm, fr);
-      face<T>(bW, bC, gzxW, gzxC, hW, hC, uW, vW, uC, vC, hdx, g, hm, fl);
-      T rh = T(0), rhu = T(0), rhv = T(0);
-      if (fr.gate) {
-        rh = rh - fr.Fh * ix;
-        rhu = rhu - fr.FN * ix;
-        rhv = rhv - fr.FT * ix;
-        rhu = rhu - fr.srcL * ix;
-      }
-      if (fl.gate) {
-        rh = rh + fl.Fh * ix;
-        rhu = rhu + fl.FN * ix;
-        rhv = rhv + fl.FT * ix;
-        rhu = rhu + fl.srcR * ix;
-      }
-      if (fy.gate) {
-        rh = rh - fy.Fh * iy;
-        rhu = rhu - fy.FT * iy;
-        rhv = rhv - fy.FN * iy;
-        rhv = rhv - fy.srcL * iy;
-      }
-      if (ygB) {
-        rh = rh + yFhB * iy;
-        rhu = rhu + yFTB * iy;
-        rhv = rhv + yFNB * iy;
-        rhv = rhv + ysRB * iy;
-      }
-      if (MODE == 0) {
-        a.rhs[0][k] = rh;
-        a.rhs[1][k] = rhu;
-        a.rhs[2][k] = rhv;
-      } else {
-        const uint8_t fg = a.flags ? a.flags[k] : (uint8_t)0;
-        T rr0 = rh;
-        if (a.rain_mode == 1) {
-          rr0 = rr0 + rain_u;
-        } else if (a.rain_mode == 2) {
-          const int64_t ii = (int64_t)a.run_i[r] - a.ngh;
-          const int64_t jj = (int64_t)a.run_j0[r] + (k - a.run_k0[r]) - a.ngh;
-          rr0 = rr0 + frame[ii * a.rain_pitch + jj];
-        }
-        T n = T(0);
-        if (P.friction) n = a.n_field ? a.n_field[k] : s_ntab[fg & GS_FLAG_CLASS_MASK];
-        T h, hu, hv;
-        update<T>(hC, huC, hvC, rr0, rhu, rhv, dt, n, P, h, hu, hv);
-        if (fg & GS_FLAG_STAGE) {
-          const T hs = eta - bC;
-          h = hs > T(0) ? hs : T(0);
-          hu = T(0);
-          hv = T(0);
-        }
-        if (a.max_h && h > a.max_h[k]) a.max_h[k] = h;
-        qout[0][k] = h;
-        qout[1][k] = hu;
-        qout[2][k] = hv;
-        if (!(fg & GS_FLAG_CFL_SKIP)) {
-          const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
-          lam = lb_ > lam ? lb_ : lam;
-        }
-      }
-    }
-  }
+  for (int64_t chunk = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + (threadIdx.x >> 6);
+       chunk < a.nchunks; chunk += nwaves)
+    flat_chunk<T, MODE>(a, x, chunk, lane, lam);
   if (MODE == 1) grid_reduce_and_plan(lam, a.clock, a.plan, true);
 }
 
@@ -325,6 +357,95 @@ static int flat_cfl_t(const GsFlat* f, const GsPhys* p, GsForcing fo, GsClock* c
   return GS_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// batched ensemble: every member's chunks in one launch, per-member clocks
+// ---------------------------------------------------------------------------
+template <typename T>
+struct EnsK {
+  const FlatK<T>* __restrict__ mem;   // member views (device)
+  const int64_t* __restrict__ chunk0; // [n_members + 1] first global chunk of each member
+  int n_members;
+  int cpp;                            // consecutive chunks per wavefront pass
+  int64_t nchunks;                    // total chunks
+  unsigned* done;                     // workgroups finished in this launch
+};
+
+// Fold a wavefront's CFL speed bits of member m into that member's clock.
+template <typename T>
+__device__ __forceinline__ void ens_flush(const EnsK<T>& e, int m, uint64_t lam) {
+  lam = wave_max_u64(lam);
+  if (m >= 0 && (threadIdx.x & 63) == 0 && lam)
+    __hip_atomic_fetch_max(&e.mem[m].clock->lam_acc, lam, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One step of every live member. The global chunk space is the members' chunks end to
+// end; a wavefront takes `cpp` consecutive chunks per pass and strides by the grid, so a
+// member boundary is crossed at most a few times per wavefront and the member view, its
+// clock values and the wave's speed maximum change only there. The last workgroup to
+// finish plans each member's next dt from its own maximum (plan_finish), as the plain
+// step's last workgroup does for its single domain.
+template <typename T>
+__global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t wave = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv;
+  const int64_t cpp = e.cpp;
+  int m = -1;
+  int64_t c0 = 0, c1 = 0;
+  bool live = false;
+  FlatCtx<T> x;
+  x.qin = nullptr;
+  x.qout = nullptr;
+  x.dt = x.rain_u = x.eta = T(0);
+  x.frame = nullptr;
+  x.ntab = nullptr;
+  uint64_t lam = 0ull;
+  for (int64_t base = wave * cpp; base < e.nchunks; base += nwaves * cpp) {
+    const int64_t stop = base + cpp < e.nchunks ? base + cpp : e.nchunks;
+    for (int64_t gc = base; gc < stop; ++gc) {
+      if (gc >= c1) {
+        ens_flush(e, m, lam);
+        lam = 0ull;
+        do {
+          ++m;
+          c0 = e.chunk0[m];
+          c1 = e.chunk0[m + 1];
+        } while (gc >= c1);
+        live = flat_ctx<T, 1>(e.mem[m], x);
+        x.ntab = (const T*)e.mem[m].ntab;
+      }
+      if (live) flat_chunk<T, 1>(e.mem[m], x, gc - c0, lane, lam);
+    }
+  }
+  ens_flush(e, m, lam);
+  __threadfence();   // every wave's maxima are performed before the hand-off below
+  __syncthreads();
+  __shared__ int s_last;
+  if (threadIdx.x == 0) {
+    const unsigned prev = __hip_atomic_fetch_add(e.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (prev == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  for (int mm = threadIdx.x; mm < e.n_members; mm += blockDim.x) {
+    GsClock* c = e.mem[mm].clock;
+    if (c->status & GS_ST_DONE) continue;
+    const uint64_t tot = __hip_atomic_load(&c->lam_acc, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    plan_finish(c, e.mem[mm].plan, tot, true);
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(e.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+static int ens_occupancy() {
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ens_step<T>, kFBlock, 0) != hipSuccess || occ < 1)
+    occ = 1;
+  return occ;
+}
+
 }  // namespace gs
 
 using namespace gs;
@@ -351,6 +472,310 @@ int gs_flat_cfl(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* cl
   if (f->dtype == GS_F32) return flat_cfl_t<float>(f, phys, forc, clock_dev, (hipStream_t)stream);
   if (f->dtype == GS_F64) return flat_cfl_t<double>(f, phys, forc, clock_dev, (hipStream_t)stream);
   return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
+}
+
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ensemble host object
+// ---------------------------------------------------------------------------
+struct GsEns {
+  int dtype = 0;
+  int n = 0;
+  void* mem = nullptr;          // device FlatK<T>[n]
+  int64_t* chunk0 = nullptr;    // device [n + 1]
+  unsigned* done = nullptr;     // device counter
+  int64_t nchunks = 0;
+  int blocks = 0;               // 0: one occupancy wave of the device
+  int cpp = 1;
+  int cus = 256;
+  int occ = 1;                  // resident k_ens_step workgroups per CU
+  std::vector<GsFlat> flat;
+  std::vector<GsClock*> clocks;
+  hipStream_t gstream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int g_batch = 0, g_blocks = -1, g_cpp = -1;
+};
+
+namespace gs {
+
+template <typename T>
+static int ens_build_views(GsEns* e, const GsFlat* members, const GsPhys* phys, const GsForcing* forcings) {
+  std::vector<FlatK<T>> views((size_t)e->n);
+  std::vector<int64_t> c0((size_t)e->n + 1, 0);
+  for (int m = 0; m < e->n; ++m) {
+    const int rc = build_flat<T>(&members[m], &phys[m], forcings[m], e->clocks[m], views[m], true);
+    if (rc) {
+      char msg[512];
+      snprintf(msg, sizeof msg, "%s", g_err);
+      return fail(rc, "member %d: %s", m, msg);
+    }
+    c0[m + 1] = c0[m] + views[m].nchunks;
+  }
+  e->nchunks = c0[e->n];
+  GS_HIP(hipMalloc(&e->mem, sizeof(FlatK<T>) * (size_t)e->n), "ensemble views");
+  GS_HIP(hipMalloc((void**)&e->chunk0, sizeof(int64_t) * ((size_t)e->n + 1)), "ensemble chunk table");
+  GS_HIP(hipMalloc((void**)&e->done, 256), "ensemble counter");
+  GS_HIP(hipMemcpy(e->mem, views.data(), sizeof(FlatK<T>) * (size_t)e->n, hipMemcpyHostToDevice), "views h2d");
+  GS_HIP(hipMemcpy(e->chunk0, c0.data(), sizeof(int64_t) * ((size_t)e->n + 1), hipMemcpyHostToDevice),
+         "chunk table h2d");
+  GS_HIP(hipMemset(e->done, 0, 256), "counter clear");
+  return GS_OK;
+}
+
+template <typename T>
+static int ens_launch(GsEns* e, hipStream_t s) {
+  EnsK<T> k;
+  k.mem = (const FlatK<T>*)e->mem;
+  k.chunk0 = e->chunk0;
+  k.n_members = e->n;
+  k.cpp = e->cpp;
+  k.nchunks = e->nchunks;
+  k.done = e->done;
+  int64_t blocks = e->blocks > 0 ? e->blocks : (int64_t)e->occ * e->cus;
+  const int64_t need = (e->nchunks + 4LL * e->cpp - 1) / (4LL * e->cpp);
+  if (blocks > need) blocks = need;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL((k_ens_step<T>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k);
+  GS_HIP(hipGetLastError(), "k_ens_step launch");
+  return GS_OK;
+}
+
+static int ens_step(GsEns* e, hipStream_t s) {
+  return e->dtype == GS_F64 ? ens_launch<double>(e, s) : ens_launch<float>(e, s);
+}
+
+static void ens_drop_graph(GsEns* e) {
+  if (e->exec) hipGraphExecDestroy(e->exec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  e->g_batch = 0;
+}
+
+static int ens_capture(GsEns* e, int batch) {
+  ens_drop_graph(e);
+  GS_HIP(hipStreamBeginCapture(e->gstream, hipStreamCaptureModeThreadLocal), "begin capture");
+  int rc = GS_OK;
+  for (int i = 0; i < batch && rc == GS_OK; ++i) rc = ens_step(e, e->gstream);
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(e->gstream, &g);
+  if (rc) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (ee != hipSuccess) return hip_fail(ee, "end capture");
+  e->graph = g;
+  GS_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0), "graph instantiate");
+  e->g_batch = batch;
+  e->g_blocks = e->blocks;
+  e->g_cpp = e->cpp;
+  return GS_OK;
+}
+
+}  // namespace gs
+
+extern "C" {
+
+int gs_ens_create(const GsFlat* members, const GsPhys* phys, const GsForcing* forcings, GsClock* const* clocks,
+                  int n_members, GsEns** out) {
+  using namespace gs;
+  if (!out) return fail(GS_E_ARG, "out is NULL");
+  *out = nullptr;
+  if (n_members < 1 || n_members > (1 << 20)) return fail(GS_E_RANGE, "n_members must be in 1..2^20");
+  if (!members || !phys || !forcings || !clocks) return fail(GS_E_ARG, "member arrays are NULL");
+  const int dt = members[0].dtype;
+  if (dt != GS_F32 && dt != GS_F64) return fail(GS_E_DTYPE, "unsupported dtype %d", dt);
+  for (int m = 0; m < n_members; ++m) {
+    if (members[m].dtype != dt) return fail(GS_E_DTYPE, "member %d dtype differs from member 0", m);
+    if (!clocks[m]) return fail(GS_E_ARG, "clock of member %d is NULL", m);
+  }
+  GsEns* e = new GsEns();
+  e->dtype = dt;
+  e->n = n_members;
+  e->flat.assign(members, members + n_members);
+  e->clocks.assign(clocks, clocks + n_members);
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e->cus < 1) e->cus = 256;
+  e->occ = dt == GS_F64 ? ens_occupancy<double>() : ens_occupancy<float>();
+  int rc = dt == GS_F64 ? ens_build_views<double>(e, members, phys, forcings)
+                        : ens_build_views<float>(e, members, phys, forcings);
+  if (rc == GS_OK) {
+    hipError_t h = hipStreamCreateWithFlags(&e->gstream, hipStreamNonBlocking);
+    if (h == hipSuccess) h = hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming);
+    if (h == hipSuccess) h = hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming);
+    if (h != hipSuccess) rc = hip_fail(h, "ensemble stream/events");
+  }
+  if (rc) {
+    gs_ens_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return GS_OK;
+}
+
+int gs_ens_destroy(GsEns* e) {
+  if (!e) return GS_OK;
+  gs::ens_drop_graph(e);
+  if (e->gstream) hipStreamSynchronize(e->gstream);
+  if (e->ev_in) hipEventDestroy(e->ev_in);
+  if (e->ev_out) hipEventDestroy(e->ev_out);
+  if (e->gstream) hipStreamDestroy(e->gstream);
+  if (e->mem) hipFree(e->mem);
+  if (e->chunk0) hipFree(e->chunk0);
+  if (e->done) hipFree(e->done);
+  delete e;
+  return GS_OK;
+}
+
+int gs_ens_set_launch(GsEns* e, int blocks, int chunks_per_pass) {
+  using namespace gs;
+  if (!e) return fail(GS_E_ARG, "ensemble is NULL");
+  if (blocks < 0 || blocks > (1 << 24)) return fail(GS_E_RANGE, "blocks must be in 0..2^24");
+  if (chunks_per_pass < 1 || chunks_per_pass > 64) return fail(GS_E_RANGE, "chunks_per_pass must be in 1..64");
+  e->blocks = blocks;
+  e->cpp = chunks_per_pass;
+  return GS_OK;
+}
+
+int gs_ens_get_launch(const GsEns* e, int* blocks, int* chunks_per_pass, int64_t* total_chunks) {
+  using namespace gs;
+  if (!e) return fail(GS_E_ARG, "ensemble is NULL");
+  if (blocks) {
+    int b = e->blocks;
+    if (b == 0) b = e->occ * e->cus;
+    *blocks = b;
+  }
+  if (chunks_per_pass) *chunks_per_pass = e->cpp;
+  if (total_chunks) *total_chunks = e->nchunks;
+  return GS_OK;
+}
+
+int gs_ens_step(GsEns* e, void* stream) {
+  using namespace gs;
+  if (!e) return fail(GS_E_ARG, "ensemble is NULL");
+  return ens_step(e, (hipStream_t)stream);
+}
+
+int gs_ens_run(GsEns* e, int64_t nsteps, int graph_batch, void* stream) {
+  using namespace gs;
+  if (!e) return fail(GS_E_ARG, "ensemble is NULL");
+  if (nsteps < 0) return fail(GS_E_ARG, "nsteps < 0");
+  hipStream_t s = (hipStream_t)stream;
+  if (graph_batch <= 0 || nsteps < graph_batch) {
+    for (int64_t i = 0; i < nsteps; ++i) {
+      const int rc = ens_step(e, s);
+      if (rc) return rc;
+    }
+    return GS_OK;
+  }
+  if (graph_batch > 100000) return fail(GS_E_RANGE, "graph_batch too large");
+  GS_HIP(hipEventRecord(e->ev_in, s), "event record");
+  GS_HIP(hipStreamWaitEvent(e->gstream, e->ev_in, 0), "stream wait");
+  if (!e->exec || e->g_batch != graph_batch || e->g_blocks != e->blocks || e->g_cpp != e->cpp) {
+    const int rc = ens_capture(e, graph_batch);
+    if (rc) return rc;
+  }
+  const int64_t reps = nsteps / graph_batch, rest = nsteps % graph_batch;
+  for (int64_t r = 0; r < reps; ++r) GS_HIP(hipGraphLaunch(e->exec, e->gstream), "graph launch");
+  for (int64_t i = 0; i < rest; ++i) {
+    const int rc = ens_step(e, e->gstream);
+    if (rc) return rc;
+  }
+  GS_HIP(hipEventRecord(e->ev_out, e->gstream), "event record");
+  GS_HIP(hipStreamWaitEvent(s, e->ev_out, 0), "stream wait");
+  return GS_OK;
+}
+
+int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
+  using namespace gs;
+  if (!e) return fail(GS_E_ARG, "ensemble is NULL");
+  if (trial_steps < 1 || trial_steps > 10000) return fail(GS_E_RANGE, "trial_steps must be in 1..10000");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t es = e->dtype == GS_F64 ? 8 : 4;
+  GS_HIP(hipStreamSynchronize(s), "sync");
+  // snapshot: the current state buffer, the running max and the clock of every member
+  std::vector<GsClock> clk((size_t)e->n);
+  std::vector<void*> snap;
+  int rc = GS_OK;
+  auto cleanup = [&]() {
+    for (void* p : snap) hipFree(p);
+    snap.clear();
+  };
+  for (int m = 0; m < e->n && rc == GS_OK; ++m) {
+    hipError_t h = hipMemcpy(&clk[m], e->clocks[m], sizeof(GsClock), hipMemcpyDeviceToHost);
+    if (h != hipSuccess) { rc = hip_fail(h, "clock d2h"); break; }
+    const int par = (int)(clk[m].nsteps & 1);
+    const size_t bytes = (size_t)e->flat[m].n_stored * es;
+    for (int k = 0; k < 4 && rc == GS_OK; ++k) {
+      const void* src = k < 3 ? e->flat[m].q[par][k] : e->flat[m].max_h;
+      void* p = nullptr;
+      if (src) {
+        h = hipMalloc(&p, bytes);
+        if (h == hipSuccess) h = hipMemcpy(p, src, bytes, hipMemcpyDeviceToDevice);
+        if (h != hipSuccess) rc = hip_fail(h, "autotune snapshot");
+      }
+      snap.push_back(p);
+    }
+  }
+  if (rc) { cleanup(); return rc; }
+  auto restore = [&]() -> int {
+    for (int m = 0; m < e->n; ++m) {
+      const int par = (int)(clk[m].nsteps & 1);
+      const size_t bytes = (size_t)e->flat[m].n_stored * es;
+      for (int k = 0; k < 4; ++k) {
+        void* dst = k < 3 ? e->flat[m].q[par][k] : e->flat[m].max_h;
+        void* p = snap[(size_t)m * 4 + k];
+        if (dst && p) GS_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToDevice, s), "restore");
+      }
+      GS_HIP(hipMemcpyAsync(e->clocks[m], &clk[m], sizeof(GsClock), hipMemcpyHostToDevice, s), "clock restore");
+    }
+    GS_HIP(hipStreamSynchronize(s), "sync");
+    return GS_OK;
+  };
+  const int occ = e->occ;
+  int per_cu[4] = {1, 2, occ > 1 ? occ / 2 : 1, occ};
+  const int cpps[4] = {1, 2, 4, 8};
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  hipEventCreate(&t0);
+  hipEventCreate(&t1);
+  double best = 1e300;
+  int best_b = e->blocks, best_c = e->cpp;
+  for (int a = 0; a < 4 && rc == GS_OK; ++a) {
+    bool dup = false;
+    for (int b = 0; b < a; ++b) dup = dup || per_cu[b] == per_cu[a];
+    if (dup || per_cu[a] > occ) continue;
+    for (int c = 0; c < 4 && rc == GS_OK; ++c) {
+      e->blocks = per_cu[a] * e->cus;
+      e->cpp = cpps[c];
+      rc = restore();
+      if (!rc) rc = ens_step(e, s);   // warm-up
+      if (!rc && hipEventRecord(t0, s) != hipSuccess) rc = fail(GS_E_STATE, "event record failed");
+      for (int i = 0; i < trial_steps && !rc; ++i) rc = ens_step(e, s);
+      if (!rc && hipEventRecord(t1, s) != hipSuccess) rc = fail(GS_E_STATE, "event record failed");
+      if (!rc && hipEventSynchronize(t1) != hipSuccess) rc = fail(GS_E_STATE, "event sync failed");
+      float ms = 0.0f;
+      if (!rc && hipEventElapsedTime(&ms, t0, t1) == hipSuccess && ms / trial_steps < best) {
+        best = ms / trial_steps;
+        best_b = e->blocks;
+        best_c = e->cpp;
+      }
+    }
+  }
+  hipEventDestroy(t0);
+  hipEventDestroy(t1);
+  e->blocks = best_b;
+  e->cpp = best_c;
+  const int rc2 = restore();
+  cleanup();
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (best_ms) *best_ms = best;
+  return GS_OK;
 }
 
 }  // extern "C"

@@ -162,7 +162,7 @@ typedef struct GsFlat {
   const int32_t* run_i;      /* [n_runs] padded row of the run */
   const int32_t* run_j0;     /* [n_runs] padded column of the first cell */
   const int32_t* run_d;      /* [n_runs][4] id offsets of rows i+1, i-1, i+2, i-2 */
-  const int32_t* chunk_run;  /* [ceil(N/64)] first run whose end is past the chunk start */
+  const int32_t* chunk_run;  /* [ceil(N/60)] first run whose end is past the chunk start */
 } GsFlat;
 
 /* ---- library ---- */
@@ -238,6 +238,36 @@ int gs_flat_residual(const GsFlat* f, const GsPhys* phys, int which, void* rhs0,
 /* Replaces: cfl_lammax_flat_blk + CompressedStepper.cfl_dt (compressed_solver.py:200-240, 721-748). */
 int gs_flat_cfl(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev,
                 void* stream);
+
+/* ---- batched ensemble of flat domains ---- */
+/*
+ * Independent flat domains (members of uneven size, each with its own bed, Manning table,
+ * rain, stage and clock) advanced together: one launch per step over all members' chunks,
+ * each member with its own CFL dt planned on the device, steps replayed from a hipGraph.
+ * No reference counterpart: the reference runs members one after another through
+ * CompressedSolver.run (compressed_solver.py:3359-3450). Member k's results are bitwise
+ * those of gs_flat_step on the same member.
+ */
+typedef struct GsEns GsEns;
+/* Build an ensemble over n_members flat descriptors (device buffers stay owned by the
+ * caller and must outlive the ensemble). All members share one dtype. phys/forcings/clocks
+ * are arrays of n_members. The clocks should already hold each member's first dt
+ * (gs_flat_cfl + gs_clock_set_target), as for gs_flat_step. */
+int gs_ens_create(const GsFlat* members, const GsPhys* phys, const GsForcing* forcings,
+                  GsClock* const* clocks, int n_members, GsEns** out);
+int gs_ens_destroy(GsEns* e);
+/* Launch shape: resident workgroups (0 = one full occupancy wave of the device) and the
+ * consecutive chunks of 60 flat ids a wavefront takes per pass (1..64). */
+int gs_ens_set_launch(GsEns* e, int blocks, int chunks_per_pass);
+int gs_ens_get_launch(const GsEns* e, int* blocks, int* chunks_per_pass, int64_t* total_chunks);
+/* One step of every member that is not done. */
+int gs_ens_step(GsEns* e, void* stream);
+/* nsteps steps; graph_batch > 0 replays a captured hipGraph of graph_batch steps. */
+int gs_ens_run(GsEns* e, int64_t nsteps, int graph_batch, void* stream);
+/* Time candidate launch shapes (trial_steps steps each, on a snapshot of the state that is
+ * restored afterwards), keep the fastest; writes its mean ms per step. Allocates scratch
+ * for the snapshot. */
+int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms);
 
 /* ---- set-up and measurement utilities ---- */
 /* Constant fill of a (rows, cols) plane with row pitch `pitch` (elements). No reference

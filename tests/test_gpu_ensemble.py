@@ -1,0 +1,83 @@
+"""Batched ensemble of uneven flat domains: every member bitwise equal to its own plain
+CompressedSolver run (same steps, same per-member dt sequence), through direct launches,
+hipGraph replays, a t_end run and after the launch-shape autotuner."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SIDES = [64, 192, 96, 128, 64]
+
+
+@pytest.fixture(scope="module")
+def cases(gpu):
+    from geoswe_amd.cases import ensemble_c4
+    return ensemble_c4(sides=SIDES, rain_frames=2)
+
+
+def flats(cases):
+    return [c.flat(track_max_depth=True) for c in cases]
+
+
+def assert_same(plain, ens_members):
+    for k, (p, e) in enumerate(zip(plain, ens_members)):
+        assert p.nsteps == e.nsteps, k
+        assert p.t == e.t, k
+        assert np.array_equal(p.q_flat(), e.q_flat()), k
+        assert np.array_equal(p.max_depth(), e.max_depth()), k
+
+
+def test_members_differ(cases):
+    """Per-member rain and roughness really differ, so per-member dt does too."""
+    assert len({float(np.asarray(c.cfg.rainfall_forcing.rate_mm_h).max()) for c in cases}) == len(cases)
+    assert len({tuple(np.round(c.ntab, 6)) for c in cases}) == len(cases)
+
+
+@pytest.mark.parametrize("graph_batch", [0, 16])
+def test_ensemble_equals_plain_runs(cases, graph_batch):
+    from geoswe_amd.ensemble import Ensemble
+    plain = flats(cases)
+    for p in plain:
+        p.advance(40)
+    mem = flats(cases)
+    ens = Ensemble(mem, graph_batch=graph_batch)
+    ens.advance(40)
+    assert_same(plain, mem)
+    assert len({m.t for m in mem}) == len(mem)       # per-member dt
+    ens.advance(7)                                    # graph replays resume after a partial batch
+    for p in plain:
+        p.advance(7)
+    assert_same(plain, mem)
+    ens.close()
+
+
+def test_ensemble_launch_shapes_and_autotune(cases):
+    from geoswe_amd.ensemble import Ensemble
+    plain = flats(cases)
+    for p in plain:
+        p.advance(30)
+    mem = flats(cases)
+    ens = Ensemble(mem, graph_batch=8)
+    got = ens.autotune(trial_steps=2)                 # restores the state it timed on
+    assert got["blocks"] >= 1 and 1 <= got["chunks_per_pass"] <= 8
+    assert all(m.nsteps == 0 for m in mem)
+    ens.advance(10)
+    ens.set_launch(blocks=3, chunks_per_pass=5)       # a small, odd grid: every chunk still visited
+    ens.advance(10)
+    ens.set_launch(blocks=0, chunks_per_pass=1)
+    ens.advance(10, graph_batch=4)
+    assert_same(plain, mem)
+    ens.close()
+
+
+def test_ensemble_run_to_t_end(cases):
+    from geoswe_amd.ensemble import Ensemble
+    plain = flats(cases)
+    steps_p = [p.run(t_end=12.5) for p in plain]
+    mem = flats(cases)
+    ens = Ensemble(mem, graph_batch=8)
+    steps_e = ens.run(t_end=12.5, batch=16)
+    assert steps_p == steps_e
+    assert all(abs(m.t - 12.5) < 1e-9 for m in mem)
+    assert_same(plain, mem)
+    ens.close()
