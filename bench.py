@@ -182,6 +182,282 @@ def coastal(args):
         sys.exit(1)
 
 
+def ensemble(args):
+    """The batched ensemble: seeded uneven coastal members stepped one member after another
+    (plain CompressedSolver runs) and together (one launch per step, hipGraph replay, the
+    autotuned launch shape); every member compared bit for bit. One JSON line."""
+    from geoswe_amd import _lib as L
+    from geoswe_amd.cases import ensemble_c4, ensemble_sides
+    from geoswe_amd.ensemble import Ensemble
+    sides = ensemble_sides(args.members)
+    t0 = time.perf_counter()
+    log(f"building {len(sides)} members, sides {min(sides)}..{max(sides)}, {sum(x * x for x in sides) / 1e6:.1f} M "
+        f"box cells")
+    cases = ensemble_c4(sides=sides)
+    plain = [c.flat(track_max_depth=False) for c in cases]
+    mem = [c.flat(track_max_depth=False) for c in cases]
+    del cases
+    act = sum(m.n_active for m in mem)
+    log(f"built in {time.perf_counter() - t0:.0f} s; {act / 1e6:.1f} M active cells")
+    steps = args.steps
+    for m in plain:                    # first dt and set-up outside the timed region
+        m._ensure_lam()
+        m._open_mode()
+    L.sync()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    L.call("gs_event_create", C.byref(e0))
+    L.call("gs_event_create", C.byref(e1))
+    w0 = time.perf_counter()
+    L.call("gs_event_record", e0, None)
+    for m in plain:
+        m.advance(steps, sync=False)
+    L.call("gs_event_record", e1, None)
+    L.sync()
+    w_plain = time.perf_counter() - w0
+    ms = C.c_float()
+    L.call("gs_event_elapsed_ms", C.byref(ms), e0, e1)
+    ev_plain = ms.value
+    log(f"sequential plain runs: {w_plain:.2f} s wall for {steps} steps x {len(plain)} members")
+    ens = Ensemble(mem, graph_batch=args.graph_batch)
+    tune = ens.autotune(trial_steps=3)
+    log(f"autotuned launch: {tune}")
+    ens.advance(1, graph_batch=0)       # first launch outside the timed region
+    L.sync()
+    w0 = time.perf_counter()
+    L.call("gs_event_record", e0, None)
+    ens.advance(steps - 1, sync=False)
+    L.call("gs_event_record", e1, None)
+    L.sync()
+    w_ens = (time.perf_counter() - w0) * steps / (steps - 1)
+    L.call("gs_event_elapsed_ms", C.byref(ms), e0, e1)
+    ev_ens = ms.value * steps / (steps - 1)
+    same = all(a.nsteps == b.nsteps and a.t == b.t and np.array_equal(a.q_flat(), b.q_flat())
+               for a, b in zip(plain, mem))
+    dts = sorted({round(m.t / m.nsteps, 6) for m in mem})
+    v_plain = act * steps / w_plain
+    v_ens = act * steps / w_ens
+    log(f"ensemble: {w_ens:.2f} s wall; {v_ens / 1e9:.2f} vs {v_plain / 1e9:.2f} G cell-updates/s "
+        f"({v_ens / v_plain:.2f}x); members bitwise equal to their plain runs: {same}")
+    # the reference's plain loop: one host read-back of the step's dt per step
+    # (compressed_solver.py _step_loop: lambda read-back after every fused step)
+    w0 = time.perf_counter()
+    for m in plain:
+        for _ in range(steps):
+            m.step()
+    L.sync()
+    w_sync = time.perf_counter() - w0
+    v_sync = act * steps / w_sync
+    log(f"sequential runs with a per-step dt read-back: {w_sync:.2f} s wall, {v_sync / 1e9:.2f} G cell-updates/s "
+        f"(ensemble {v_ens / v_sync:.2f}x)")
+    achieved = 33.0 * act * steps / (ev_ens * 1e-3) / 1e9
+    out = {
+        "metric": "cell-updates/s (batched ensemble of flat domains, 1 MI355X)",
+        "value": v_ens, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": 1e3 * w_ens / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded coastal members: fBm DEM, Voronoi classes, fBm rain, tide)",
+        "config": {"workload": f"{len(sides)} coastal members, sides {min(sides)}..{max(sides)} (log-uniform), "
+                               f"{act} active cells, rain x U[0.5,2], n x (1+U[-0.3,0.3]), per-member CFL dt",
+                   "members": len(sides), "active_cells": int(act), "box_cells": int(sum(x * x for x in sides))},
+        "sequential_plain_value": v_plain, "speedup_vs_sequential": v_ens / v_plain,
+        "sequential_readback_value": v_sync, "speedup_vs_sequential_readback": v_ens / v_sync,
+        "sequential_plain_s": w_plain, "ensemble_s": w_ens, "event_ms_plain": ev_plain, "event_ms_ensemble": ev_ens,
+        "launch": tune, "members_bitwise_equal_plain": same, "distinct_member_dt": len(dts),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gs::k_ens_step<float>",
+                     "bytes_per_cell": 33.0},
+    }
+    print(json.dumps(out), flush=True)
+    if not same:
+        sys.exit(1)
+
+
+def adjoint(args):
+    """Gauge-misfit calibration gradients on the coastal case: forward and adjoint cost,
+    central finite differences (eps 1e-3) on every class n and on seeded bed cells near the
+    gauges, and float64 torch autograd on a small coastal case. One JSON line."""
+    from geoswe_amd import _lib as L
+    from geoswe_amd.adjoint import GaugeMisfit
+    from geoswe_amd.cases import coastal_c3
+    n, S, K = args.n_adjoint, args.steps, args.checkpoint
+    t0 = time.perf_counter()
+    case = coastal_c3(n=n)
+    ref = case.flat(track_max_depth=False)
+    ref._ensure_lam()
+    ref._open_mode()
+    dts = []
+    for _ in range(S):
+        dts.append(ref.step())
+    del ref
+    f = case.flat(track_max_depth=False)
+    fm = f.fm
+    h0 = f.q_flat()[0]
+    rng = np.random.default_rng(20261205)
+    gauges = np.sort(rng.choice(np.flatnonzero(fm.active & (h0 > 0.05)), size=32, replace=False))
+    prob = GaugeMisfit(f, gauges, np.array(dts), checkpoint_every=K)
+    n0, b0 = prob.params()
+    n_true = n0 * (1.0 + 0.3 * rng.uniform(-1.0, 1.0, len(n0)))
+    prob.set_params(ntab=n_true)
+    prob.h_star = prob.simulate(keep_checkpoints=False).copy()
+    prob.set_params(ntab=n0)
+    log(f"built {n}x{n} ({fm.n_active / 1e6:.2f} M active), {S} steps to t={sum(dts):.1f} s in "
+        f"{time.perf_counter() - t0:.0f} s")
+    prob.simulate()
+    L.sync()
+    w0 = time.perf_counter()
+    prob.simulate()
+    t_fwd = time.perf_counter() - w0
+    J0 = prob.loss()
+    w0 = time.perf_counter()
+    gn, gb = prob.gradient()
+    t_adj = time.perf_counter() - w0
+    log(f"J = {J0:.6e}; forward {1e3 * t_fwd / S:.3f} ms/step, gradient {1e3 * t_adj / S:.3f} ms/step "
+        f"({t_adj / t_fwd:.2f}x forward)")
+
+    def J(nn=None, bb=None):
+        prob.set_params(ntab=n0 if nn is None else nn, bed=b0 if bb is None else bb)
+        prob.simulate(keep_checkpoints=False)
+        return prob.loss()
+
+    eps = 1e-3
+    fd_n = np.zeros_like(n0)
+    for k in range(len(n0)):
+        e = np.zeros_like(n0)
+        e[k] = eps
+        fd_n[k] = (J(nn=n0 + e) - J(nn=n0 - e)) / (2 * eps)
+    # 64 seeded bed cells among active cells within 4 cells of a gauge
+    gij = fm.ij[gauges]
+    near = np.flatnonzero(fm.active & (np.abs(fm.ij[:, None, :] - gij[None]).max(axis=2).min(axis=1) <= 4))
+    cells = np.sort(rng.choice(near, size=min(64, len(near)), replace=False))
+    fd_b = np.zeros(len(cells))
+    for t_, c in enumerate(cells):
+        e = np.zeros_like(b0)
+        e[c] = eps
+        fd_b[t_] = (J(bb=b0 + e) - J(bb=b0 - e)) / (2 * eps)
+    prob.set_params(ntab=n0, bed=b0)
+    rel_n = np.abs(gn - fd_n) / np.maximum(np.abs(fd_n), 1e-12)
+    ab = gb[cells]
+    big = np.abs(fd_b) >= 1e-2 * np.abs(fd_b).max()
+    rel_b = np.abs(ab - fd_b)[big] / np.abs(fd_b)[big]
+    log(f"FD: n rel err max {rel_n.max():.2e} (median {np.median(rel_n):.2e}); bed rel err over the "
+        f"{int(big.sum())} cells with |g| >= 1% of max: max {rel_b.max():.2e}, median {np.median(rel_b):.2e}; "
+        f"norm-wise {np.linalg.norm(ab - fd_b) / np.linalg.norm(fd_b):.2e}")
+    torch_err = None
+    try:
+        torch_err = torch_check()
+        log(f"float64 torch autograd check (128x128, 200 steps): {torch_err}")
+    except ImportError:
+        log("torch not importable: float64 autograd check skipped")
+    out = {
+        "metric": "adjoint step cost / forward step cost (gauge-misfit gradient, 1 MI355X)",
+        "value": t_adj / t_fwd, "unit": "x forward", "n_gpus": 1, "steps": S, "warmup": 1,
+        "ms_per_step": 1e3 * t_adj / S, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (coastal generator, truth run with perturbed Manning classes)",
+        "config": {"workload": f"coastal {n}x{n} flat, {fm.n_active} active cells, {S} steps, checkpoint every "
+                               f"{K}, 32 gauges, gradients w.r.t. {len(n0)} class n and {fm.n_stored} bed cells"},
+        "forward_ms_per_step": 1e3 * t_fwd / S, "adjoint_ms_per_step": 1e3 * t_adj / S,
+        "forward_cell_updates_per_s": fm.n_active * S / t_fwd,
+        "fd_eps": eps, "fd_rel_err_n_max": float(rel_n.max()), "fd_rel_err_n": rel_n.tolist(),
+        "fd_rel_err_bed_max": float(rel_b.max()), "fd_rel_err_bed_median": float(np.median(rel_b)),
+        "fd_bed_cells_compared": int(big.sum()),
+        "fd_rel_err_bed_normwise": float(np.linalg.norm(ab - fd_b) / np.linalg.norm(fd_b)),
+        "torch_f64_autograd": torch_err,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def torch_check():
+    """Relative gradient differences of the float64 device adjoint against torch autograd
+    through the float64 torch restatement of the step (tests/torch_ref.py), 128x128, 200 steps."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import torch
+    import adjoint_cases as AC
+    case, prob = AC.coastal_problem(n=128, S=200, K=50, dtype="float64", n_gauges=32)
+    n0, b0 = prob.params()
+    prob.simulate()
+    gn, gb = prob.gradient()
+    nt = torch.tensor(n0, requires_grad=True)
+    bt = torch.tensor(b0, requires_grad=True)
+    Jt, _ = AC.torch_misfit(case, prob, nt, bt)
+    Jt.backward()
+    tn, tb = nt.grad.numpy(), bt.grad.numpy()
+    return {"n_max_rel": float(np.abs(gn - tn).max() / np.abs(tn).max()),
+            "bed_normwise_rel": float(np.linalg.norm(gb - tb) / np.linalg.norm(tb)),
+            "bed_max_rel": float(np.abs(gb - tb).max() / np.abs(tb).max())}
+
+
+def dambreak(args):
+    """The 256x256 circular dam break (walls, no friction, no rain), 2000 steps of
+    step(cfl_dt()): the float32 GPU run against the float64 GPU run and against the float64
+    NumPy oracle run; max |dh|, relative L2 of h, hu, hv at steps 1, 100 and the last, the
+    final time and the dt sequences. The float32 oracle is timed on the same run (one
+    thread) as the CPU baseline. One JSON line."""
+    from geoswe_amd import Solver2D
+    from geoswe_amd.cases import dam_break_c1
+    from oracle import swe_oracle as O
+    steps = args.steps
+    marks = sorted({1, min(100, steps), steps})
+    runs = {}
+    for dt_name in ("float32", "float64"):
+        mesh, cfg, q0, b = dam_break_c1(dtype=dt_name)
+        s = Solver2D(mesh, cfg, q0, b)
+        snaps, dts = {}, []
+        w0 = time.perf_counter()
+        for k in range(1, steps + 1):
+            dts.append(s.step())
+            if k in marks:
+                snaps[k] = s.q_interior.astype(np.float64)
+        runs[dt_name] = (snaps, np.array(dts), s.t, time.perf_counter() - w0)
+        log(f"GPU {dt_name}: {steps} steps to t={s.t:.6f} s in {runs[dt_name][3]:.2f} s")
+    mesh, cfg, q0, b = dam_break_c1(dtype="float64")
+    orc = {}
+    for dt_name, T in (("float64", np.float64), ("float32", np.float32)):
+        case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=b, dx=1.0, dy=1.0, bc_x="wall", bc_y="wall")
+        o = O.DenseOracle(case, O.Physics(cfl=0.5), T)
+        snaps, dts = {}, []
+        w0 = time.perf_counter()
+        for k in range(1, steps + 1):
+            dt = o.next_dt()
+            dts.append(dt)
+            o.step(dt)
+            if k in marks:
+                snaps[k] = o.interior.astype(np.float64)
+        orc[dt_name] = (snaps, np.array(dts), o.t, time.perf_counter() - w0)
+        log(f"oracle {dt_name}: {steps} steps in {orc[dt_name][3]:.1f} s")
+
+    def compare(a, b_):
+        out = {}
+        for k in marks:
+            qa, qb = a[0][k], b_[0][k]
+            out[str(k)] = {"max_abs_dh": float(np.abs(qa[0] - qb[0]).max()),
+                           "rel_l2": [float(np.linalg.norm(qa[m] - qb[m]) / max(np.linalg.norm(qb[m]), 1e-300))
+                                      for m in range(3)]}
+        out["t_final_diff"] = float(a[2] - b_[2])
+        out["max_rel_dt_diff"] = float(np.abs(a[1] - b_[1]).max() / b_[1].max())
+        return out
+
+    gpu32_vs_gpu64 = compare(runs["float32"], runs["float64"])
+    gpu64_vs_oracle64 = compare(runs["float64"], orc["float64"])
+    gpu32_vs_oracle32 = compare(runs["float32"], orc["float32"])
+    exact32 = all(np.array_equal(runs["float32"][0][k], orc["float32"][0][k]) for k in marks)
+    log(f"float32 vs float64 after {steps} steps: max |dh| {gpu32_vs_gpu64[str(steps)]['max_abs_dh']:.3e} m; "
+        f"GPU float32 == oracle float32 bit for bit: {exact32}")
+    cells = 256 * 256
+    w32 = runs["float32"][3]
+    cpu = orc["float32"][3]
+    out = {
+        "metric": "cell-updates/s (circular dam break 256x256, step(cfl_dt()) loop, 1 MI355X)",
+        "value": cells * steps / w32, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": 1e3 * w32 / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (h=10 m inside r<40 cells, 1 m outside, flat bed, walls)",
+        "config": {"workload": "circular dam break 256x256, dx=1 m, CFL 0.5, no friction, no rain, walls"},
+        "float32_vs_float64_gpu": gpu32_vs_gpu64, "float64_gpu_vs_float64_oracle": gpu64_vs_oracle64,
+        "float32_gpu_vs_float32_oracle": gpu32_vs_oracle32, "float32_gpu_bitwise_oracle": exact32,
+        "cpu_baseline": {"value": cells * steps / cpu, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                         "sample": f"NumPy oracle float32, the full run ({steps} steps) in {cpu:.1f} s"},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,8 +473,13 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead of the step, time the plain copy kernel over this many bytes (HBM calibration)")
     ap.add_argument("--copy-reps", type=int, default=10)
-    ap.add_argument("--workload", choices=["rain-on-slope", "coastal"], default="rain-on-slope",
-                    help="coastal: the compressed coastal case (flat and dense layouts, bitwise cross-check)")
+    ap.add_argument("--workload", choices=["rain-on-slope", "coastal", "ensemble", "adjoint", "dambreak"], default="rain-on-slope",
+                    help="coastal: the compressed coastal case (flat and dense layouts, bitwise cross-check); "
+                         "ensemble: batched uneven members vs sequential runs; adjoint: gauge-misfit gradients")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--graph-batch", type=int, default=50)
+    ap.add_argument("--n-adjoint", type=int, default=2048)
+    ap.add_argument("--checkpoint", type=int, default=100)
     ap.add_argument("--n", type=int, default=16384, help="coastal box side")
     args = ap.parse_args()
     if args.copy_bytes:
@@ -210,6 +491,12 @@ def main():
 
     if args.workload == "coastal":
         return coastal(args)
+    if args.workload == "ensemble":
+        return ensemble(args)
+    if args.workload == "adjoint":
+        return adjoint(args)
+    if args.workload == "dambreak":
+        return dambreak(args)
 
     from geoswe_amd import _lib as L
     from geoswe_amd.cases import FbmBed, C2_SEED, rain_on_slope_c2

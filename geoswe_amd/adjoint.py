@@ -1,0 +1,267 @@
+"""Gradients of a gauge-depth misfit with respect to per-class Manning n and per-cell bed.
+
+The forward model is a ``CompressedSolver`` stepped with a frozen dt sequence (the CFL
+steps of a reference run, replayed with ``gs_clock_set_dt``), so dt carries no
+derivative. The loss is
+
+    J = sum_{s=1..S} sum_g (h_s(g) - h*_s(g))^2
+
+over the depths h_s at gauge cells after every step. Its gradient comes from the
+hand-written reverse step ``gs_adj_flat_step`` (geoswe_amd/csrc/gs_adjoint.hip) with
+two-level checkpointing: the forward pass keeps the state every K steps; the reverse pass
+recomputes each K-step segment from its checkpoint (``gs_flat_step_pre``, which also keeps
+the pre-friction state the update's reverse needs), stores that segment's states on the
+device, and runs the K reverse steps over them. Memory is O(S/K + K) states; compute is
+two forward passes plus the reverse steps.
+
+``GaugeMisfitFn`` wraps this in a ``torch.autograd.Function`` of (ntab, bed).
+
+The reference (src/geoswe) has no adjoint; the forward it differentiates is the flat
+fused step of compressed_rhs.py:673-865 as built here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib as L
+from .compressed import CompressedSolver
+
+
+def flat_neighbours(fm) -> np.ndarray:
+    """(N, 4) int32 flat ids of (i+1, j), (i-1, j), (i, j+1), (i, j-1) of every stored
+    cell, -1 where that cell is not stored. Checks that every active cell has its four
+    neighbours and their neighbours along the same axis (the ring-2 stored set)."""
+    idx = np.full((fm.nxp, fm.nyp), -1, np.int64)
+    i, j = fm.ij[:, 0].astype(np.int64), fm.ij[:, 1].astype(np.int64)
+    idx[i, j] = np.arange(fm.n_stored)
+
+    def at(di, dj):
+        ii, jj = i + di, j + dj
+        ok = (ii >= 0) & (ii < fm.nxp) & (jj >= 0) & (jj < fm.nyp)
+        out = np.full(fm.n_stored, -1, np.int64)
+        out[ok] = idx[ii[ok], jj[ok]]
+        return out
+
+    nb = np.stack([at(1, 0), at(-1, 0), at(0, 1), at(0, -1)], 1)
+    act = fm.active
+    if (nb[act] < 0).any():
+        raise ValueError("flat_neighbours: an active cell lacks a stored neighbour")
+    for ax, (u, d) in enumerate(((0, 1), (2, 3))):
+        for k in (u, d):
+            m = nb[act, k]
+            if (nb[m, u] < 0).any() or (nb[m, d] < 0).any():
+                raise ValueError("flat_neighbours: the stored set is not two cells deep around the active set")
+    return nb.astype(np.int32)
+
+
+class GaugeMisfit:
+    """J(ntab, bed) for one CompressedSolver, a frozen dt sequence and gauge cells."""
+
+    def __init__(self, solver: CompressedSolver, gauges: np.ndarray, dts: np.ndarray,
+                 h_star: Optional[np.ndarray] = None, checkpoint_every: int = 100):
+        self.s = solver
+        fm = solver.fm
+        self.dtype = solver.dtype
+        self.N = fm.n_stored
+        self.gauges = np.asarray(gauges, np.int64)
+        if self.gauges.ndim != 1 or not len(self.gauges):
+            raise ValueError("gauges must be a non-empty 1-D array of flat ids")
+        if (self.gauges < 0).any() or (self.gauges >= self.N).any() or not fm.active[self.gauges].all():
+            raise ValueError("gauges must be active flat ids")
+        self.dts = np.asarray(dts, np.float64)
+        if self.dts.ndim != 1 or not len(self.dts) or not np.all(self.dts > 0):
+            raise ValueError("dts must be a non-empty 1-D array of positive steps")
+        self.S = len(self.dts)
+        self.K = int(checkpoint_every)
+        if self.K < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        G = len(self.gauges)
+        self.h_star = np.zeros((self.S, G)) if h_star is None else np.asarray(h_star, np.float64)
+        if self.h_star.shape != (self.S, G):
+            raise ValueError(f"h_star must have shape ({self.S}, {G})")
+        self._d_gauges = L.DeviceArray.from_host(self.gauges)
+        self._d_samples = L.DeviceArray((self.S, G), self.dtype, zero=True)
+        self._nbr = L.DeviceArray.from_host(flat_neighbours(fm))
+        self._active = L.DeviceArray.from_host(fm.active.astype(np.uint8))
+        cur = solver.nsteps & 1
+        self._q0 = [L.DeviceArray((self.N,), self.dtype) for _ in range(3)]
+        for k in range(3):
+            L.call("gs_memcpy_d2d", C.c_void_p(self._q0[k].ptr), C.c_void_p(solver._bufs[cur][k].ptr),
+                   C.c_size_t(self._q0[k].nbytes), None)
+        self.t0 = solver.t
+        self._ckpt = {}
+        self._ckpt_t = {}
+        self.samples = None
+
+    # ------------------------------------------------------------------ parameters
+    @property
+    def n_classes(self) -> int:
+        return self.s._n_classes
+
+    def set_params(self, ntab=None, bed=None) -> None:
+        """Manning n per class (n_classes,) and/or the flat bed (N,), host arrays."""
+        if ntab is not None:
+            t = np.zeros(L.MAX_CLASSES, self.dtype)
+            nt = np.asarray(ntab, np.float64).ravel()
+            if len(nt) != self.n_classes:
+                raise ValueError(f"ntab has {len(nt)} entries, the solver {self.n_classes} classes")
+            t[:len(nt)] = nt
+            self.s._ntab.upload(t)
+        if bed is not None:
+            b = np.asarray(bed).ravel()
+            if len(b) != self.N:
+                raise ValueError(f"bed has {len(b)} entries, the flat mesh {self.N}")
+            self.s._bed.upload(b.astype(self.dtype))
+
+    def params(self):
+        return (self.s._ntab.download()[:self.n_classes].astype(np.float64),
+                self.s._bed.download().astype(np.float64))
+
+    # ------------------------------------------------------------------ forward
+    def _reset(self, q_src, t0: float) -> None:
+        s = self.s
+        for k in range(3):
+            L.call("gs_memcpy_d2d", C.c_void_p(s._bufs[0][k].ptr), C.c_void_p(q_src[k].ptr),
+                   C.c_size_t(self._q0[k].nbytes), None)
+        L.call("gs_clock_reset", C.c_void_p(s._clock.ptr), C.c_double(t0), None)
+        s._lam_valid = False
+        s._target = None
+
+    def _set_dt(self, dt: float, p) -> None:
+        s = self.s
+        L.call("gs_clock_set_dt", C.c_void_p(s._clock.ptr), C.c_double(dt), p, s._fo.s, None)
+
+    def simulate(self, keep_checkpoints: bool = True) -> np.ndarray:
+        """S frozen-dt steps from the initial state; gauge depths (S, G) after each step."""
+        s = self.s
+        self._reset(self._q0, self.t0)
+        d, p, clk = C.byref(s._desc()), C.byref(s._phys()), C.c_void_p(s._clock.ptr)
+        G = len(self.gauges)
+        es = self._d_samples.dtype.itemsize
+        for st in range(self.S):
+            if keep_checkpoints and st % self.K == 0:
+                self._save_ckpt(st // self.K)
+            self._set_dt(float(self.dts[st]), p)
+            L.call("gs_flat_step", d, p, s._fo.s, clk, None)
+            L.call("gs_flat_sample", d, clk, C.c_void_p(self._d_gauges.ptr), G,
+                   C.c_void_p(self._d_samples.ptr + st * G * es), None)
+        L.sync(None)
+        c = s._clockv()
+        if c.status & L.ST_NONFINITE:
+            raise FloatingPointError("GaugeMisfit: non-finite state in the forward run")
+        if c.nsteps != self.S:
+            raise RuntimeError(f"GaugeMisfit: {c.nsteps} steps taken, {self.S} expected")
+        self.samples = self._d_samples.download().astype(np.float64)
+        return self.samples
+
+    def _save_ckpt(self, k: int) -> None:
+        s = self.s
+        cur = s.nsteps & 1
+        if k not in self._ckpt:
+            self._ckpt[k] = [L.DeviceArray((self.N,), self.dtype) for _ in range(3)]
+        for m in range(3):
+            L.call("gs_memcpy_d2d", C.c_void_p(self._ckpt[k][m].ptr), C.c_void_p(s._bufs[cur][m].ptr),
+                   C.c_size_t(self._q0[m].nbytes), None)
+        self._ckpt_t[k] = s.t
+
+    def loss(self, samples: Optional[np.ndarray] = None) -> float:
+        h = self.samples if samples is None else samples
+        return float(np.sum((h - self.h_star) ** 2))
+
+    # ------------------------------------------------------------------ reverse
+    def gradient(self):
+        """(dJ/dn per class (n_classes,), dJ/dbed (N,)) at the parameters of the last
+        simulate() (float64 host arrays)."""
+        if self.samples is None:
+            raise RuntimeError("gradient: call simulate() first")
+        s, N, K, S = self.s, self.N, self.K, self.S
+        dt_ = self.dtype
+        G = len(self.gauges)
+        nseg = (S + K - 1) // K
+        seg_states = [[L.DeviceArray((N,), dt_) for _ in range(3)] for _ in range(K + 1)]
+        seg_pre = [[L.DeviceArray((N,), dt_) for _ in range(3)] for _ in range(K)]
+        lam = [L.DeviceArray((N,), dt_, zero=True) for _ in range(3)]
+        w = [L.DeviceArray((N,), dt_, zero=True) for _ in range(3)]
+        xf = L.DeviceArray((10, N), dt_, zero=True)
+        yf = L.DeviceArray((10, N), dt_, zero=True)
+        gz = L.DeviceArray((2, N), dt_, zero=True)
+        bbar = L.DeviceArray((N,), np.float64, zero=True)
+        nbar = L.DeviceArray((L.MAX_CLASSES,), np.float64, zero=True)
+        coef = (2.0 * (self.samples - self.h_star)).astype(dt_)          # (S, G): dJ/dh_s at gauges
+        d_coef = L.DeviceArray.from_host(coef)
+        es = d_coef.dtype.itemsize
+        dcode = L.GS_F32 if dt_ == np.float32 else L.GS_F64
+        a = L.GsAdjFlat()
+        a.n_stored, a.dtype = N, dcode
+        a.dx, a.dy = float(s.mesh.dx), float(s.mesh.dy)
+        a.active, a.nbr, a.bed = self._active.ptr, self._nbr.ptr, s._bed.ptr
+        a.flags, a.ntab = s._flags.ptr, s._ntab.ptr
+        for k in range(3):
+            a.lam[k], a.w[k] = lam[k].ptr, w[k].ptr
+        a.xf, a.yf, a.gz, a.bbar, a.nbar = xf.ptr, yf.ptr, gz.ptr, bbar.ptr, nbar.ptr
+        phys = s._phys()
+        p = C.byref(phys)
+        desc = s._desc()
+        clk = C.c_void_p(s._clock.ptr)
+        # cotangent of q_S: the last step's gauge term
+        L.call("gs_adj_add", C.c_void_p(lam[0].ptr), dcode, C.c_void_p(self._d_gauges.ptr),
+               C.c_void_p(d_coef.ptr + (S - 1) * G * es), G, None)
+        nbytes = self._q0[0].nbytes
+        for seg in range(nseg - 1, -1, -1):
+            s0 = seg * K
+            n_in = min(K, S - s0)
+            # recompute the segment from its checkpoint, keeping states and pre-friction states
+            self._reset(self._ckpt[seg], self._ckpt_t[seg])
+            for m in range(3):
+                L.call("gs_memcpy_d2d", C.c_void_p(seg_states[0][m].ptr), C.c_void_p(self._ckpt[seg][m].ptr),
+                       C.c_size_t(nbytes), None)
+            d = C.byref(desc)
+            for j in range(n_in):
+                self._set_dt(float(self.dts[s0 + j]), p)
+                L.call("gs_flat_step_pre", d, p, s._fo.s, clk, C.c_void_p(seg_pre[j][0].ptr),
+                       C.c_void_p(seg_pre[j][1].ptr), C.c_void_p(seg_pre[j][2].ptr), None)
+                cur = (j + 1) & 1
+                for m in range(3):
+                    L.call("gs_memcpy_d2d", C.c_void_p(seg_states[j + 1][m].ptr),
+                           C.c_void_p(s._bufs[cur][m].ptr), C.c_size_t(nbytes), None)
+            # reverse steps of the segment
+            for j in range(n_in - 1, -1, -1):
+                st = s0 + j
+                for m in range(3):
+                    a.q[m] = seg_states[j][m].ptr
+                    a.pre[m] = seg_pre[j][m].ptr
+                a.h_next = seg_states[j + 1][0].ptr
+                L.call("gs_adj_flat_step", C.byref(a), p, C.c_double(float(self.dts[st])), None)
+                if st >= 1:     # dJ/dh_st at gauges (the state after step st)
+                    L.call("gs_adj_add", C.c_void_p(lam[0].ptr), dcode, C.c_void_p(self._d_gauges.ptr),
+                           C.c_void_p(d_coef.ptr + (st - 1) * G * es), G, None)
+        L.sync(None)
+        return nbar.download()[:self.n_classes].copy(), bbar.download().copy()
+
+
+try:
+    import torch
+
+    class GaugeMisfitFn(torch.autograd.Function):
+        """J = GaugeMisfitFn.apply(ntab, bed, problem): the misfit as a torch scalar whose
+        backward runs the device adjoint (ntab (n_classes,), bed (N,) in flat order)."""
+
+        @staticmethod
+        def forward(ctx, ntab, bed, problem):
+            problem.set_params(ntab.detach().cpu().double().numpy(), bed.detach().cpu().double().numpy())
+            problem.simulate()
+            ctx.problem = problem
+            ctx.dev = (ntab.device, bed.device)
+            return torch.tensor(problem.loss(), dtype=torch.float64, device=ntab.device)
+
+        @staticmethod
+        def backward(ctx, g):
+            gn, gb = ctx.problem.gradient()
+            gs = float(g)
+            return (torch.as_tensor(gn * gs, device=ctx.dev[0]), torch.as_tensor(gb * gs, device=ctx.dev[1]),
+                    None)
+except ImportError:     # torch is optional for the solver path
+    GaugeMisfitFn = None

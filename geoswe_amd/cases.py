@@ -59,13 +59,14 @@ class FbmBed:
 
 
 def dam_break_c1(n: int = 256, dtype: str = "float32"):
-    """(mesh, cfg, q0, b) of the circular dam break: h = 10 inside r < 40 cells, 1 outside."""
+    """(mesh, cfg, q0, b) of the circular dam break: h = 10 inside r < 40 cells, 1 outside;
+    dry floor 1e-6 m in either precision."""
     i = np.arange(n)[:, None]
     j = np.arange(n)[None, :]
     h = np.where(np.hypot(i - (n - 1) / 2.0, j - (n - 1) / 2.0) < 40, 10.0, 1.0)
     q0 = np.stack([h, 0 * h, 0 * h])
     mesh = Mesh2D(n, n, 1.0, 1.0, ngh=4)
-    cfg = Config(bc_x="wall", bc_y="wall", cfl=0.5, dtype=dtype)
+    cfg = Config(bc_x="wall", bc_y="wall", cfl=0.5, dtype=dtype, h_min=1.0e-6, h_min_cfl=1.0e-6)
     return mesh, cfg, q0, np.zeros((n, n))
 
 

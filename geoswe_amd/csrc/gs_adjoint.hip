@@ -1,0 +1,274 @@
+// gs_adjoint.hip -- reverse step of the flat (compressed) fused step, for gradients of a
+// scalar loss of the states with respect to the per-class Manning n and the per-cell bed.
+//
+// One reverse step maps the cotangent of q_{s+1} to that of q_s and accumulates the
+// parameter cotangents, with the step's dt frozen. It runs in four passes over the stored
+// cells, all gathers (no floating-point atomics on the state):
+//   A  per active cell: reverse of the update (friction, wet/dry) from the stored
+//      pre-friction state -> w = cotangent of (h1, hu1, hv1); residual cotangent is dt*w;
+//      Manning n per class (workgroup sums, one atomic per class and workgroup);
+//      stage cells pass -cot(h) to their bed where eta - b > 0.
+//   B  per face, each face once: reverse of face() + prims() from the residual
+//      cotangents of its two cells; 10 numbers into the face's slot, indexed by its lower
+//      cell (x: (c, i+1), y: (c, j+1)). An active cell owns its upper faces and its lower
+//      faces whose lower cell is not active.
+//   C  per stored cell: cotangent of q_s = w + the four face slots touching the cell; bed
+//      cotangent from the faces' direct terms; the cell's bed-gradient cotangents.
+//   D  per stored cell: bed cotangent through the central bed gradients of its four
+//      neighbours, gz_X = 0.5*(b_{X+1} - b_{X-1})*inv_d.
+// The arithmetic is gs_adjoint.h's (checked against finite differences on the CPU).
+#include "gs_common.h"
+#include "gs_adjoint.h"
+
+namespace gs {
+
+constexpr int kABlock = 256;
+
+template <typename T>
+struct AdjK {
+  int64_t n;
+  const uint8_t* active;
+  const int32_t* nbr;     // [n][4]: i+1, i-1, j+1, j-1
+  const T* bed;
+  const uint8_t* flags;
+  const T* ntab;
+  int n_classes;
+  const T* q[3];
+  const T* h_next;
+  const T* pre[3];
+  T* lam[3];
+  T* w[3];
+  T* xf;                  // [10][n]
+  T* yf;                  // [10][n]
+  T* gz;                  // [2][n]
+  double* bbar;
+  double* nbar;
+  Phys<T> P;
+  T dt;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kABlock) void k_adj_update(AdjK<T> a) {
+  __shared__ double s_n[32];
+  if (threadIdx.x < 32) s_n[threadIdx.x] = 0.0;
+  __syncthreads();
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * blockDim.x) {
+    if (!a.active[c]) continue;
+    const uint8_t fl = a.flags ? a.flags[c] : (uint8_t)0;
+    const T lh = a.lam[0][c], lhu = a.lam[1][c], lhv = a.lam[2][c];
+    if (fl & GS_FLAG_STAGE) {
+      // h = max(eta - b, 0), hu = hv = 0: the step's own inputs get nothing
+      if (a.h_next[c] > T(0)) a.bbar[c] -= (double)lh;
+      a.w[0][c] = T(0);
+      a.w[1][c] = T(0);
+      a.w[2][c] = T(0);
+      continue;
+    }
+    const int cls = fl & GS_FLAG_CLASS_MASK;
+    const T n = a.P.friction ? a.ntab[cls] : T(0);
+    T g1, gu1, gv1, gn;
+    update_vjp<T>(a.pre[0][c], a.pre[1][c], a.pre[2][c], a.dt, n, a.P, lh, lhu, lhv, g1, gu1, gv1, gn);
+    a.w[0][c] = g1;
+    a.w[1][c] = gu1;
+    a.w[2][c] = gv1;
+    if (gn != T(0)) atomicAdd(&s_n[cls], (double)gn);
+  }
+  __syncthreads();
+  if (threadIdx.x < (unsigned)a.n_classes && s_n[threadIdx.x] != 0.0) atomicAdd(&a.nbar[threadIdx.x], s_n[threadIdx.x]);
+}
+
+// Reverse of one face (L, R) along x (axis 0) or y (axis 1) into slot L of `slots`.
+template <typename T>
+__device__ __forceinline__ void face_slot(const AdjK<T>& a, int axis, int64_t L, int64_t R, T* slots) {
+  const T hm = a.P.h_min, g = a.P.g;
+  const T inv = axis == 0 ? a.P.inv_dx : a.P.inv_dy;
+  const T half = axis == 0 ? a.P.half_dx : a.P.half_dy;
+  const int up = axis == 0 ? 0 : 2, dn = axis == 0 ? 1 : 3;
+  const int64_t n = a.n;
+  const T hL = a.q[0][L], huL = a.q[1][L], hvL = a.q[2][L];
+  const T hR = a.q[0][R], huR = a.q[1][R], hvR = a.q[2][R];
+  T* s = slots + L;
+  if (hL < hm && hR < hm) {   // dry face: no contribution either way
+    for (int k = 0; k < 10; ++k) s[k * n] = T(0);
+    return;
+  }
+  const T bL = a.bed[L], bR = a.bed[R];
+  // the cells either side of L and R along the axis are stored for every face of an
+  // active cell (ring-2 set, checked when the table is built); clamp all the same
+  const int32_t* nl = a.nbr + 4 * L;
+  const int32_t* nr = a.nbr + 4 * R;
+  const int64_t lu = nl[up] >= 0 ? nl[up] : L, ld = nl[dn] >= 0 ? nl[dn] : L;
+  const int64_t ru = nr[up] >= 0 ? nr[up] : R, rd = nr[dn] >= 0 ? nr[dn] : R;
+  const T gzL = (T(0.5) * (a.bed[lu] - a.bed[ld])) * inv;
+  const T gzR = (T(0.5) * (a.bed[ru] - a.bed[rd])) * inv;
+  T uL, vL, uR, vR;
+  prims(hL, huL, hvL, hm, uL, vL);
+  prims(hR, huR, hvR, hm, uR, vR);
+  // residual cotangents dt*w of both cells (zero outside active non-stage cells)
+  const T dt = a.dt;
+  const T rhL = dt * a.w[0][L], ruL = dt * a.w[1][L], rvL = dt * a.w[2][L];
+  const T rhR = dt * a.w[0][R], ruR = dt * a.w[1][R], rvR = dt * a.w[2][R];
+  FaceGrad<T> G;
+  if (axis == 0) {
+    face_vjp<T>(bL, bR, gzL, gzR, hL, hR, uL, vL, uR, vR, half, g, hm, inv * (rhR - rhL), inv * (ruR - ruL),
+                inv * (rvR - rvL), -(inv * ruL), inv * ruR, G);
+  } else {
+    face_vjp<T>(bL, bR, gzL, gzR, hL, hR, vL, uL, vR, uR, half, g, hm, inv * (rhR - rhL), inv * (rvR - rvL),
+                inv * (ruR - ruL), -(inv * rvL), inv * rvR, G);
+  }
+  T ghL = G.hL, ghuL = T(0), ghvL = T(0), ghR = G.hR, ghuR = T(0), ghvR = T(0);
+  if (axis == 0) {
+    prims_vjp<T>(hL, huL, hvL, hm, G.uNL, G.uTL, ghL, ghuL, ghvL);
+    prims_vjp<T>(hR, huR, hvR, hm, G.uNR, G.uTR, ghR, ghuR, ghvR);
+  } else {
+    prims_vjp<T>(hL, huL, hvL, hm, G.uTL, G.uNL, ghL, ghuL, ghvL);
+    prims_vjp<T>(hR, huR, hvR, hm, G.uTR, G.uNR, ghR, ghuR, ghvR);
+  }
+  s[0] = ghL;
+  s[n] = ghuL;
+  s[2 * n] = ghvL;
+  s[3 * n] = G.bL;
+  s[4 * n] = G.gzL;
+  s[5 * n] = ghR;
+  s[6 * n] = ghuR;
+  s[7 * n] = ghvR;
+  s[8 * n] = G.bR;
+  s[9 * n] = G.gzR;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kABlock) void k_adj_faces(AdjK<T> a) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * blockDim.x) {
+    if (!a.active[c]) continue;
+    const int32_t* nb = a.nbr + 4 * c;
+    if (nb[0] >= 0) face_slot<T>(a, 0, c, nb[0], a.xf);
+    if (nb[1] >= 0 && !a.active[nb[1]]) face_slot<T>(a, 0, nb[1], c, a.xf);
+    if (nb[2] >= 0) face_slot<T>(a, 1, c, nb[2], a.yf);
+    if (nb[3] >= 0 && !a.active[nb[3]]) face_slot<T>(a, 1, nb[3], c, a.yf);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kABlock) void k_adj_gather(AdjK<T> a) {
+  const int64_t n = a.n;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t* nb = a.nbr + 4 * c;
+    const int64_t W = nb[1], S = nb[3];
+    const T* x = a.xf;
+    const T* y = a.yf;
+    T gzx = x[4 * n + c], gzy = y[4 * n + c];
+    T bb = x[3 * n + c] + y[3 * n + c];
+    if (W >= 0) { gzx += x[9 * n + W]; bb += x[8 * n + W]; }
+    if (S >= 0) { gzy += y[9 * n + S]; bb += y[8 * n + S]; }
+    a.gz[c] = gzx;
+    a.gz[n + c] = gzy;
+    a.bbar[c] += (double)bb;
+    if (a.active[c]) {
+      for (int k = 0; k < 3; ++k) {
+        T l = a.w[k][c] + x[k * n + c] + y[k * n + c];
+        if (W >= 0) l += x[(5 + k) * n + W];
+        if (S >= 0) l += y[(5 + k) * n + S];
+        a.lam[k][c] = l;
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kABlock) void k_adj_bedgrad(AdjK<T> a) {
+  const int64_t n = a.n;
+  const T hx = T(0.5) * a.P.inv_dx, hy = T(0.5) * a.P.inv_dy;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t* nb = a.nbr + 4 * c;
+    T v = T(0);
+    if (nb[1] >= 0) v += hx * a.gz[nb[1]];
+    if (nb[0] >= 0) v -= hx * a.gz[nb[0]];
+    if (nb[3] >= 0) v += hy * a.gz[n + nb[3]];
+    if (nb[2] >= 0) v -= hy * a.gz[n + nb[2]];
+    a.bbar[c] += (double)v;
+  }
+}
+
+template <typename T>
+__global__ void k_adj_add(T* lam_h, const int64_t* ids, const T* coef, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) lam_h[ids[t]] += coef[t];
+}
+
+template <typename T>
+static int adj_step_t(const GsAdjFlat* f, const GsPhys* p, double dt, hipStream_t s) {
+  AdjK<T> a;
+  memset(&a, 0, sizeof a);
+  a.n = f->n_stored;
+  a.active = f->active;
+  a.nbr = f->nbr;
+  a.bed = (const T*)f->bed;
+  a.flags = f->flags;
+  a.ntab = (const T*)f->ntab;
+  a.n_classes = p->n_classes;
+  for (int k = 0; k < 3; ++k) {
+    a.q[k] = (const T*)f->q[k];
+    a.pre[k] = (const T*)f->pre[k];
+    a.lam[k] = (T*)f->lam[k];
+    a.w[k] = (T*)f->w[k];
+  }
+  a.h_next = (const T*)f->h_next;
+  a.xf = (T*)f->xf;
+  a.yf = (T*)f->yf;
+  a.gz = (T*)f->gz;
+  a.bbar = f->bbar;
+  a.nbar = f->nbar;
+  a.P = make_phys<T>(*p, f->dx, f->dy);
+  a.dt = (T)dt;
+  int64_t blocks = (a.n + kABlock - 1) / kABlock;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  const dim3 gr((unsigned)blocks), bl(kABlock);
+  hipLaunchKernelGGL(k_adj_update<T>, gr, bl, 0, s, a);
+  hipLaunchKernelGGL(k_adj_faces<T>, gr, bl, 0, s, a);
+  hipLaunchKernelGGL(k_adj_gather<T>, gr, bl, 0, s, a);
+  hipLaunchKernelGGL(k_adj_bedgrad<T>, gr, bl, 0, s, a);
+  GS_HIP(hipGetLastError(), "adjoint step launch");
+  return GS_OK;
+}
+
+}  // namespace gs
+
+extern "C" {
+
+int gs_adj_flat_step(const GsAdjFlat* f, const GsPhys* phys, double dt, void* stream) {
+  using namespace gs;
+  if (!f || !phys) return fail(GS_E_ARG, "NULL argument");
+  if (f->n_stored < 1) return fail(GS_E_RANGE, "n_stored < 1");
+  if (!f->active || !f->nbr || !f->bed || !f->h_next || !f->xf || !f->yf || !f->gz || !f->bbar || !f->nbar)
+    return fail(GS_E_ARG, "adjoint arrays are NULL");
+  for (int k = 0; k < 3; ++k)
+    if (!f->q[k] || !f->pre[k] || !f->lam[k] || !f->w[k]) return fail(GS_E_ARG, "state/cotangent arrays are NULL");
+  int rc = check_phys(phys);
+  if (rc) return rc;
+  if (phys->friction && !f->ntab) return fail(GS_E_ARG, "friction needs ntab");
+  if (!(dt > 0.0) || !isfinite(dt)) return fail(GS_E_ARG, "dt must be positive and finite");
+  if (f->dtype == GS_F64) return adj_step_t<double>(f, phys, dt, (hipStream_t)stream);
+  if (f->dtype == GS_F32) return adj_step_t<float>(f, phys, dt, (hipStream_t)stream);
+  return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
+}
+
+int gs_adj_add(void* lam_h, int dtype, const int64_t* ids, const void* coef, int n, void* stream) {
+  using namespace gs;
+  if (n < 0) return fail(GS_E_ARG, "n < 0");
+  if (n == 0) return GS_OK;
+  if (!lam_h || !ids || !coef) return fail(GS_E_ARG, "NULL argument");
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (dtype == GS_F64)
+    hipLaunchKernelGGL(k_adj_add<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (double*)lam_h, ids,
+                       (const double*)coef, n);
+  else if (dtype == GS_F32)
+    hipLaunchKernelGGL(k_adj_add<float>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (float*)lam_h, ids,
+                       (const float*)coef, n);
+  else
+    return fail(GS_E_DTYPE, "unsupported dtype %d", dtype);
+  GS_HIP(hipGetLastError(), "k_adj_add launch");
+  return GS_OK;
+}
+
+}  // extern "C"

@@ -49,22 +49,60 @@ struct FlatK {
   int64_t frame_stride, rain_pitch;
   GsClock* clock;
   Plan plan;
+  T* pre[3];        // nullable: pre-friction h, hu, hv of active cells (adjoint recomputation)
 };
 
 template <typename T> __device__ __forceinline__ T fl_upper(T x) { return __shfl_down(x, 1, 64); }
 template <typename T> __device__ __forceinline__ T fl_lower(T x) { return __shfl_up(x, 1, 64); }
 
-// Per-launch inputs of a step that come from the clock (or the residual selector).
+// Everything one chunk pass reads, as values: the state buffers picked for this step (a
+// runtime index into the argument's buffer table would put the table in private memory and
+// every state access behind a generic pointer), the mesh arrays, the clock's per-step
+// values and the physics scalars. Filled from the kernel argument (plain step) or from
+// the member table (ensemble).
+#define GS_G __attribute__((address_space(1)))
+
 template <typename T>
 struct FlatCtx {
-  T* const* qin;
-  T* const* qout;
+  const GS_G T* qi0;
+  const GS_G T* qi1;
+  const GS_G T* qi2;
+  GS_G T* qo0;
+  GS_G T* qo1;
+  GS_G T* qo2;
+  const GS_G T* bed;
+  const GS_G uint8_t* flags;
+  const GS_G T* n_field;
+  GS_G T* max_h;
+  GS_G T* rhs0;
+  GS_G T* rhs1;
+  GS_G T* rhs2;
+  GS_G T* pre0;
+  GS_G T* pre1;
+  GS_G T* pre2;
+  const GS_G int64_t* run_k0;
+  const GS_G int32_t* run_len;
+  const GS_G int32_t* run_i;
+  const GS_G int32_t* run_j0;
+  const GS_G int32_t* run_d;
+  const GS_G int32_t* chunk_run;
+  const GS_G T* frame;   // gridded rain frame of this step (rain_mode 2)
+  const T* ntab;         // Manning n per class (LDS copy or the member's table)
+  int64_t n, rain_pitch;
+  int n_runs, ngh, rain_mode;
   T dt, rain_u, eta;
-  const T* frame;   // gridded rain frame of this step (rain_mode 2)
-  const T* ntab;    // Manning n per class (LDS copy or the member's table)
+  Phys<T> P;
 };
 
-// Reads the clock into a step context; false when the clock says the run is done.
+// Device buffers are global memory; the explicit address space lets the compiler use global
+// (not generic) accesses for pointers that come from device memory (the member table).
+template <typename P>
+__device__ __forceinline__ GS_G P* gp(P* p) {
+  return (GS_G P*)p;
+}
+
+// Reads the clock and the layout into a step context; false when the clock says the run
+// is done.
 template <typename T, int MODE>
 __device__ __forceinline__ bool flat_ctx(const FlatK<T>& a, FlatCtx<T>& x) {
   int par = 0;
@@ -72,8 +110,9 @@ __device__ __forceinline__ bool flat_ctx(const FlatK<T>& a, FlatCtx<T>& x) {
   x.rain_u = T(0);
   x.eta = T(0);
   x.frame = nullptr;
+  x.rain_mode = a.rain_mode;
   if (MODE == 1) {
-    const GsClock* c = a.clock;
+    const GS_G GsClock* c = gp(a.clock);
     if (c->status & GS_ST_DONE) return false;
     par = (int)(c->nsteps & 1);
     const int ridx = c->rain_idx;
@@ -83,13 +122,39 @@ __device__ __forceinline__ bool flat_ctx(const FlatK<T>& a, FlatCtx<T>& x) {
       x.eta = (T)c->eta32;
     } else {
       x.dt = (T)c->dt;
-      x.rain_u = a.rain_mode == 1 ? (T)a.plan.rain_rate[ridx] : T(0);
+      x.rain_u = a.rain_mode == 1 ? (T)gp(a.plan.rain_rate)[ridx] : T(0);
       x.eta = (T)c->eta64;
     }
-    if (a.rain_mode == 2) x.frame = a.rain_frames + (int64_t)ridx * a.frame_stride;
+    if (a.rain_mode == 2) x.frame = gp(a.rain_frames) + (int64_t)ridx * a.frame_stride;
   }
-  x.qin = MODE == 1 ? a.buf[par] : a.buf[a.which];
-  x.qout = a.buf[par ^ 1];
+  const bool in1 = (MODE == 1 ? par : a.which) != 0;
+  x.qi0 = gp(in1 ? a.buf[1][0] : a.buf[0][0]);
+  x.qi1 = gp(in1 ? a.buf[1][1] : a.buf[0][1]);
+  x.qi2 = gp(in1 ? a.buf[1][2] : a.buf[0][2]);
+  x.qo0 = gp(par ? a.buf[0][0] : a.buf[1][0]);
+  x.qo1 = gp(par ? a.buf[0][1] : a.buf[1][1]);
+  x.qo2 = gp(par ? a.buf[0][2] : a.buf[1][2]);
+  x.bed = gp(a.bed);
+  x.flags = gp(a.flags);
+  x.n_field = gp(a.n_field);
+  x.max_h = gp(a.max_h);
+  x.rhs0 = gp(a.rhs[0]);
+  x.rhs1 = gp(a.rhs[1]);
+  x.rhs2 = gp(a.rhs[2]);
+  x.pre0 = gp(a.pre[0]);
+  x.pre1 = gp(a.pre[1]);
+  x.pre2 = gp(a.pre[2]);
+  x.run_k0 = gp(a.run_k0);
+  x.run_len = gp(a.run_len);
+  x.run_i = gp(a.run_i);
+  x.run_j0 = gp(a.run_j0);
+  x.run_d = gp(a.run_d);
+  x.chunk_run = gp(a.chunk_run);
+  x.n = a.n;
+  x.rain_pitch = a.rain_pitch;
+  x.n_runs = a.n_runs;
+  x.ngh = a.ngh;
+  x.P = a.P;
   return true;
 }
 
@@ -97,15 +162,14 @@ __device__ __forceinline__ bool flat_ctx(const FlatK<T>& a, FlatCtx<T>& x) {
 // fused update (MODE 1) of the 60 middle ids, folding their CFL speed bits into lam.
 // Every lane of the wavefront must call it (cross-lane shuffles).
 template <typename T, int MODE>
-__device__ __forceinline__ void flat_chunk(const FlatK<T>& a, const FlatCtx<T>& x, int64_t chunk, int lane,
-                                           uint64_t& lam) {
+__device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, int lane, uint64_t& lam) {
+  const FlatCtx<T>& a = x;
   const Phys<T>& P = a.P;
   const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
-  T* const* qin = x.qin;
   const int64_t k = chunk * kFOut + lane - 2;
   const int64_t kl = k < 0 ? 0 : (k >= a.n ? a.n - 1 : k);
   // this lane's cell (needed by every lane for the y-face shuffles)
-  const T hC = qin[0][kl], huC = qin[1][kl], hvC = qin[2][kl];
+  const T hC = x.qi0[kl], huC = x.qi1[kl], hvC = x.qi2[kl];
   const T bC = a.bed[kl];
   T uC, vC;
   prims(hC, huC, hvC, hm, uC, vC);
@@ -128,11 +192,11 @@ __device__ __forceinline__ void flat_chunk(const FlatK<T>& a, const FlatCtx<T>& 
     if (rr < a.n_runs && a.run_k0[rr] <= k) r = rr;
   }
   if (r < 0) return;
-  const int32_t* d = a.run_d + 4 * (int64_t)r;
+  const GS_G int32_t* d = a.run_d + 4 * (int64_t)r;
   const int64_t kE = k + d[0], kW = k + d[1], kEE = k + d[2], kWW = k + d[3];
   const T bE = a.bed[kE], bW = a.bed[kW], bEE = a.bed[kEE], bWW = a.bed[kWW];
-  const T hE = qin[0][kE], huE = qin[1][kE], hvE = qin[2][kE];
-  const T hW = qin[0][kW], huW = qin[1][kW], hvW = qin[2][kW];
+  const T hE = x.qi0[kE], huE = x.qi1[kE], hvE = x.qi2[kE];
+  const T hW = x.qi0[kW], huW = x.qi1[kW], hvW = x.qi2[kW];
   T uE, vE, uW, vW;
   prims(hE, huE, hvE, hm, uE, vE);
   prims(hW, huW, hvW, hm, uW, vW);
@@ -168,9 +232,9 @@ __device__ __forceinline__ void flat_chunk(const FlatK<T>& a, const FlatCtx<T>& 
     rhv = rhv + ysRB * iy;
   }
   if (MODE == 0) {
-    a.rhs[0][k] = rh;
-    a.rhs[1][k] = rhu;
-    a.rhs[2][k] = rhv;
+    a.rhs0[k] = rh;
+    a.rhs1[k] = rhu;
+    a.rhs2[k] = rhv;
     return;
   }
   const uint8_t fg = a.flags ? a.flags[k] : (uint8_t)0;
@@ -181,6 +245,11 @@ __device__ __forceinline__ void flat_chunk(const FlatK<T>& a, const FlatCtx<T>& 
     const int64_t ii = (int64_t)a.run_i[r] - a.ngh;
     const int64_t jj = (int64_t)a.run_j0[r] + (k - a.run_k0[r]) - a.ngh;
     rr0 = rr0 + x.frame[ii * a.rain_pitch + jj];
+  }
+  if (a.pre0) {   // the first three statements of update(), kept for its reverse
+    a.pre0[k] = hC + x.dt * rr0;
+    a.pre1[k] = huC + x.dt * rhu;
+    a.pre2[k] = hvC + x.dt * rhv;
   }
   T n = T(0);
   if (P.friction) n = a.n_field ? a.n_field[k] : x.ntab[fg & GS_FLAG_CLASS_MASK];
@@ -193,9 +262,9 @@ __device__ __forceinline__ void flat_chunk(const FlatK<T>& a, const FlatCtx<T>& 
     hv = T(0);
   }
   if (a.max_h && h > a.max_h[k]) a.max_h[k] = h;
-  x.qout[0][k] = h;
-  x.qout[1][k] = hu;
-  x.qout[2][k] = hv;
+  x.qo0[k] = h;
+  x.qo1[k] = hu;
+  x.qo2[k] = hv;
   if (!(fg & GS_FLAG_CFL_SKIP)) {
     const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
     lam = lb_ > lam ? lb_ : lam;
@@ -224,7 +293,7 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
   uint64_t lam = 0ull;
   for (int64_t chunk = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + (threadIdx.x >> 6);
        chunk < a.nchunks; chunk += nwaves)
-    flat_chunk<T, MODE>(a, x, chunk, lane, lam);
+    flat_chunk<T, MODE>(x, chunk, lane, lam);
   if (MODE == 1) grid_reduce_and_plan(lam, a.clock, a.plan, true);
 }
 
@@ -358,6 +427,15 @@ static int flat_cfl_t(const GsFlat* f, const GsPhys* p, GsForcing fo, GsClock* c
 }
 
 
+// Depth of the current state at listed flat ids (gauges), read after a step.
+template <typename T>
+__global__ void k_flat_sample(FlatK<T> a, const int64_t* ids, int n, T* out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int par = (int)(a.clock->nsteps & 1);
+  out[t] = a.buf[par][0][ids[t]];
+}
+
 // ---------------------------------------------------------------------------
 // batched ensemble: every member's chunks in one launch, per-member clocks
 // ---------------------------------------------------------------------------
@@ -386,7 +464,8 @@ __device__ __forceinline__ void ens_flush(const EnsK<T>& e, int m, uint64_t lam)
 // finish plans each member's next dt from its own maximum (plan_finish), as the plain
 // step's last workgroup does for its single domain.
 template <typename T>
-__global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e) {
+__global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>* __restrict__ mem,
+                                                      const int64_t* __restrict__ chunk0) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -394,13 +473,6 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e) {
   const int64_t cpp = e.cpp;
   int m = -1;
   int64_t c0 = 0, c1 = 0;
-  bool live = false;
-  FlatCtx<T> x;
-  x.qin = nullptr;
-  x.qout = nullptr;
-  x.dt = x.rain_u = x.eta = T(0);
-  x.frame = nullptr;
-  x.ntab = nullptr;
   uint64_t lam = 0ull;
   for (int64_t base = wave * cpp; base < e.nchunks; base += nwaves * cpp) {
     const int64_t stop = base + cpp < e.nchunks ? base + cpp : e.nchunks;
@@ -410,13 +482,16 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e) {
         lam = 0ull;
         do {
           ++m;
-          c0 = e.chunk0[m];
-          c1 = e.chunk0[m + 1];
+          c0 = chunk0[m];
+          c1 = chunk0[m + 1];
         } while (gc >= c1);
-        live = flat_ctx<T, 1>(e.mem[m], x);
-        x.ntab = (const T*)e.mem[m].ntab;
       }
-      if (live) flat_chunk<T, 1>(e.mem[m], x, gc - c0, lane, lam);
+      // the member's context, rebuilt per chunk from the (scalar-cached) member table so that
+      // it stays a register temporary
+      FlatCtx<T> x;
+      if (!flat_ctx<T, 1>(mem[m], x)) continue;
+      x.ntab = mem[m].ntab;
+      flat_chunk<T, 1>(x, gc - c0, lane, lam);
     }
   }
   ens_flush(e, m, lam);
@@ -472,6 +547,58 @@ int gs_flat_cfl(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* cl
   if (f->dtype == GS_F32) return flat_cfl_t<float>(f, phys, forc, clock_dev, (hipStream_t)stream);
   if (f->dtype == GS_F64) return flat_cfl_t<double>(f, phys, forc, clock_dev, (hipStream_t)stream);
   return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
+}
+
+int gs_flat_step_pre(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev, void* pre0,
+                     void* pre1, void* pre2, void* stream) {
+  using namespace gs;
+  if (!pre0 || !pre1 || !pre2) return fail(GS_E_ARG, "pre arrays are NULL");
+  if (!f) return fail(GS_E_ARG, "flat descriptor is NULL");
+  if (f->dtype == GS_F64) {
+    FlatK<double> a;
+    int rc = build_flat<double>(f, phys, forc, clock_dev, a, true);
+    if (rc) return rc;
+    a.pre[0] = (double*)pre0; a.pre[1] = (double*)pre1; a.pre[2] = (double*)pre2;
+    hipLaunchKernelGGL((k_flat_step<double, 1>), dim3(flat_grid<double, 1>(a.nchunks)), dim3(kFBlock), 0,
+                       (hipStream_t)stream, a);
+  } else if (f->dtype == GS_F32) {
+    FlatK<float> a;
+    int rc = build_flat<float>(f, phys, forc, clock_dev, a, true);
+    if (rc) return rc;
+    a.pre[0] = (float*)pre0; a.pre[1] = (float*)pre1; a.pre[2] = (float*)pre2;
+    hipLaunchKernelGGL((k_flat_step<float, 1>), dim3(flat_grid<float, 1>(a.nchunks)), dim3(kFBlock), 0,
+                       (hipStream_t)stream, a);
+  } else {
+    return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
+  }
+  GS_HIP(hipGetLastError(), "k_flat_step(pre) launch");
+  return GS_OK;
+}
+
+int gs_flat_sample(const GsFlat* f, const GsClock* clock_dev, const int64_t* ids, int n, void* out, void* stream) {
+  using namespace gs;
+  if (!f || !clock_dev || (n > 0 && (!ids || !out))) return fail(GS_E_ARG, "NULL argument");
+  if (n < 0) return fail(GS_E_ARG, "n < 0");
+  if (n == 0) return GS_OK;
+  GsPhys p;
+  memset(&p, 0, sizeof p);
+  p.g = 9.81; p.h_min = 1e-6; p.h_cfl = 1e-6; p.cfl = 0.5; p.dmin = 1.0; p.n_classes = 1;
+  GsForcing fo;
+  memset(&fo, 0, sizeof fo);
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  if (f->dtype == GS_F64) {
+    FlatK<double> a;
+    int rc = build_flat<double>(f, &p, fo, (GsClock*)clock_dev, a, true);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_flat_sample<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, a, ids, n, (double*)out);
+  } else {
+    FlatK<float> a;
+    int rc = build_flat<float>(f, &p, fo, (GsClock*)clock_dev, a, true);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_flat_sample<float>, dim3(nb), dim3(256), 0, (hipStream_t)stream, a, ids, n, (float*)out);
+  }
+  GS_HIP(hipGetLastError(), "k_flat_sample launch");
+  return GS_OK;
 }
 
 
@@ -539,7 +666,7 @@ static int ens_launch(GsEns* e, hipStream_t s) {
   const int64_t need = (e->nchunks + 4LL * e->cpp - 1) / (4LL * e->cpp);
   if (blocks > need) blocks = need;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_ens_step<T>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k);
+  hipLaunchKernelGGL((k_ens_step<T>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k, k.mem, k.chunk0);
   GS_HIP(hipGetLastError(), "k_ens_step launch");
   return GS_OK;
 }
@@ -549,8 +676,8 @@ static int ens_step(GsEns* e, hipStream_t s) {
 }
 
 static void ens_drop_graph(GsEns* e) {
-  if (e->exec) hipGraphExecDestroy(e->exec);
-  if (e->graph) hipGraphDestroy(e->graph);
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
   e->exec = nullptr;
   e->graph = nullptr;
   e->g_batch = 0;
@@ -564,7 +691,7 @@ static int ens_capture(GsEns* e, int batch) {
   hipGraph_t g = nullptr;
   const hipError_t ee = hipStreamEndCapture(e->gstream, &g);
   if (rc) {
-    if (g) hipGraphDestroy(g);
+    if (g) (void)hipGraphDestroy(g);
     return rc;
   }
   if (ee != hipSuccess) return hip_fail(ee, "end capture");
@@ -599,7 +726,7 @@ int gs_ens_create(const GsFlat* members, const GsPhys* phys, const GsForcing* fo
   e->flat.assign(members, members + n_members);
   e->clocks.assign(clocks, clocks + n_members);
   int dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess) hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e->cus < 1) e->cus = 256;
   e->occ = dt == GS_F64 ? ens_occupancy<double>() : ens_occupancy<float>();
   int rc = dt == GS_F64 ? ens_build_views<double>(e, members, phys, forcings)
@@ -621,13 +748,13 @@ int gs_ens_create(const GsFlat* members, const GsPhys* phys, const GsForcing* fo
 int gs_ens_destroy(GsEns* e) {
   if (!e) return GS_OK;
   gs::ens_drop_graph(e);
-  if (e->gstream) hipStreamSynchronize(e->gstream);
-  if (e->ev_in) hipEventDestroy(e->ev_in);
-  if (e->ev_out) hipEventDestroy(e->ev_out);
-  if (e->gstream) hipStreamDestroy(e->gstream);
-  if (e->mem) hipFree(e->mem);
-  if (e->chunk0) hipFree(e->chunk0);
-  if (e->done) hipFree(e->done);
+  if (e->gstream) (void)hipStreamSynchronize(e->gstream);
+  if (e->ev_in) (void)hipEventDestroy(e->ev_in);
+  if (e->ev_out) (void)hipEventDestroy(e->ev_out);
+  if (e->gstream) (void)hipStreamDestroy(e->gstream);
+  if (e->mem) (void)hipFree(e->mem);
+  if (e->chunk0) (void)hipFree(e->chunk0);
+  if (e->done) (void)hipFree(e->done);
   delete e;
   return GS_OK;
 }
@@ -703,7 +830,7 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
   std::vector<void*> snap;
   int rc = GS_OK;
   auto cleanup = [&]() {
-    for (void* p : snap) hipFree(p);
+    for (void* p : snap) (void)hipFree(p);
     snap.clear();
   };
   for (int m = 0; m < e->n && rc == GS_OK; ++m) {
@@ -741,8 +868,8 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
   int per_cu[4] = {1, 2, occ > 1 ? occ / 2 : 1, occ};
   const int cpps[4] = {1, 2, 4, 8};
   hipEvent_t t0 = nullptr, t1 = nullptr;
-  hipEventCreate(&t0);
-  hipEventCreate(&t1);
+  (void)hipEventCreate(&t0);
+  (void)hipEventCreate(&t1);
   double best = 1e300;
   int best_b = e->blocks, best_c = e->cpp;
   for (int a = 0; a < 4 && rc == GS_OK; ++a) {
@@ -766,8 +893,8 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
       }
     }
   }
-  hipEventDestroy(t0);
-  hipEventDestroy(t1);
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
   e->blocks = best_b;
   e->cpp = best_c;
   const int rc2 = restore();

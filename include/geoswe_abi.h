@@ -239,6 +239,49 @@ int gs_flat_residual(const GsFlat* f, const GsPhys* phys, int which, void* rhs0,
 int gs_flat_cfl(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev,
                 void* stream);
 
+/* Replaces: nothing in the reference (the adjoint recomputation needs it): gs_flat_step
+ * that also stores the pre-friction state h + dt*rr0, hu + dt*r1, hv + dt*r2 of active
+ * cells into pre0..2 (flat arrays of n_stored), the input of the update's reverse. */
+int gs_flat_step_pre(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev, void* pre0,
+                     void* pre1, void* pre2, void* stream);
+/* Depth of the current state (q[clock->nsteps & 1]) at n flat ids, into out (state dtype):
+ * gauge sampling, the device side of reading s.depth() at gauge cells. */
+int gs_flat_sample(const GsFlat* f, const GsClock* clock_dev, const int64_t* ids, int n, void* out,
+                   void* stream);
+
+/* ---- reverse (adjoint) step of the flat layout ---- */
+/*
+ * No reference counterpart (the reference has no adjoint). One reverse step of the flat
+ * fused step with a frozen dt: the cotangent of q_{s+1} (lam, in) becomes that of q_s
+ * (lam, out) and the cotangents of the per-class Manning n (nbar) and of the per-cell bed
+ * (bbar) accumulate. Stored cells carry a full 4-neighbour table; w, xf, yf and gz are
+ * scratch that must be zero-initialised once (w, xf, yf) before the first reverse step.
+ */
+typedef struct GsAdjFlat {
+  int64_t n_stored;
+  int32_t dtype;
+  int32_t pad;
+  double dx, dy;
+  const uint8_t* active;     /* [N] 1 for cells the step updates */
+  const int32_t* nbr;        /* [N][4] flat ids of (i+1, j), (i-1, j), (i, j+1), (i, j-1); -1: not stored */
+  const void* bed;           /* [N] */
+  const uint8_t* flags;      /* [N] GS_FLAG_* (class, stage) */
+  const void* ntab;          /* Manning n per class */
+  const void* q[3];          /* state q_s */
+  const void* h_next;        /* depth of q_{s+1} (stage cells) */
+  const void* pre[3];        /* pre-friction state of the step s -> s+1 (gs_flat_step_pre) */
+  void* lam[3];              /* cotangent of q_{s+1} in, of q_s out */
+  void* w[3];                /* scratch [N] each */
+  void* xf;                  /* scratch [10][N] */
+  void* yf;                  /* scratch [10][N] */
+  void* gz;                  /* scratch [2][N] */
+  double* bbar;              /* [N] accumulated d(loss)/d(bed) */
+  double* nbar;              /* [n_classes] accumulated d(loss)/d(n) */
+} GsAdjFlat;
+int gs_adj_flat_step(const GsAdjFlat* a, const GsPhys* phys, double dt, void* stream);
+/* lam_h[ids[k]] += coef[k]: the loss's direct cotangent of a state (gauge misfit). */
+int gs_adj_add(void* lam_h, int dtype, const int64_t* ids, const void* coef, int n, void* stream);
+
 /* ---- batched ensemble of flat domains ---- */
 /*
  * Independent flat domains (members of uneven size, each with its own bed, Manning table,

@@ -1,0 +1,101 @@
+"""The device adjoint of the gauge misfit (float64 build) against autograd through the
+float64 torch restatement of the step, against central finite differences of the device
+forward, and through the torch.autograd.Function wrapper."""
+import numpy as np
+import pytest
+
+import adjoint_cases as AC
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def problem(gpu):
+    return AC.coastal_problem(n=48, S=40, K=16, dtype="float64")
+
+
+def test_forward_matches_torch_and_adjoint_matches_autograd(problem):
+    torch = pytest.importorskip("torch")
+    case, prob = problem
+    n0, b0 = prob.params()
+    prob.simulate()
+    J = prob.loss()
+    gn, gb = prob.gradient()
+    nt = torch.tensor(n0, dtype=torch.float64, requires_grad=True)
+    bt = torch.tensor(b0, dtype=torch.float64, requires_grad=True)
+    Jt, samples = AC.torch_misfit(case, prob, nt, bt)
+    assert np.allclose(samples, prob.samples, rtol=1e-10, atol=1e-13)
+    assert abs(float(Jt.detach()) - J) <= 1e-9 * J and J > 0
+    Jt.backward()
+    tn, tb = nt.grad.numpy(), bt.grad.numpy()
+    assert np.abs(tn).max() > 0 and np.abs(tb).max() > 0
+    assert np.abs(gn - tn).max() <= 1e-7 * np.abs(tn).max(), (gn, tn)
+    assert np.linalg.norm(gb - tb) <= 1e-7 * np.linalg.norm(tb)
+    assert np.abs(gb - tb).max() <= 1e-6 * np.abs(tb).max()
+
+
+@pytest.fixture(scope="module")
+def flowing(gpu):
+    return AC.flowing_problem()
+
+
+def test_adjoint_matches_central_differences(flowing):
+    prob = flowing
+    n0, b0 = prob.params()
+    prob.simulate()
+    gn, gb = prob.gradient()
+
+    def J(n=None, b=None):
+        prob.set_params(ntab=n if n is not None else n0, bed=b if b is not None else b0)
+        prob.simulate(keep_checkpoints=False)
+        return prob.loss()
+
+    eps = 1e-6
+    for k in range(len(n0)):
+        e = np.zeros_like(n0)
+        e[k] = eps
+        fd = (J(n=n0 + e) - J(n=n0 - e)) / (2 * eps)
+        assert abs(gn[k] - fd) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(gn).max()), (k, gn[k], fd)
+    # bed cells where the gradient is largest (near the gauges) and a few seeded ones, away
+    # from exact bed ties with a neighbour (a frozen ghost ring copies the edge bed, so
+    # b_R - b_L = 0 exactly there: a kink of max(b_R - b_L, 0) whose one-sided derivatives
+    # differ and which a central difference averages)
+    from geoswe_amd.adjoint import flat_neighbours
+    nb = flat_neighbours(prob.s.fm)
+    gap = np.min(np.where(nb >= 0, np.abs(b0[:, None] - b0[np.maximum(nb, 0)]), np.inf), axis=1)
+    ok = gap > 1e-3
+    rng = np.random.default_rng(3)
+    order = [c for c in np.argsort(-np.abs(gb)) if ok[c]][:8]
+    cells = np.unique(np.concatenate([order, rng.choice(np.flatnonzero(prob.s.fm.active & ok), 4,
+                                                        replace=False)]))
+    scale = np.abs(gb).max()
+    compared = 0
+    for c in cells:
+        e = np.zeros_like(b0)
+        e[c] = eps
+        fd = (J(b=b0 + e) - J(b=b0 - e)) / (2 * eps)
+        fd4 = (J(b=b0 + e / 4) - J(b=b0 - e / 4)) / (eps / 2)
+        # the difference quotient carries ~1e-9 of float64 rounding of J; a state-dependent
+        # kink (a wet/dry switch) within +-eps shows as fd != fd4 beyond that
+        ref = max(abs(fd), 1e-2 * scale)
+        if abs(fd - fd4) > 1e-4 * ref:
+            continue
+        compared += 1
+        assert abs(gb[c] - fd) <= 1e-4 * ref, (c, gb[c], fd)
+    assert compared >= 10, compared
+    prob.set_params(ntab=n0, bed=b0)
+
+
+def test_autograd_function_wrapper(problem):
+    torch = pytest.importorskip("torch")
+    from geoswe_amd.adjoint import GaugeMisfitFn
+    case, prob = problem
+    n0, b0 = prob.params()
+    nt = torch.tensor(n0, requires_grad=True)
+    bt = torch.tensor(b0, requires_grad=True)
+    J = GaugeMisfitFn.apply(nt, bt, prob)
+    (2.0 * J).backward()
+    prob.simulate()
+    gn, gb = prob.gradient()
+    assert np.allclose(nt.grad.numpy(), 2.0 * gn, rtol=1e-12, atol=0)
+    assert np.allclose(bt.grad.numpy(), 2.0 * gb, rtol=1e-12, atol=0)
