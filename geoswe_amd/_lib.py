@@ -68,7 +68,8 @@ class GsAdjFlat(C.Structure):
                 ("dy", C.c_double), ("active", C.c_void_p), ("nbr", C.c_void_p), ("bed", C.c_void_p),
                 ("flags", C.c_void_p), ("ntab", C.c_void_p), ("q", C.c_void_p * 3), ("h_next", C.c_void_p),
                 ("pre", C.c_void_p * 3), ("lam", C.c_void_p * 3), ("w", C.c_void_p * 3), ("xf", C.c_void_p),
-                ("yf", C.c_void_p), ("gz", C.c_void_p), ("bbar", C.c_void_p), ("nbar", C.c_void_p)]
+                ("yf", C.c_void_p), ("gz", C.c_void_p), ("gz_prev", C.c_void_p), ("bbar", C.c_void_p),
+                ("nbar", C.c_void_p)]
 
 
 # symbol -> (restype, argtypes); every entry point returns int
@@ -105,6 +106,12 @@ _SIGS = {
     "gs_flat_sample": (C.c_int, [C.POINTER(GsFlat), _P, _P, C.c_int, _P, _P]),
     "gs_adj_flat_step": (C.c_int, [C.POINTER(GsAdjFlat), C.POINTER(GsPhys), C.c_double, _P]),
     "gs_adj_add": (C.c_int, [_P, C.c_int, _P, _P, C.c_int, _P]),
+    "gs_flat_run_frozen": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, _P, C.c_int, _P, _P, C.c_int,
+                                     _P, C.c_int, _P, _P]),
+    "gs_flat_replay_pre": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, _P, C.c_int, _P, _P, _P, _P]),
+    "gs_adj_flat_sweep": (C.c_int, [C.POINTER(GsAdjFlat), C.POINTER(GsPhys), C.c_int, _P, _P, _P, _P, _P, C.c_int,
+                                    _P, C.c_int64, _P]),
+    "gs_adj_flat_bedgrad": (C.c_int, [C.POINTER(GsAdjFlat), C.POINTER(GsPhys), _P]),
     "gs_ens_create": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), C.POINTER(GsForcing), C.POINTER(_P), C.c_int,
                                 C.POINTER(_P)]),
     "gs_ens_destroy": (C.c_int, [_P]),

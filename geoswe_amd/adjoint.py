@@ -75,6 +75,7 @@ class GaugeMisfit:
         if self.dts.ndim != 1 or not len(self.dts) or not np.all(self.dts > 0):
             raise ValueError("dts must be a non-empty 1-D array of positive steps")
         self.S = len(self.dts)
+        self._dts_c = np.ascontiguousarray(self.dts, np.float64)     # read by the C loops on the host
         self.K = int(checkpoint_every)
         if self.K < 1:
             raise ValueError("checkpoint_every must be >= 1")
@@ -138,16 +139,22 @@ class GaugeMisfit:
         """S frozen-dt steps from the initial state; gauge depths (S, G) after each step."""
         s = self.s
         self._reset(self._q0, self.t0)
-        d, p, clk = C.byref(s._desc()), C.byref(s._phys()), C.c_void_p(s._clock.ptr)
-        G = len(self.gauges)
-        es = self._d_samples.dtype.itemsize
-        for st in range(self.S):
-            if keep_checkpoints and st % self.K == 0:
-                self._save_ckpt(st // self.K)
-            self._set_dt(float(self.dts[st]), p)
-            L.call("gs_flat_step", d, p, s._fo.s, clk, None)
-            L.call("gs_flat_sample", d, clk, C.c_void_p(self._d_gauges.ptr), G,
-                   C.c_void_p(self._d_samples.ptr + st * G * es), None)
+        K = self.K
+        nseg = (self.S + K - 1) // K
+        ck = None
+        if keep_checkpoints:
+            for k in range(nseg):
+                if k not in self._ckpt:
+                    self._ckpt[k] = [L.DeviceArray((self.N,), self.dtype) for _ in range(3)]
+            ck = (C.c_void_p * (3 * nseg))(*[self._ckpt[k][m].ptr for k in range(nseg) for m in range(3)])
+            t = self.t0                     # the clock's own float64 sums, in the same order
+            for st in range(self.S):
+                if st % K == 0:
+                    self._ckpt_t[st // K] = t
+                t = t + float(self.dts[st])
+        L.call("gs_flat_run_frozen", C.byref(s._desc()), C.byref(s._phys()), s._fo.s, C.c_void_p(s._clock.ptr),
+               self.S, C.c_void_p(self._dts_c.ctypes.data), C.c_void_p(self._d_gauges.ptr), len(self.gauges),
+               C.c_void_p(self._d_samples.ptr), K, ck, None)
         L.sync(None)
         c = s._clockv()
         if c.status & L.ST_NONFINITE:
@@ -156,16 +163,6 @@ class GaugeMisfit:
             raise RuntimeError(f"GaugeMisfit: {c.nsteps} steps taken, {self.S} expected")
         self.samples = self._d_samples.download().astype(np.float64)
         return self.samples
-
-    def _save_ckpt(self, k: int) -> None:
-        s = self.s
-        cur = s.nsteps & 1
-        if k not in self._ckpt:
-            self._ckpt[k] = [L.DeviceArray((self.N,), self.dtype) for _ in range(3)]
-        for m in range(3):
-            L.call("gs_memcpy_d2d", C.c_void_p(self._ckpt[k][m].ptr), C.c_void_p(s._bufs[cur][m].ptr),
-                   C.c_size_t(self._q0[m].nbytes), None)
-        self._ckpt_t[k] = s.t
 
     def loss(self, samples: Optional[np.ndarray] = None) -> float:
         h = self.samples if samples is None else samples
@@ -187,7 +184,7 @@ class GaugeMisfit:
         w = [L.DeviceArray((N,), dt_, zero=True) for _ in range(3)]
         xf = L.DeviceArray((10, N), dt_, zero=True)
         yf = L.DeviceArray((10, N), dt_, zero=True)
-        gz = L.DeviceArray((2, N), dt_, zero=True)
+        gz = L.DeviceArray((2, 2, N), dt_, zero=True)     # double-buffered bed-gradient cotangents
         bbar = L.DeviceArray((N,), np.float64, zero=True)
         nbar = L.DeviceArray((L.MAX_CLASSES,), np.float64, zero=True)
         coef = (2.0 * (self.samples - self.h_star)).astype(dt_)          # (S, G): dJ/dh_s at gauges
@@ -210,34 +207,33 @@ class GaugeMisfit:
         L.call("gs_adj_add", C.c_void_p(lam[0].ptr), dcode, C.c_void_p(self._d_gauges.ptr),
                C.c_void_p(d_coef.ptr + (S - 1) * G * es), G, None)
         nbytes = self._q0[0].nbytes
+        # every slot of the segment store starts as the initial state: the step writes only
+        # active cells, and the ring cells are frozen at their initial values
+        for st_ in seg_states:
+            for m in range(3):
+                L.call("gs_memcpy_d2d", C.c_void_p(st_[m].ptr), C.c_void_p(self._q0[m].ptr), C.c_size_t(nbytes),
+                       None)
+        state_ptrs = (C.c_void_p * (3 * (K + 1)))(*[x[m].ptr for x in seg_states for m in range(3)])
+        pre_ptrs = (C.c_void_p * (3 * K))(*[x[m].ptr for x in seg_pre for m in range(3)])
         for seg in range(nseg - 1, -1, -1):
             s0 = seg * K
             n_in = min(K, S - s0)
-            # recompute the segment from its checkpoint, keeping states and pre-friction states
+            # recompute the segment from its checkpoint straight into the store (step j reads
+            # slot j, writes slot j + 1 and keeps its pre-friction state), then sweep it back
             self._reset(self._ckpt[seg], self._ckpt_t[seg])
             for m in range(3):
                 L.call("gs_memcpy_d2d", C.c_void_p(seg_states[0][m].ptr), C.c_void_p(self._ckpt[seg][m].ptr),
                        C.c_size_t(nbytes), None)
-            d = C.byref(desc)
-            for j in range(n_in):
-                self._set_dt(float(self.dts[s0 + j]), p)
-                L.call("gs_flat_step_pre", d, p, s._fo.s, clk, C.c_void_p(seg_pre[j][0].ptr),
-                       C.c_void_p(seg_pre[j][1].ptr), C.c_void_p(seg_pre[j][2].ptr), None)
-                cur = (j + 1) & 1
-                for m in range(3):
-                    L.call("gs_memcpy_d2d", C.c_void_p(seg_states[j + 1][m].ptr),
-                           C.c_void_p(s._bufs[cur][m].ptr), C.c_size_t(nbytes), None)
-            # reverse steps of the segment
-            for j in range(n_in - 1, -1, -1):
-                st = s0 + j
-                for m in range(3):
-                    a.q[m] = seg_states[j][m].ptr
-                    a.pre[m] = seg_pre[j][m].ptr
-                a.h_next = seg_states[j + 1][0].ptr
-                L.call("gs_adj_flat_step", C.byref(a), p, C.c_double(float(self.dts[st])), None)
-                if st >= 1:     # dJ/dh_st at gauges (the state after step st)
-                    L.call("gs_adj_add", C.c_void_p(lam[0].ptr), dcode, C.c_void_p(self._d_gauges.ptr),
-                           C.c_void_p(d_coef.ptr + (st - 1) * G * es), G, None)
+            dptr = C.c_void_p(self._dts_c.ctypes.data + 8 * s0)
+            L.call("gs_flat_replay_pre", C.byref(desc), p, s._fo.s, clk, n_in, dptr, state_ptrs, pre_ptrs, None)
+            # the loss term of the state after step s0 + j, for s0 + j >= 1
+            coef_ptrs = (C.c_void_p * n_in)(*[(d_coef.ptr + (s0 + j - 1) * G * es) if s0 + j >= 1 else None
+                                              for j in range(n_in)])
+            L.call("gs_adj_flat_sweep", C.byref(a), p, n_in, dptr, state_ptrs, pre_ptrs,
+                   C.c_void_p(self._d_gauges.ptr), coef_ptrs, G, C.c_void_p(gz.ptr), C.c_int64(s0), None)
+        # the bed-gradient term of the last reverse step (step 0 wrote half 0)
+        a.gz, a.gz_prev = gz.ptr, None
+        L.call("gs_adj_flat_bedgrad", C.byref(a), p, None)
         L.sync(None)
         return nbar.download()[:self.n_classes].copy(), bbar.download().copy()
 

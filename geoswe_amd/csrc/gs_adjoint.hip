@@ -41,6 +41,7 @@ struct AdjK {
   T* xf;                  // [10][n]
   T* yf;                  // [10][n]
   T* gz;                  // [2][n]
+  const T* gz_prev;       // [2][n] of the previous reverse step, or NULL
   double* bbar;
   double* nbar;
   Phys<T> P;
@@ -162,6 +163,15 @@ __global__ __launch_bounds__(kABlock) void k_adj_gather(AdjK<T> a) {
     if (S >= 0) { gzy += y[9 * n + S]; bb += y[8 * n + S]; }
     a.gz[c] = gzx;
     a.gz[n + c] = gzy;
+    if (a.gz_prev) {   // the bed-gradient term of the previous reverse step (its gz is complete)
+      const T hx = T(0.5) * a.P.inv_dx, hy = T(0.5) * a.P.inv_dy;
+      const int64_t E = nb[0], Nn = nb[2];
+      const T* gp_ = a.gz_prev;
+      if (W >= 0) bb += hx * gp_[W];
+      if (E >= 0) bb -= hx * gp_[E];
+      if (S >= 0) bb += hy * gp_[n + S];
+      if (Nn >= 0) bb -= hy * gp_[n + Nn];
+    }
     a.bbar[c] += (double)bb;
     if (a.active[c]) {
       for (int k = 0; k < 3; ++k) {
@@ -216,6 +226,7 @@ static int adj_step_t(const GsAdjFlat* f, const GsPhys* p, double dt, hipStream_
   a.xf = (T*)f->xf;
   a.yf = (T*)f->yf;
   a.gz = (T*)f->gz;
+  a.gz_prev = (const T*)f->gz_prev;
   a.bbar = f->bbar;
   a.nbar = f->nbar;
   a.P = make_phys<T>(*p, f->dx, f->dy);
@@ -227,8 +238,27 @@ static int adj_step_t(const GsAdjFlat* f, const GsPhys* p, double dt, hipStream_
   hipLaunchKernelGGL(k_adj_update<T>, gr, bl, 0, s, a);
   hipLaunchKernelGGL(k_adj_faces<T>, gr, bl, 0, s, a);
   hipLaunchKernelGGL(k_adj_gather<T>, gr, bl, 0, s, a);
-  hipLaunchKernelGGL(k_adj_bedgrad<T>, gr, bl, 0, s, a);
+  // with gz_prev the caller folds this step's bed-gradient term into the next reverse step's
+  // gather (double-buffered gz) and flushes the last one with gs_adj_flat_bedgrad
+  if (!a.gz_prev) hipLaunchKernelGGL(k_adj_bedgrad<T>, gr, bl, 0, s, a);
   GS_HIP(hipGetLastError(), "adjoint step launch");
+  return GS_OK;
+}
+
+template <typename T>
+static int adj_bedgrad_t(const GsAdjFlat* f, const GsPhys* p, hipStream_t s) {
+  AdjK<T> a;
+  memset(&a, 0, sizeof a);
+  a.n = f->n_stored;
+  a.nbr = f->nbr;
+  a.gz = (T*)f->gz;
+  a.bbar = f->bbar;
+  a.P = make_phys<T>(*p, f->dx, f->dy);
+  int64_t blocks = (a.n + kABlock - 1) / kABlock;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_adj_bedgrad<T>, dim3((unsigned)blocks), dim3(kABlock), 0, s, a);
+  GS_HIP(hipGetLastError(), "k_adj_bedgrad launch");
   return GS_OK;
 }
 
@@ -253,6 +283,15 @@ int gs_adj_flat_step(const GsAdjFlat* f, const GsPhys* phys, double dt, void* st
   return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
 }
 
+int gs_adj_flat_bedgrad(const GsAdjFlat* f, const GsPhys* phys, void* stream) {
+  using namespace gs;
+  if (!f || !phys || !f->nbr || !f->gz || !f->bbar) return fail(GS_E_ARG, "NULL argument");
+  if (f->n_stored < 1) return fail(GS_E_RANGE, "n_stored < 1");
+  if (f->dtype == GS_F64) return adj_bedgrad_t<double>(f, phys, (hipStream_t)stream);
+  if (f->dtype == GS_F32) return adj_bedgrad_t<float>(f, phys, (hipStream_t)stream);
+  return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
+}
+
 int gs_adj_add(void* lam_h, int dtype, const int64_t* ids, const void* coef, int n, void* stream) {
   using namespace gs;
   if (n < 0) return fail(GS_E_ARG, "n < 0");
@@ -268,6 +307,77 @@ int gs_adj_add(void* lam_h, int dtype, const int64_t* ids, const void* coef, int
   else
     return fail(GS_E_DTYPE, "unsupported dtype %d", dtype);
   GS_HIP(hipGetLastError(), "k_adj_add launch");
+  return GS_OK;
+}
+
+int gs_flat_run_frozen(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev, int n,
+                       const double* dts, const int64_t* gauge_ids, int n_gauges, void* samples, int ckpt_every,
+                       void* const* ckpt, void* stream) {
+  using namespace gs;
+  if (!f || !phys || !clock_dev || (n > 0 && !dts)) return fail(GS_E_ARG, "NULL argument");
+  if (n < 0 || n_gauges < 0) return fail(GS_E_ARG, "negative count");
+  if (n_gauges > 0 && (!gauge_ids || !samples)) return fail(GS_E_ARG, "gauge arrays are NULL");
+  if (ckpt && ckpt_every < 1) return fail(GS_E_ARG, "ckpt_every must be >= 1");
+  const size_t es = f->dtype == GS_F64 ? 8 : 4;
+  const size_t bytes = (size_t)f->n_stored * es;
+  hipStream_t s = (hipStream_t)stream;
+  for (int j = 0; j < n; ++j) {
+    if (ckpt && j % ckpt_every == 0) {   // the clock counts from 0: the state is q[j & 1]
+      const int c = j / ckpt_every;
+      for (int k = 0; k < 3; ++k)
+        GS_HIP(hipMemcpyAsync(ckpt[3 * c + k], f->q[j & 1][k], bytes, hipMemcpyDeviceToDevice, s), "checkpoint");
+    }
+    int rc = gs_clock_set_dt(clock_dev, dts[j], phys, forc, stream);
+    if (!rc) rc = gs_flat_step(f, phys, forc, clock_dev, stream);
+    if (!rc && n_gauges > 0)
+      rc = gs_flat_sample(f, clock_dev, gauge_ids, n_gauges, (char*)samples + (size_t)j * n_gauges * es, stream);
+    if (rc) return rc;
+  }
+  return GS_OK;
+}
+
+int gs_flat_replay_pre(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev, int n,
+                       const double* dts, void* const* states, void* const* pre, void* stream) {
+  using namespace gs;
+  if (!f || !phys || !clock_dev || (n > 0 && (!dts || !states || !pre))) return fail(GS_E_ARG, "NULL argument");
+  if (n < 0) return fail(GS_E_ARG, "n < 0");
+  GsFlat d = *f;
+  for (int j = 0; j < n; ++j) {
+    const int par = j & 1;   // the clock counts the replayed steps from 0
+    for (int k = 0; k < 3; ++k) {
+      d.q[par][k] = states[3 * j + k];
+      d.q[par ^ 1][k] = states[3 * (j + 1) + k];
+    }
+    int rc = gs_clock_set_dt(clock_dev, dts[j], phys, forc, stream);
+    if (!rc) rc = gs_flat_step_pre(&d, phys, forc, clock_dev, pre[3 * j], pre[3 * j + 1], pre[3 * j + 2], stream);
+    if (rc) return rc;
+  }
+  return GS_OK;
+}
+
+int gs_adj_flat_sweep(const GsAdjFlat* a, const GsPhys* phys, int n, const double* dts, const void* const* states,
+                      const void* const* pre, const int64_t* gauge_ids, const void* const* coef, int n_gauges,
+                      void* gz_pair, int64_t first_step, void* stream) {
+  using namespace gs;
+  if (!a || !phys || (n > 0 && (!dts || !states || !pre))) return fail(GS_E_ARG, "NULL argument");
+  if (n < 0 || n_gauges < 0 || first_step < 0) return fail(GS_E_ARG, "negative count");
+  GsAdjFlat d = *a;
+  const size_t half = (size_t)2 * (size_t)a->n_stored * (a->dtype == GS_F64 ? 8 : 4);
+  for (int j = n - 1; j >= 0; --j) {
+    for (int k = 0; k < 3; ++k) {
+      d.q[k] = states[3 * j + k];
+      d.pre[k] = pre[3 * j + k];
+    }
+    d.h_next = states[3 * (j + 1)];
+    if (gz_pair) {   // step s writes half s & 1 and folds in the half of step s + 1
+      const int64_t st = first_step + j;
+      d.gz = (char*)gz_pair + (size_t)(st & 1) * half;
+      d.gz_prev = (char*)gz_pair + (size_t)((st + 1) & 1) * half;
+    }
+    int rc = gs_adj_flat_step(&d, phys, dts[j], stream);
+    if (!rc && coef && coef[j] && n_gauges > 0) rc = gs_adj_add(d.lam[0], d.dtype, gauge_ids, coef[j], n_gauges, stream);
+    if (rc) return rc;
+  }
   return GS_OK;
 }
 

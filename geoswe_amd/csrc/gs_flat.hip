@@ -101,39 +101,58 @@ __device__ __forceinline__ GS_G P* gp(P* p) {
   return (GS_G P*)p;
 }
 
-// Reads the clock and the layout into a step context; false when the clock says the run
-// is done.
+// The clock's per-step values for one domain (read once per launch, or once per member
+// switch in the ensemble).
+template <typename T>
+struct FlatClk {
+  bool live;        // false when the clock says the run is done
+  int par;          // input buffer (MODE 1: nsteps & 1; MODE 0: the residual selector)
+  T dt, rain_u, eta;
+  const GS_G T* frame;
+};
+
 template <typename T, int MODE>
-__device__ __forceinline__ bool flat_ctx(const FlatK<T>& a, FlatCtx<T>& x) {
-  int par = 0;
-  x.dt = T(0);
-  x.rain_u = T(0);
-  x.eta = T(0);
-  x.frame = nullptr;
-  x.rain_mode = a.rain_mode;
+__device__ __forceinline__ FlatClk<T> flat_clock(const FlatK<T>& a) {
+  FlatClk<T> k;
+  k.live = true;
+  k.par = MODE == 1 ? 0 : a.which;
+  k.dt = T(0);
+  k.rain_u = T(0);
+  k.eta = T(0);
+  k.frame = nullptr;
   if (MODE == 1) {
     const GS_G GsClock* c = gp(a.clock);
-    if (c->status & GS_ST_DONE) return false;
-    par = (int)(c->nsteps & 1);
+    if (c->status & GS_ST_DONE) {
+      k.live = false;
+      return k;
+    }
+    k.par = (int)(c->nsteps & 1);
     const int ridx = c->rain_idx;
     if (sizeof(T) == 4) {
-      x.dt = (T)c->dt32;
-      x.rain_u = (T)c->rain32;
-      x.eta = (T)c->eta32;
+      k.dt = (T)c->dt32;
+      k.rain_u = (T)c->rain32;
+      k.eta = (T)c->eta32;
     } else {
-      x.dt = (T)c->dt;
-      x.rain_u = a.rain_mode == 1 ? (T)gp(a.plan.rain_rate)[ridx] : T(0);
-      x.eta = (T)c->eta64;
+      k.dt = (T)c->dt;
+      k.rain_u = a.rain_mode == 1 ? (T)gp(a.plan.rain_rate)[ridx] : T(0);
+      k.eta = (T)c->eta64;
     }
-    if (a.rain_mode == 2) x.frame = gp(a.rain_frames) + (int64_t)ridx * a.frame_stride;
+    if (a.rain_mode == 2) k.frame = gp(a.rain_frames) + (int64_t)ridx * a.frame_stride;
   }
-  const bool in1 = (MODE == 1 ? par : a.which) != 0;
+  return k;
+}
+
+// The step context: the layout of `a` with the buffers the clock values pick.
+template <typename T, int MODE>
+__device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k, FlatCtx<T>& x) {
+  const bool in1 = k.par != 0;
+  const bool out1 = MODE == 1 ? !in1 : true;   // MODE 0 writes no state
   x.qi0 = gp(in1 ? a.buf[1][0] : a.buf[0][0]);
   x.qi1 = gp(in1 ? a.buf[1][1] : a.buf[0][1]);
   x.qi2 = gp(in1 ? a.buf[1][2] : a.buf[0][2]);
-  x.qo0 = gp(par ? a.buf[0][0] : a.buf[1][0]);
-  x.qo1 = gp(par ? a.buf[0][1] : a.buf[1][1]);
-  x.qo2 = gp(par ? a.buf[0][2] : a.buf[1][2]);
+  x.qo0 = gp(out1 ? a.buf[1][0] : a.buf[0][0]);
+  x.qo1 = gp(out1 ? a.buf[1][1] : a.buf[0][1]);
+  x.qo2 = gp(out1 ? a.buf[1][2] : a.buf[0][2]);
   x.bed = gp(a.bed);
   x.flags = gp(a.flags);
   x.n_field = gp(a.n_field);
@@ -150,12 +169,17 @@ __device__ __forceinline__ bool flat_ctx(const FlatK<T>& a, FlatCtx<T>& x) {
   x.run_j0 = gp(a.run_j0);
   x.run_d = gp(a.run_d);
   x.chunk_run = gp(a.chunk_run);
+  x.frame = k.frame;
+  x.ntab = a.ntab;
   x.n = a.n;
   x.rain_pitch = a.rain_pitch;
   x.n_runs = a.n_runs;
   x.ngh = a.ngh;
+  x.rain_mode = a.rain_mode;
+  x.dt = k.dt;
+  x.rain_u = k.rain_u;
+  x.eta = k.eta;
   x.P = a.P;
-  return true;
 }
 
 // One wavefront pass over flat ids chunk*60-2 .. chunk*60+61: the residual (MODE 0) or the
@@ -282,8 +306,10 @@ __device__ __forceinline__ unsigned xcd_rank(unsigned b, unsigned nb) {
 // One step of one flat domain on a resident grid, each wavefront striding over chunks.
 template <typename T, int MODE>
 __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
+  const FlatClk<T> ck = flat_clock<T, MODE>(a);
+  if (!ck.live) return;   // the whole grid sees the same clock
   FlatCtx<T> x;
-  if (!flat_ctx<T, MODE>(a, x)) return;   // the whole grid sees the same clock
+  flat_ctx<T, MODE>(a, ck, x);
   __shared__ T s_ntab[32];
   if (MODE == 1 && threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
   __syncthreads();
@@ -474,6 +500,11 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>*
   int m = -1;
   int64_t c0 = 0, c1 = 0;
   uint64_t lam = 0ull;
+  FlatClk<T> ck;
+  ck.live = false;
+  ck.par = 0;
+  ck.dt = ck.rain_u = ck.eta = T(0);
+  ck.frame = nullptr;
   for (int64_t base = wave * cpp; base < e.nchunks; base += nwaves * cpp) {
     const int64_t stop = base + cpp < e.nchunks ? base + cpp : e.nchunks;
     for (int64_t gc = base; gc < stop; ++gc) {
@@ -485,12 +516,13 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>*
           c0 = chunk0[m];
           c1 = chunk0[m + 1];
         } while (gc >= c1);
+        ck = flat_clock<T, 1>(mem[m]);   // the member's clock values, once per switch
       }
-      // the member's context, rebuilt per chunk from the (scalar-cached) member table so that
+      if (!ck.live) continue;
+      // the member's layout, rebuilt per chunk from the (scalar-loaded) member table so that
       // it stays a register temporary
       FlatCtx<T> x;
-      if (!flat_ctx<T, 1>(mem[m], x)) continue;
-      x.ntab = mem[m].ntab;
+      flat_ctx<T, 1>(mem[m], ck, x);
       flat_chunk<T, 1>(x, gc - c0, lane, lam);
     }
   }

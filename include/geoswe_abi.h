@@ -275,12 +275,36 @@ typedef struct GsAdjFlat {
   void* xf;                  /* scratch [10][N] */
   void* yf;                  /* scratch [10][N] */
   void* gz;                  /* scratch [2][N] */
+  const void* gz_prev;       /* NULL, or the gz of the previous reverse step: its bed-gradient
+                                term is then folded into this step's gather pass */
   double* bbar;              /* [N] accumulated d(loss)/d(bed) */
   double* nbar;              /* [n_classes] accumulated d(loss)/d(n) */
 } GsAdjFlat;
 int gs_adj_flat_step(const GsAdjFlat* a, const GsPhys* phys, double dt, void* stream);
+/* The bed-gradient term of a->gz alone (the flush after the last step run with gz_prev). */
+int gs_adj_flat_bedgrad(const GsAdjFlat* a, const GsPhys* phys, void* stream);
 /* lam_h[ids[k]] += coef[k]: the loss's direct cotangent of a state (gauge misfit). */
 int gs_adj_add(void* lam_h, int dtype, const int64_t* ids, const void* coef, int n, void* stream);
+/* Forward pass of a calibration run: n frozen-dt steps (gs_clock_set_dt + gs_flat_step)
+ * from q[0] with the clock at step 0, the depth at n_gauges flat ids after every step into
+ * samples[n][n_gauges], and when ckpt is not NULL the state every ckpt_every steps
+ * (before step j) into ckpt[3*(j/ckpt_every) .. +2]. */
+int gs_flat_run_frozen(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev, int n,
+                       const double* dts, const int64_t* gauge_ids, int n_gauges, void* samples, int ckpt_every,
+                       void* const* ckpt, void* stream);
+/* Segment recomputation: n frozen-dt steps (gs_clock_set_dt + gs_flat_step_pre) where step j
+ * reads the state states[3j..3j+2] and writes states[3j+3..3j+5] (device pointers; the
+ * clock's step count starts at 0), keeping the pre-friction state in pre[3j..3j+2]. */
+int gs_flat_replay_pre(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev, int n,
+                       const double* dts, void* const* states, void* const* pre, void* stream);
+/* Reverse sweep of such a segment: for j = n-1 .. 0 the reverse step at dts[j] over
+ * states[3j..], the depth states[3j+3] and pre[3j..], then, when coef[j] is not NULL,
+ * gs_adj_add of coef[j] at the n_gauges gauge ids (the loss term of the state after step j).
+ * With gz_pair (two [2][N] halves) step first_step + j writes half (first_step + j) & 1 and
+ * folds in the other half's bed-gradient term; flush the last with gs_adj_flat_bedgrad. */
+int gs_adj_flat_sweep(const GsAdjFlat* a, const GsPhys* phys, int n, const double* dts, const void* const* states,
+                      const void* const* pre, const int64_t* gauge_ids, const void* const* coef, int n_gauges,
+                      void* gz_pair, int64_t first_step, void* stream);
 
 /* ---- batched ensemble of flat domains ---- */
 /*
