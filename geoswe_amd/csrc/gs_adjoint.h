@@ -48,86 +48,79 @@ GS_HD void hllc_vjp(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min, T bFh,
   else if (!(SM >= z) && !(SR >= z)) br = 1;
   else br = (SM >= z) ? 2 : 3;
 
-  // ---- reverse
-  T gSL = z, gSR = z, gSM = z;
-  if (br == 0) {
-    // Fh = hL uNL, FN = hL uNL uNL + (0.5 g hL) hL, FT = hL uNL uTL
-    const T huN = hL * uNL;
-    const T ghuN = bFh + bFN * uNL + bFT * uTL;
-    guNL += bFN * huN;
-    guTL += bFT * huN;
-    ghL += bFN * (g * hL);
-    ghL += ghuN * uNL;
-    guNL += ghuN * hL;
-  } else if (br == 1) {
-    const T huN = hR * uNR;
-    const T ghuN = bFh + bFN * uNR + bFT * uTR;
-    guNR += bFN * huN;
-    guTR += bFT * huN;
-    ghR += bFN * (g * hR);
-    ghR += ghuN * uNR;
-    guNR += ghuN * hR;
-  } else {
-    const bool lft = (br == 2);
-    const T S = lft ? SL : SR;
-    const T uN = lft ? uNL : uNR;
-    const T uT = lft ? uTL : uTR;
-    const T hh = lft ? hL : hR;
-    const T huN = hh * uN;
-    const T den = (S - SM) + T(1.0e-30);
-    const T fac = (S - uN) / den;
-    const T hs = hh * fac;
-    // Fh = huN + S (hs - hh); FN = F_N + S (hs SM - huN); FT = huN uT + S (hs uT - hh uT)
-    T gS = bFh * (hs - hh) + bFN * (hs * SM - huN) + bFT * (hs * uT - hh * uT);
-    const T ghs = bFh * S + bFN * (S * SM) + bFT * (S * uT);
-    T ghh = -(bFh * S) - bFT * (S * uT);
-    T ghuN = bFh - bFN * S + bFT * uT;
-    gSM += bFN * (S * hs);
-    const T guT = bFT * huN + bFT * (S * (hs - hh));
-    const T gF_N = bFN;
-    // hs = hh fac
-    ghh += ghs * fac;
-    const T gfac = ghs * hh;
-    // fac = (S - uN) / den
-    const T gnum = gfac / den;
-    const T gden = -(gfac * fac) / den;
-    gS += gnum + gden;
-    T guN = -gnum;
-    gSM -= gden;
-    // F_N = huN uN + (0.5 g hh) hh
-    ghuN += gF_N * uN;
-    guN += gF_N * huN;
-    ghh += gF_N * (g * hh);
-    // huN = hh uN
-    ghh += ghuN * uN;
-    guN += ghuN * hh;
-    if (lft) { ghL += ghh; guNL += guN; guTL += guT; gSL += gS; }
-    else { ghR += ghh; guNR += guN; guTR += guT; gSR += gS; }
-    // SM
-    if (big) {
-      const T gNn = gSM / denom;
-      const T gdn = -(gSM * SM) / denom;
-      gSL += gNn * (hR * (uNR - SR) + SR * hL) + gdn * hL;
-      gSR += gNn * (-(SL * hR) - hL * (uNL - SL)) - gdn * hR;
-      ghR += gNn * (SL * (uNR - SR)) + gdn * (uNR - SR);
-      ghL += gNn * (-(SR * (uNL - SL))) - gdn * (uNL - SL);
-      guNR += gNn * (SL * hR) + gdn * hR;
-      guNL += gNn * (-(SR * hL)) - gdn * hL;
-    } else {
-      guNL += T(0.5) * gSM;
-      guNR += T(0.5) * gSM;
-    }
-  }
-  // SL, SR
-  T gaL = z, gaR = z, gust = z, gast = z;
-  if (!both) {
-    if (dryL) { guNR += gSL; gaR -= T(2) * gSL; }
-    else if (xl <= yl) { guNL += gSL; gaL -= gSL; }
-    else { gust += gSL; gast -= gSL; }
-    if (dryR) { guNL += gSR; gaL += T(2) * gSR; }
-    else if (xr >= yr) { guNR += gSR; gaR += gSR; }
-    else { gust += gSR; gast += gSR; }
-  }
+  // ---- reverse, branch-free: the one-sided and the star contributions are both formed and
+  // combined with value selects (the unselected side adds an exact zero). A branch that picks
+  // which accumulator to update makes the compiler select between their addresses and keep
+  // them in memory.
+  const bool one = br <= 1;
+  // one-sided flux of side L (br 0) or R (br 1): Fh = h uN, FN = h uN uN + (0.5 g h) h,
+  // FT = h uN uT
+  const bool l = (br == 0);
+  const T h1 = l ? hL : hR, uN1 = l ? uNL : uNR, uT1 = l ? uTL : uTR;
+  const T huN1 = h1 * uN1;
+  const T ghuN1 = bFh + bFN * uN1 + bFT * uT1;
+  const T guN_1 = bFN * huN1 + ghuN1 * h1;
+  const T guT_1 = bFT * huN1;
+  const T gh_1 = bFN * (g * h1) + ghuN1 * uN1;
+  // star state of side lft (br 2: left, br 3: right)
+  const bool lft = (br == 2);
+  const T S = lft ? SL : SR;
+  const T uN = lft ? uNL : uNR;
+  const T uT = lft ? uTL : uTR;
+  const T hh = lft ? hL : hR;
+  const T huN = hh * uN;
+  const T den = (S - SM) + T(1.0e-30);
+  const T fac = one ? z : (S - uN) / den;
+  const T hs = hh * fac;
+  // Fh = huN + S (hs - hh); FN = F_N + S (hs SM - huN); FT = huN uT + S (hs uT - hh uT)
+  T gS = bFh * (hs - hh) + bFN * (hs * SM - huN) + bFT * (hs * uT - hh * uT);
+  const T ghs = bFh * S + bFN * (S * SM) + bFT * (S * uT);
+  T ghh = -(bFh * S) - bFT * (S * uT);
+  T ghuN = bFh - bFN * S + bFT * uT;
+  const T gSMs = bFN * (S * hs);
+  const T guT = bFT * huN + bFT * (S * (hs - hh));
+  const T gF_N = bFN;
+  ghh += ghs * fac;
+  const T gfac = ghs * hh;
+  const T gnum = one ? z : gfac / den;
+  const T gden = one ? z : -(gfac * fac) / den;
+  gS += gnum + gden;
+  T guN = -gnum;
+  ghuN += gF_N * uN;
+  guN += gF_N * huN;
+  ghh += gF_N * (g * hh);
+  ghh += ghuN * uN;
+  guN += ghuN * hh;
+  const bool sl = !one && lft, sr = !one && !lft;
+  ghL += l && one ? gh_1 : (sl ? ghh : z);
+  guNL += l && one ? guN_1 : (sl ? guN : z);
+  guTL += l && one ? guT_1 : (sl ? guT : z);
+  ghR += !l && one ? gh_1 : (sr ? ghh : z);
+  guNR += !l && one ? guN_1 : (sr ? guN : z);
+  guTR += !l && one ? guT_1 : (sr ? guT : z);
+  T gSL = sl ? gS : z, gSR = sr ? gS : z;
+  const T gSM = one ? z : gSMs - gden;
+  // SM = Nn / denom (|denom| > 1e-14) or 0.5 (uNL + uNR)
+  const T gNn = big ? gSM / denom : z;
+  const T gdn = big ? -(gSM * SM) / denom : z;
+  gSL += gNn * (hR * (uNR - SR) + SR * hL) + gdn * hL;
+  gSR += gNn * (-(SL * hR) - hL * (uNL - SL)) - gdn * hR;
+  ghR += gNn * (SL * (uNR - SR)) + gdn * (uNR - SR);
+  ghL += gNn * (-(SR * (uNL - SL))) - gdn * (uNL - SL);
+  guNR += gNn * (SL * hR) + gdn * hR + (big ? z : T(0.5) * gSM);
+  guNL += gNn * (-(SR * hL)) - gdn * hL + (big ? z : T(0.5) * gSM);
+  // SL, SR (both dry: constants)
+  const T gsl = both ? z : gSL, gsr = both ? z : gSR;
+  const bool l1 = !dryL && (xl <= yl), l2 = !dryL && !(xl <= yl);
+  const bool r1 = !dryR && (xr >= yr), r2 = !dryR && !(xr >= yr);
+  guNR += dryL ? gsl : z;
+  guNL += l1 ? gsl : z;
+  guNL += dryR ? gsr : z;
+  guNR += r1 ? gsr : z;
+  T gaL = (dryR ? T(2) * gsr : z) - (l1 ? gsl : z);
+  T gaR = (r1 ? gsr : z) - (dryL ? T(2) * gsl : z);
+  T gust = (l2 ? gsl : z) + (r2 ? gsr : z);
+  const T gast = (r2 ? gsr : z) - (l2 ? gsl : z);
   const T ghrt = (h_rt > z) ? gast : z;
   gaL += T(0.5) * ghrt;
   gaR += T(0.5) * ghrt;
@@ -186,11 +179,13 @@ GS_HD void face_vjp(T bL, T bR, T gzL, T gzR, T hL, T hR, T uNL, T uTL, T uNR, T
   gzf0 += gBR;
   const T gdzR = -gBR;
   gdb -= gBR;
-  if (hL < h_min) {
-    if (headR > z) gheadR += gdzR;
-  } else if (mdr > z) {
-    if (zlr <= headR) { gzLv += gdzR; gzRv -= gdzR; }
-    else gheadR += gdzR;
+  {
+    const bool pr = (hL < h_min) ? (headR > z) : false;
+    const bool pz = !(hL < h_min) && (mdr > z) && (zlr <= headR);
+    const bool ph = !(hL < h_min) && (mdr > z) && !(zlr <= headR);
+    gheadR += (pr || ph) ? gdzR : z;
+    gzLv += pz ? gdzR : z;
+    gzRv -= pz ? gdzR : z;
   }
   gzf0 += gheadR;
   getaR -= gheadR;
@@ -200,26 +195,33 @@ GS_HD void face_vjp(T bL, T bR, T gzL, T gzR, T hL, T hR, T uNL, T uTL, T uNR, T
   ghL += (T(0.5) * g) * gAL;
   gzf0 += gBL;
   const T gdzL = -gBL;
-  if (hR < h_min) {
-    if (headL > z) gheadL += gdzL;
-  } else if (mdl > z) {
-    if (dzc <= headL) gdzc += gdzL;
-    else gheadL += gdzL;
+  {
+    const bool ph = (hR < h_min) ? (headL > z) : ((mdl > z) && !(dzc <= headL));
+    const bool pc = !(hR < h_min) && (mdl > z) && (dzc <= headL);
+    gheadL += ph ? gdzL : z;
+    gdzc += pc ? gdzL : z;
   }
   gzf0 += gheadL;
   ghL -= gheadL;
   // flux
   hllc_vjp<T>(hLf, uNL, uTL, hRf, uNR, uTR, g, h_min, bFh, bFN, bFT, ghLf, guNL, guTL, ghRf, guNR, guTR);
   // face depths
-  if (aRf > z) { getaR += ghRf; gdetaR += ghRf; gzf0 -= ghRf; }
-  if (aLf > z) { ghL += ghLf; gdetaL += ghLf; gzf0 -= ghLf; }
-  if (mR > z) {
-    if (-dz <= hL - etaR) gdz -= gdetaR;
-    else { ghL += gdetaR; getaR -= gdetaR; }
-  }
-  if (mL > z) {
-    if (dz <= etaR - hL) gdz += gdetaL;
-    else { getaR += gdetaL; ghL -= gdetaL; }
+  const T gR_ = (aRf > z) ? ghRf : z, gL_ = (aLf > z) ? ghLf : z;
+  getaR += gR_;
+  gdetaR += gR_;
+  gzf0 -= gR_;
+  ghL += gL_;
+  gdetaL += gL_;
+  gzf0 -= gL_;
+  {
+    const bool a1 = (mR > z) && (-dz <= hL - etaR), a2 = (mR > z) && !(-dz <= hL - etaR);
+    gdz -= a1 ? gdetaR : z;
+    ghL += a2 ? gdetaR : z;
+    getaR -= a2 ? gdetaR : z;
+    const bool b1 = (mL > z) && (dz <= etaR - hL), b2 = (mL > z) && !(dz <= etaR - hL);
+    gdz += b1 ? gdetaL : z;
+    getaR += b2 ? gdetaL : z;
+    ghL -= b2 ? gdetaL : z;
   }
   ghR += getaR;
   gdb += getaR;
