@@ -58,7 +58,7 @@ def timed_steps(s, steps):
     return wall, float(ms.value)
 
 
-def cpu_baseline(bed_spec, crop=(1024, 1000), steps=30):
+def cpu_baseline(bed_spec, crop=(1024, 1000), steps=80):
     """The NumPy oracle (float32, one thread) on a crop of the same workload."""
     from oracle import generators as GN
     from oracle import swe_oracle as O
@@ -153,6 +153,10 @@ def coastal(args):
     nact = fm.n_active
     bpc = 33.0      # q 12 + bed 4 + flags 1 + rain frame 4 + q' 12 per active cell
     achieved = bpc * nact / (ms_f * 1e-3) / 1e9
+    traffic = None
+    tp = ROOT / "profiles" / "flat_step_coastal_16384_traffic.json"
+    if n == 16384 and tp.exists():
+        traffic = json.loads(tp.read_text()).get("bytes_per_launch")
     log(f"flat {ms_f:.3f} ms/step ({nact / (ms_f * 1e-3) / 1e9:.1f} G active cell-updates/s), dense "
         f"{ms_d:.3f} ms/step over the whole box; {achieved:.0f} GB/s algorithmic (flat)")
     out = {
@@ -172,7 +176,7 @@ def coastal(args):
                                f"gridded rain every 300 s, tidal stage on the sea edge, fall edges, CFL 0.5 (L-inf)",
                    "layout": "flat", "active_cells": int(nact), "stored_cells": int(fm.n_stored)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gs::k_flat_step<float, 1>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_flat_step<float, 1>",
                      "kernel_ms": ms_f, "bytes_per_cell": bpc},
         "dense_ms_per_step": ms_d,
         "dense_equals_flat": ok,
@@ -542,9 +546,17 @@ def main():
     value = cells * args.steps / wall
     achieved = BYTES_PER_CELL[args.layout] * cells / (kern_ms * 1e-3) / 1e9
     traffic = None
+    valu = None
     tp = Path(args.traffic)
     if tp.exists() and args.layout == "dense" and (nx, ny) == (25600, 25000):
         traffic = json.loads(tp.read_text()).get("bytes_per_launch")
+        vp = ROOT / "profiles" / "dense_sweep_rain_on_slope_640M_valu.json"
+        if vp.exists():
+            v = json.loads(vp.read_text())
+            # the step is instruction-bound: VALU wave-instructions per launch (SQ_INSTS_VALU)
+            # over this run's launch time
+            valu = {"insts_per_cell": v["valu_insts_per_cell"], "wave_insts_per_launch": v["valu_wave_insts_per_launch"],
+                    "wave_insts_per_s": v["valu_wave_insts_per_launch"] / (kern_ms * 1e-3), "source": v["source"]}
     log(f"{ms_step:.3f} ms/step wall, {kern_ms:.3f} ms/launch (events), {value / 1e9:.2f} G cell-updates/s, "
         f"{achieved:.0f} GB/s algorithmic")
     h32 = None
@@ -588,7 +600,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "gs::k_dense_sweep<float, 1>" if args.layout == "dense" else "gs::k_flat_step<float, 1>",
-                     "kernel_ms": kern_ms, "bytes_per_cell": BYTES_PER_CELL[args.layout]},
+                     "kernel_ms": kern_ms, "bytes_per_cell": BYTES_PER_CELL[args.layout], "valu": valu},
         "max_depth_err_vs_fp64_m": err64,
         "cpu_baseline": cpu,
     }
