@@ -21,6 +21,7 @@ GS_F32, GS_F64 = 0, 1
 FLAG_CLASS_MASK, FLAG_CFL_SKIP, FLAG_INACTIVE, FLAG_STAGE = 0x1F, 0x20, 0x40, 0x80
 MAX_CLASSES = 32
 ST_DONE, ST_NONFINITE = 1, 2
+ADJ_MAX_BLOCKS = 8192
 
 
 class GsClock(C.Structure):
@@ -64,12 +65,12 @@ class GsFlat(C.Structure):
 
 
 class GsAdjFlat(C.Structure):
-    _fields_ = [("n_stored", C.c_int64), ("dtype", C.c_int32), ("pad", C.c_int32), ("dx", C.c_double),
+    _fields_ = [("n_stored", C.c_int64), ("dtype", C.c_int32), ("n_classes", C.c_int32), ("dx", C.c_double),
                 ("dy", C.c_double), ("active", C.c_void_p), ("nbr", C.c_void_p), ("bed", C.c_void_p),
                 ("flags", C.c_void_p), ("ntab", C.c_void_p), ("q", C.c_void_p * 3), ("h_next", C.c_void_p),
                 ("pre", C.c_void_p * 3), ("lam", C.c_void_p * 3), ("w", C.c_void_p * 3), ("xf", C.c_void_p),
                 ("yf", C.c_void_p), ("gz", C.c_void_p), ("gz_prev", C.c_void_p), ("bbar", C.c_void_p),
-                ("nbar", C.c_void_p)]
+                ("nbar", C.c_void_p), ("npart", C.c_void_p)]
 
 
 # symbol -> (restype, argtypes); every entry point returns int
@@ -112,6 +113,7 @@ _SIGS = {
     "gs_adj_flat_sweep": (C.c_int, [C.POINTER(GsAdjFlat), C.POINTER(GsPhys), C.c_int, _P, _P, _P, _P, _P, C.c_int,
                                     _P, C.c_int64, _P]),
     "gs_adj_flat_bedgrad": (C.c_int, [C.POINTER(GsAdjFlat), C.POINTER(GsPhys), _P]),
+    "gs_adj_flat_nsum": (C.c_int, [C.POINTER(GsAdjFlat), _P]),
     "gs_ens_create": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), C.POINTER(GsForcing), C.POINTER(_P), C.c_int,
                                 C.POINTER(_P)]),
     "gs_ens_destroy": (C.c_int, [_P]),

@@ -171,7 +171,7 @@ class GaugeMisfit:
     # ------------------------------------------------------------------ reverse
     def gradient(self):
         """(dJ/dn per class (n_classes,), dJ/dbed (N,)) at the parameters of the last
-        simulate() (float64 host arrays)."""
+        simulate() (float64 host arrays). Leaves the solver at the initial state and time."""
         if self.samples is None:
             raise RuntimeError("gradient: call simulate() first")
         s, N, K, S = self.s, self.N, self.K, self.S
@@ -187,18 +187,20 @@ class GaugeMisfit:
         gz = L.DeviceArray((2, 2, N), dt_, zero=True)     # double-buffered bed-gradient cotangents
         bbar = L.DeviceArray((N,), np.float64, zero=True)
         nbar = L.DeviceArray((L.MAX_CLASSES,), np.float64, zero=True)
+        nblk = min(max(1, (N + 255) // 256), L.ADJ_MAX_BLOCKS)
+        npart = L.DeviceArray((nblk, 32), np.float64, zero=True)
         coef = (2.0 * (self.samples - self.h_star)).astype(dt_)          # (S, G): dJ/dh_s at gauges
         d_coef = L.DeviceArray.from_host(coef)
         es = d_coef.dtype.itemsize
         dcode = L.GS_F32 if dt_ == np.float32 else L.GS_F64
         a = L.GsAdjFlat()
-        a.n_stored, a.dtype = N, dcode
+        a.n_stored, a.dtype, a.n_classes = N, dcode, self.n_classes
         a.dx, a.dy = float(s.mesh.dx), float(s.mesh.dy)
         a.active, a.nbr, a.bed = self._active.ptr, self._nbr.ptr, s._bed.ptr
         a.flags, a.ntab = s._flags.ptr, s._ntab.ptr
         for k in range(3):
             a.lam[k], a.w[k] = lam[k].ptr, w[k].ptr
-        a.xf, a.yf, a.gz, a.bbar, a.nbar = xf.ptr, yf.ptr, gz.ptr, bbar.ptr, nbar.ptr
+        a.xf, a.yf, a.gz, a.bbar, a.nbar, a.npart = xf.ptr, yf.ptr, gz.ptr, bbar.ptr, nbar.ptr, npart.ptr
         phys = s._phys()
         p = C.byref(phys)
         desc = s._desc()
@@ -231,9 +233,12 @@ class GaugeMisfit:
                                               for j in range(n_in)])
             L.call("gs_adj_flat_sweep", C.byref(a), p, n_in, dptr, state_ptrs, pre_ptrs,
                    C.c_void_p(self._d_gauges.ptr), coef_ptrs, G, C.c_void_p(gz.ptr), C.c_int64(s0), None)
-        # the bed-gradient term of the last reverse step (step 0 wrote half 0)
+        # the bed-gradient term of the last reverse step (step 0 wrote half 0), the class sums
         a.gz, a.gz_prev = gz.ptr, None
         L.call("gs_adj_flat_bedgrad", C.byref(a), p, None)
+        L.call("gs_adj_flat_nsum", C.byref(a), None)
+        # leave the solver at the initial state and time (the replay moved its clock)
+        self._reset(self._q0, self.t0)
         L.sync(None)
         return nbar.download()[:self.n_classes].copy(), bbar.download().copy()
 

@@ -23,6 +23,7 @@
 namespace gs {
 
 constexpr int kABlock = 256;
+constexpr int kAdjMaxBlocks = GS_ADJ_MAX_BLOCKS;
 
 template <typename T>
 struct AdjK {
@@ -44,38 +45,72 @@ struct AdjK {
   const T* gz_prev;       // [2][n] of the previous reverse step, or NULL
   double* bbar;
   double* nbar;
+  double* npart;          // [gridDim.x][32] per-workgroup class sums (deterministic order)
   Phys<T> P;
   T dt;
 };
 
+// Butterfly sum over the 64 lanes (fixed order, so every lane gets the same bits every run).
+__device__ __forceinline__ double wave_sum_f64(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kABlock) void k_adj_update(AdjK<T> a) {
-  __shared__ double s_n[32];
-  if (threadIdx.x < 32) s_n[threadIdx.x] = 0.0;
+  // per-class n cotangents: butterfly sums per wavefront, wavefronts combined in a fixed order
+  // and added to this workgroup's own slot of npart (the grid is the same every reverse step),
+  // so the gradient has the same bits on every run
+  __shared__ double s_acc[kABlock / 64][32];
+  const int wv = threadIdx.x >> 6;
+  if (threadIdx.x < 32)
+    for (int w = 0; w < kABlock / 64; ++w) s_acc[w][threadIdx.x] = 0.0;
   __syncthreads();
-  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * blockDim.x) {
-    if (!a.active[c]) continue;
-    const uint8_t fl = a.flags ? a.flags[c] : (uint8_t)0;
-    const T lh = a.lam[0][c], lhu = a.lam[1][c], lhv = a.lam[2][c];
-    if (fl & GS_FLAG_STAGE) {
-      // h = max(eta - b, 0), hu = hv = 0: the step's own inputs get nothing
-      if (a.h_next[c] > T(0)) a.bbar[c] -= (double)lh;
-      a.w[0][c] = T(0);
-      a.w[1][c] = T(0);
-      a.w[2][c] = T(0);
-      continue;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < a.n; base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = base + threadIdx.x;
+    int cls = -1;
+    T gn = T(0);
+    if (c < a.n && a.active[c]) {
+      const uint8_t fl = a.flags ? a.flags[c] : (uint8_t)0;
+      const T lh = a.lam[0][c], lhu = a.lam[1][c], lhv = a.lam[2][c];
+      if (fl & GS_FLAG_STAGE) {
+        // h = max(eta - b, 0), hu = hv = 0: the step's own inputs get nothing
+        if (a.h_next[c] > T(0)) a.bbar[c] -= (double)lh;
+        a.w[0][c] = T(0);
+        a.w[1][c] = T(0);
+        a.w[2][c] = T(0);
+      } else {
+        cls = fl & GS_FLAG_CLASS_MASK;
+        const T n = a.P.friction ? a.ntab[cls] : T(0);
+        T g1, gu1, gv1;
+        update_vjp<T>(a.pre[0][c], a.pre[1][c], a.pre[2][c], a.dt, n, a.P, lh, lhu, lhv, g1, gu1, gv1, gn);
+        a.w[0][c] = g1;
+        a.w[1][c] = gu1;
+        a.w[2][c] = gv1;
+      }
     }
-    const int cls = fl & GS_FLAG_CLASS_MASK;
-    const T n = a.P.friction ? a.ntab[cls] : T(0);
-    T g1, gu1, gv1, gn;
-    update_vjp<T>(a.pre[0][c], a.pre[1][c], a.pre[2][c], a.dt, n, a.P, lh, lhu, lhv, g1, gu1, gv1, gn);
-    a.w[0][c] = g1;
-    a.w[1][c] = gu1;
-    a.w[2][c] = gv1;
-    if (gn != T(0)) atomicAdd(&s_n[cls], (double)gn);
+    if (a.P.friction) {
+      for (int k = 0; k < a.n_classes; ++k) {   // uniform trip count: every lane shuffles
+        const double v = wave_sum_f64(cls == k ? (double)gn : 0.0);
+        if ((threadIdx.x & 63) == 0) s_acc[wv][k] += v;
+      }
+    }
   }
   __syncthreads();
-  if (threadIdx.x < (unsigned)a.n_classes && s_n[threadIdx.x] != 0.0) atomicAdd(&a.nbar[threadIdx.x], s_n[threadIdx.x]);
+  if (threadIdx.x < (unsigned)a.n_classes) {
+    double v = 0.0;
+    for (int w = 0; w < kABlock / 64; ++w) v += s_acc[w][threadIdx.x];
+    a.npart[(int64_t)blockIdx.x * 32 + threadIdx.x] += v;
+  }
+}
+
+// nbar[k] += sum over workgroup slots of npart[.][k], in slot order.
+__global__ void k_adj_nsum(const double* npart, int nblocks, int n_classes, double* nbar) {
+  const int k = threadIdx.x;
+  if (k >= n_classes) return;
+  double v = 0.0;
+  for (int b = 0; b < nblocks; ++b) v += npart[(int64_t)b * 32 + k];
+  nbar[k] += v;
 }
 
 // Reverse of one face (L, R) along x (axis 0) or y (axis 1) into slot L of `slots`.
@@ -205,6 +240,13 @@ __global__ void k_adj_add(T* lam_h, const int64_t* ids, const T* coef, int n) {
   if (t < n) lam_h[ids[t]] += coef[t];
 }
 
+// The grid of every adjoint pass; fixed for a given n (npart has one slot row per workgroup).
+static inline int64_t adj_blocks(int64_t n) {
+  int64_t blocks = (n + kABlock - 1) / kABlock;
+  if (blocks > kAdjMaxBlocks) blocks = kAdjMaxBlocks;
+  return blocks < 1 ? 1 : blocks;
+}
+
 template <typename T>
 static int adj_step_t(const GsAdjFlat* f, const GsPhys* p, double dt, hipStream_t s) {
   AdjK<T> a;
@@ -229,11 +271,10 @@ static int adj_step_t(const GsAdjFlat* f, const GsPhys* p, double dt, hipStream_
   a.gz_prev = (const T*)f->gz_prev;
   a.bbar = f->bbar;
   a.nbar = f->nbar;
+  a.npart = f->npart;
   a.P = make_phys<T>(*p, f->dx, f->dy);
   a.dt = (T)dt;
-  int64_t blocks = (a.n + kABlock - 1) / kABlock;
-  if (blocks > 8192) blocks = 8192;
-  if (blocks < 1) blocks = 1;
+  const int64_t blocks = adj_blocks(a.n);
   const dim3 gr((unsigned)blocks), bl(kABlock);
   hipLaunchKernelGGL(k_adj_update<T>, gr, bl, 0, s, a);
   hipLaunchKernelGGL(k_adj_faces<T>, gr, bl, 0, s, a);
@@ -254,9 +295,7 @@ static int adj_bedgrad_t(const GsAdjFlat* f, const GsPhys* p, hipStream_t s) {
   a.gz = (T*)f->gz;
   a.bbar = f->bbar;
   a.P = make_phys<T>(*p, f->dx, f->dy);
-  int64_t blocks = (a.n + kABlock - 1) / kABlock;
-  if (blocks > 8192) blocks = 8192;
-  if (blocks < 1) blocks = 1;
+  const int64_t blocks = adj_blocks(a.n);
   hipLaunchKernelGGL(k_adj_bedgrad<T>, dim3((unsigned)blocks), dim3(kABlock), 0, s, a);
   GS_HIP(hipGetLastError(), "k_adj_bedgrad launch");
   return GS_OK;
@@ -270,7 +309,8 @@ int gs_adj_flat_step(const GsAdjFlat* f, const GsPhys* phys, double dt, void* st
   using namespace gs;
   if (!f || !phys) return fail(GS_E_ARG, "NULL argument");
   if (f->n_stored < 1) return fail(GS_E_RANGE, "n_stored < 1");
-  if (!f->active || !f->nbr || !f->bed || !f->h_next || !f->xf || !f->yf || !f->gz || !f->bbar || !f->nbar)
+  if (!f->active || !f->nbr || !f->bed || !f->h_next || !f->xf || !f->yf || !f->gz || !f->bbar || !f->nbar ||
+      !f->npart)
     return fail(GS_E_ARG, "adjoint arrays are NULL");
   for (int k = 0; k < 3; ++k)
     if (!f->q[k] || !f->pre[k] || !f->lam[k] || !f->w[k]) return fail(GS_E_ARG, "state/cotangent arrays are NULL");
@@ -281,6 +321,17 @@ int gs_adj_flat_step(const GsAdjFlat* f, const GsPhys* phys, double dt, void* st
   if (f->dtype == GS_F64) return adj_step_t<double>(f, phys, dt, (hipStream_t)stream);
   if (f->dtype == GS_F32) return adj_step_t<float>(f, phys, dt, (hipStream_t)stream);
   return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
+}
+
+int gs_adj_flat_nsum(const GsAdjFlat* f, void* stream) {
+  using namespace gs;
+  if (!f || !f->npart || !f->nbar) return fail(GS_E_ARG, "NULL argument");
+  if (f->n_stored < 1) return fail(GS_E_RANGE, "n_stored < 1");
+  if (f->n_classes < 1 || f->n_classes > 32) return fail(GS_E_RANGE, "n_classes must be in 1..32");
+  hipLaunchKernelGGL(k_adj_nsum, dim3(1), dim3(32), 0, (hipStream_t)stream, f->npart, (int)adj_blocks(f->n_stored),
+                     f->n_classes, f->nbar);
+  GS_HIP(hipGetLastError(), "k_adj_nsum launch");
+  return GS_OK;
 }
 
 int gs_adj_flat_bedgrad(const GsAdjFlat* f, const GsPhys* phys, void* stream) {

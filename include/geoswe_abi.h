@@ -257,10 +257,11 @@ int gs_flat_sample(const GsFlat* f, const GsClock* clock_dev, const int64_t* ids
  * (bbar) accumulate. Stored cells carry a full 4-neighbour table; w, xf, yf and gz are
  * scratch that must be zero-initialised once (w, xf, yf) before the first reverse step.
  */
+#define GS_ADJ_MAX_BLOCKS 8192   /* workgroups of every adjoint pass (npart has this many rows at most) */
 typedef struct GsAdjFlat {
   int64_t n_stored;
   int32_t dtype;
-  int32_t pad;
+  int32_t n_classes;         /* Manning classes (1..32) */
   double dx, dy;
   const uint8_t* active;     /* [N] 1 for cells the step updates */
   const int32_t* nbr;        /* [N][4] flat ids of (i+1, j), (i-1, j), (i, j+1), (i, j-1); -1: not stored */
@@ -278,9 +279,13 @@ typedef struct GsAdjFlat {
   const void* gz_prev;       /* NULL, or the gz of the previous reverse step: its bed-gradient
                                 term is then folded into this step's gather pass */
   double* bbar;              /* [N] accumulated d(loss)/d(bed) */
-  double* nbar;              /* [n_classes] accumulated d(loss)/d(n) */
+  double* nbar;              /* [n_classes] d(loss)/d(n), from gs_adj_flat_nsum */
+  double* npart;             /* [min(ceil(N/256), GS_ADJ_MAX_BLOCKS)][32] per-workgroup class sums,
+                                zero-initialised once; summed into nbar in a fixed order */
 } GsAdjFlat;
 int gs_adj_flat_step(const GsAdjFlat* a, const GsPhys* phys, double dt, void* stream);
+/* nbar[k] += the per-workgroup class sums npart[.][k] in workgroup order (deterministic). */
+int gs_adj_flat_nsum(const GsAdjFlat* a, void* stream);
 /* The bed-gradient term of a->gz alone (the flush after the last step run with gz_prev). */
 int gs_adj_flat_bedgrad(const GsAdjFlat* a, const GsPhys* phys, void* stream);
 /* lam_h[ids[k]] += coef[k]: the loss's direct cotangent of a state (gauge misfit). */

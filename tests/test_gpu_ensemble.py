@@ -81,3 +81,20 @@ def test_ensemble_run_to_t_end(cases):
     assert all(abs(m.t - 12.5) < 1e-9 for m in mem)
     assert_same(plain, mem)
     ens.close()
+
+
+def test_ensemble_float64_equals_plain_runs(gpu):
+    from geoswe_amd.cases import ensemble_c4
+    from geoswe_amd.ensemble import Ensemble
+    cases = ensemble_c4(sides=[64, 128, 96], rain_frames=2)
+    for c in cases:
+        c.cfg.dtype = "float64"
+    plain = flats(cases)
+    for p in plain:
+        p.advance(25)
+    mem = flats(cases)
+    ens = Ensemble(mem, graph_batch=8)
+    ens.advance(25)
+    assert all(m.dtype == np.float64 for m in mem)
+    assert_same(plain, mem)
+    ens.close()
