@@ -34,14 +34,16 @@ def flat_neighbours(fm) -> np.ndarray:
     """(N, 4) int32 flat ids of (i+1, j), (i-1, j), (i, j+1), (i, j-1) of every stored
     cell, -1 where that cell is not stored. Checks that every active cell has its four
     neighbours and their neighbours along the same axis (the ring-2 stored set)."""
-    idx = np.full((fm.nxp, fm.nyp), -1, np.int64)
+    if fm.n_stored >= 2 ** 31:
+        raise ValueError("flat_neighbours: more than 2^31 stored cells")
+    idx = np.full((fm.nxp, fm.nyp), -1, np.int32)
     i, j = fm.ij[:, 0].astype(np.int64), fm.ij[:, 1].astype(np.int64)
-    idx[i, j] = np.arange(fm.n_stored)
+    idx[i, j] = np.arange(fm.n_stored, dtype=np.int32)
 
     def at(di, dj):
         ii, jj = i + di, j + dj
         ok = (ii >= 0) & (ii < fm.nxp) & (jj >= 0) & (jj < fm.nyp)
-        out = np.full(fm.n_stored, -1, np.int64)
+        out = np.full(fm.n_stored, -1, np.int32)
         out[ok] = idx[ii[ok], jj[ok]]
         return out
 
@@ -178,17 +180,23 @@ class GaugeMisfit:
         dt_ = self.dtype
         G = len(self.gauges)
         nseg = (S + K - 1) // K
-        seg_states = [[L.DeviceArray((N,), dt_) for _ in range(3)] for _ in range(K + 1)]
-        seg_pre = [[L.DeviceArray((N,), dt_) for _ in range(3)] for _ in range(K)]
-        lam = [L.DeviceArray((N,), dt_, zero=True) for _ in range(3)]
-        w = [L.DeviceArray((N,), dt_, zero=True) for _ in range(3)]
-        xf = L.DeviceArray((10, N), dt_, zero=True)
-        yf = L.DeviceArray((10, N), dt_, zero=True)
-        gz = L.DeviceArray((2, 2, N), dt_, zero=True)     # double-buffered bed-gradient cotangents
-        bbar = L.DeviceArray((N,), np.float64, zero=True)
-        nbar = L.DeviceArray((L.MAX_CLASSES,), np.float64, zero=True)
-        nblk = min(max(1, (N + 255) // 256), L.ADJ_MAX_BLOCKS)
-        npart = L.DeviceArray((nblk, 32), np.float64, zero=True)
+        if not hasattr(self, "_ws"):     # device workspace, allocated once per problem
+            nblk = min(max(1, (N + 255) // 256), L.ADJ_MAX_BLOCKS)
+            self._ws = dict(
+                seg_states=[[L.DeviceArray((N,), dt_) for _ in range(3)] for _ in range(K + 1)],
+                seg_pre=[[L.DeviceArray((N,), dt_) for _ in range(3)] for _ in range(K)],
+                lam=[L.DeviceArray((N,), dt_) for _ in range(3)],
+                w=[L.DeviceArray((N,), dt_) for _ in range(3)],
+                xf=L.DeviceArray((10, N), dt_), yf=L.DeviceArray((10, N), dt_),
+                gz=L.DeviceArray((2, 2, N), dt_),          # double-buffered bed-gradient cotangents
+                bbar=L.DeviceArray((N,), np.float64), nbar=L.DeviceArray((L.MAX_CLASSES,), np.float64),
+                npart=L.DeviceArray((nblk, 32), np.float64))
+        ws = self._ws
+        seg_states, seg_pre = ws["seg_states"], ws["seg_pre"]
+        lam, w, xf, yf, gz = ws["lam"], ws["w"], ws["xf"], ws["yf"], ws["gz"]
+        bbar, nbar, npart = ws["bbar"], ws["nbar"], ws["npart"]
+        for arr in lam + w + [xf, yf, gz, bbar, nbar, npart]:   # accumulators and scratch start at zero
+            L.call("gs_memset", C.c_void_p(arr.ptr), 0, C.c_size_t(arr.nbytes), None)
         coef = (2.0 * (self.samples - self.h_star)).astype(dt_)          # (S, G): dJ/dh_s at gauges
         d_coef = L.DeviceArray.from_host(coef)
         es = d_coef.dtype.itemsize

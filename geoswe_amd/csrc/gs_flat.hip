@@ -14,6 +14,7 @@
 // upper y-face with the lane above; the lower y-face comes from the lane below by a
 // cross-lane shuffle. The face arithmetic is gs_device.h's, shared with the dense sweep,
 // so the residual is bitwise the dense one.
+#include <atomic>
 #include <vector>
 
 #include "gs_common.h"
@@ -395,16 +396,16 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
 // device and instantiation); the grid never exceeds the work.
 template <typename T, int MODE>
 static unsigned flat_grid(int64_t nchunks) {
-  static int cache[64];
+  static std::atomic<int> cache[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = -1;
-  int per = (dev >= 0 && dev < 64) ? cache[dev] : 0;
+  int per = (dev >= 0 && dev < 64) ? cache[dev].load(std::memory_order_relaxed) : 0;
   if (per <= 0) {
     int occ = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_flat_step<T, MODE>, kFBlock, 0) != hipSuccess) occ = 4;
     if (dev < 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
     per = (occ > 0 ? occ : 1) * (cus > 0 ? cus : 1);
-    if (dev >= 0 && dev < 64) cache[dev] = per;
+    if (dev >= 0 && dev < 64) cache[dev].store(per, std::memory_order_relaxed);
   }
   const int64_t need = (nchunks + 3) / 4;
   return (unsigned)(need < per ? (need > 0 ? need : 1) : per);
