@@ -54,3 +54,25 @@ def test_struct_layout_matches_header(tmp_path):
         assert int(want[sname]) == C.sizeof(cls), sname
         for fname, _ in cls._fields_:
             assert int(want[f"{sname}.{fname}"]) == getattr(cls, fname).offset, (sname, fname)
+
+
+def test_new_entry_points_reject_bad_arguments():
+    """Argument checks of the ensemble and adjoint entry points run before any device call."""
+    lib = L.lib()
+    out = C.c_void_p()
+    assert lib.gs_ens_create(None, None, None, None, 1, C.byref(out)) == 1
+    assert b"NULL" in lib.gs_last_error()
+    assert lib.gs_ens_create(None, None, None, None, 0, C.byref(out)) == 3     # GS_E_RANGE
+    assert lib.gs_ens_step(None, None) == 1
+    assert lib.gs_ens_set_launch(None, 0, 1) == 1
+    assert lib.gs_ens_destroy(None) == 0
+    assert lib.gs_adj_flat_step(None, None, C.c_double(1.0), None) == 1
+    a = L.GsAdjFlat()
+    a.n_stored = 10
+    assert lib.gs_adj_flat_step(C.byref(a), C.byref(L.GsPhys()), C.c_double(1.0), None) == 1
+    assert b"NULL" in lib.gs_last_error()
+    assert lib.gs_adj_add(None, 0, None, None, 0, None) == 0                  # nothing to add
+    assert lib.gs_adj_add(None, 0, None, None, -1, None) == 1
+    assert lib.gs_flat_replay_pre(None, None, L.GsForcing(), None, 1, None, None, None, None) == 1
+    assert lib.gs_flat_run_frozen(None, None, L.GsForcing(), None, 1, None, None, 0, None, 1, None, None) == 1
+    assert lib.gs_adj_flat_nsum(None, None) == 1
