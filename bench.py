@@ -154,7 +154,7 @@ def coastal(args):
     bpc = 33.0      # q 12 + bed 4 + flags 1 + rain frame 4 + q' 12 per active cell
     achieved = bpc * nact / (ms_f * 1e-3) / 1e9
     traffic = None
-    tp = ROOT / "profiles" / "flat_step_coastal_16384_traffic.json"
+    tp = ROOT / "profiles" / "flat_sweep_coastal_16384_traffic.json"
     if n == 16384 and tp.exists():
         traffic = json.loads(tp.read_text()).get("bytes_per_launch")
     log(f"flat {ms_f:.3f} ms/step ({nact / (ms_f * 1e-3) / 1e9:.1f} G active cell-updates/s), dense "
@@ -176,7 +176,7 @@ def coastal(args):
                                f"gridded rain every 300 s, tidal stage on the sea edge, fall edges, CFL 0.5 (L-inf)",
                    "layout": "flat", "active_cells": int(nact), "stored_cells": int(fm.n_stored)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_flat_step<float, 1>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_flat_sweep<float>",
                      "kernel_ms": ms_f, "bytes_per_cell": bpc},
         "dense_ms_per_step": ms_d,
         "dense_equals_flat": ok,
@@ -254,6 +254,10 @@ def ensemble(args):
     log(f"sequential runs with a per-step dt read-back: {w_sync:.2f} s wall, {v_sync / 1e9:.2f} G cell-updates/s "
         f"(ensemble {v_ens / v_sync:.2f}x)")
     achieved = 33.0 * act * steps / (ev_ens * 1e-3) / 1e9
+    traffic = None
+    tp = ROOT / "profiles" / "ens_step_64_members_traffic.json"
+    if args.members == 64 and tp.exists():
+        traffic = json.loads(tp.read_text()).get("bytes_per_launch")
     out = {
         "metric": "cell-updates/s (batched ensemble of flat domains, 1 MI355X)",
         "value": v_ens, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": 1,
@@ -267,7 +271,7 @@ def ensemble(args):
         "sequential_plain_s": w_plain, "ensemble_s": w_ens, "event_ms_plain": ev_plain, "event_ms_ensemble": ev_ens,
         "launch": tune, "members_bitwise_equal_plain": same, "distinct_member_dt": len(dts),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "gs::k_ens_step<float>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_ens_step<float>",
                      "bytes_per_cell": 33.0},
     }
     print(json.dumps(out), flush=True)
@@ -599,7 +603,7 @@ def main():
                    "layout": args.layout, "cells": cells},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "gs::k_dense_sweep<float, 1>" if args.layout == "dense" else "gs::k_flat_step<float, 1>",
+                     "kernel": "gs::k_dense_sweep<float, 1>" if args.layout == "dense" else "gs::k_flat_sweep<float>",
                      "kernel_ms": kern_ms, "bytes_per_cell": BYTES_PER_CELL[args.layout], "valu": valu},
         "max_depth_err_vs_fp64_m": err64,
         "cpu_baseline": cpu,

@@ -61,7 +61,8 @@ class GsFlat(C.Structure):
                 ("bed", C.c_void_p), ("flags", C.c_void_p), ("ntab", C.c_void_p), ("n_field", C.c_void_p),
                 ("max_h", C.c_void_p), ("run_k0", C.c_void_p), ("run_len", C.c_void_p),
                 ("run_i", C.c_void_p), ("run_j0", C.c_void_p), ("run_d", C.c_void_p),
-                ("chunk_run", C.c_void_p)]
+                ("chunk_run", C.c_void_p), ("n_tiles", C.c_int64), ("tile_e0", C.c_void_p),
+                ("tile_irc", C.c_void_p), ("ent", C.c_void_p)]
 
 
 class GsAdjFlat(C.Structure):

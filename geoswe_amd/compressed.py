@@ -45,6 +45,10 @@ class CompressedSolver:
         self._max_h = L.DeviceArray((N,), self.dtype, zero=True) if track_max_depth else None
         self._runs = [L.DeviceArray.from_host(a) for a in (fm.run_k0, fm.run_len, fm.run_i, fm.run_j0,
                                                             fm.run_d.reshape(-1), fm.chunk_run)]
+        self._tiles = [L.DeviceArray.from_host(a) for a in (fm.tile_e0, fm.tile_irc.reshape(-1), fm.ent.reshape(-1))]
+        # "tiles": the step sweeps 64-column windows down the rows, each face once (default);
+        # "chunks": it walks 60-id chunks of the run table (a second traversal, same bits)
+        self.traversal = "tiles"
         self._fo = _Forcing(cfg, min(mesh.dx, mesh.dy), self.dtype, (mesh.nx, mesh.ny))
         self.stream = None
         self._clock = L.new_clock(0.0)
@@ -55,14 +59,16 @@ class CompressedSolver:
 
     # ------------------------------------------------------------------ construction
     @classmethod
-    def from_dense(cls, s: Solver2D, cfl_linf: bool = False, track_max_depth: bool = True, **_ignored):
-        """Pack a dense solver (its current state, bed, Manning, mask, forcings)."""
+    def from_dense(cls, s: Solver2D, cfl_linf: bool = False, track_max_depth: bool = True, sweep_rows: int = None,
+                   **_ignored):
+        """Pack a dense solver (its current state, bed, Manning, mask, forcings). `sweep_rows`:
+        rows per row-sweep tile of the step (None: chosen from the domain's size)."""
         mesh, cfg = s.mesh, s.cfg
         g, nx, ny = mesh.ngh, mesh.nx, mesh.ny
         nxp, nyp = nx + 2 * g, ny + 2 * g
         inside = np.zeros((nxp, nyp), bool)
         inside[g:g + nx, g:g + ny] = s.inside_mask if s.inside_mask is not None else True
-        fm = build_flat_mesh(inside, g, ring=2)
+        fm = build_flat_mesh(inside, g, ring=2, sweep_rows=sweep_rows)
         q_pad = s.q                                  # ghosts filled from the boundary kinds
         b_pad = s.b
         flags = np.zeros((nxp, nyp), np.uint8)
@@ -115,6 +121,10 @@ class CompressedSolver:
         f.n_field = self._n_field.ptr if self._n_field is not None else None
         f.max_h = self._max_h.ptr if self._max_h is not None else None
         f.run_k0, f.run_len, f.run_i, f.run_j0, f.run_d, f.chunk_run = (r.ptr for r in self._runs)
+        if self.traversal not in ("tiles", "chunks"):
+            raise ValueError(f"traversal must be 'tiles' or 'chunks', got {self.traversal!r}")
+        f.n_tiles = len(fm.tile_e0) if self.traversal == "tiles" else 0
+        f.tile_e0, f.tile_irc, f.ent = (t.ptr for t in self._tiles)
         return f
 
     # ------------------------------------------------------------------ state

@@ -105,12 +105,14 @@ __global__ __launch_bounds__(kABlock) void k_adj_update(AdjK<T> a) {
 }
 
 // nbar[k] += sum over workgroup slots of npart[.][k], in slot order.
-__global__ void k_adj_nsum(const double* npart, int nblocks, int n_classes, double* nbar) {
-  const int k = threadIdx.x;
-  if (k >= n_classes) return;
+// One wavefront per class: lane l sums the slots of workgroups l, l+64, ... in order, then a
+// butterfly over the lanes; the order is fixed, so the sum has the same bits on every run.
+__global__ __launch_bounds__(64) void k_adj_nsum(const double* npart, int nblocks, int n_classes, double* nbar) {
+  const int k = blockIdx.x;
   double v = 0.0;
-  for (int b = 0; b < nblocks; ++b) v += npart[(int64_t)b * 32 + k];
-  nbar[k] += v;
+  for (int b = (int)threadIdx.x; b < nblocks; b += 64) v += npart[(int64_t)b * 32 + k];
+  v = wave_sum_f64(v);
+  if (threadIdx.x == 0 && k < n_classes) nbar[k] += v;
 }
 
 // Reverse of one face (L, R) along x (axis 0) or y (axis 1) into slot L of `slots`.
@@ -328,8 +330,8 @@ int gs_adj_flat_nsum(const GsAdjFlat* f, void* stream) {
   if (!f || !f->npart || !f->nbar) return fail(GS_E_ARG, "NULL argument");
   if (f->n_stored < 1) return fail(GS_E_RANGE, "n_stored < 1");
   if (f->n_classes < 1 || f->n_classes > 32) return fail(GS_E_RANGE, "n_classes must be in 1..32");
-  hipLaunchKernelGGL(k_adj_nsum, dim3(1), dim3(32), 0, (hipStream_t)stream, f->npart, (int)adj_blocks(f->n_stored),
-                     f->n_classes, f->nbar);
+  hipLaunchKernelGGL(k_adj_nsum, dim3((unsigned)f->n_classes), dim3(64), 0, (hipStream_t)stream, f->npart,
+                     (int)adj_blocks(f->n_stored), f->n_classes, f->nbar);
   GS_HIP(hipGetLastError(), "k_adj_nsum launch");
   return GS_OK;
 }

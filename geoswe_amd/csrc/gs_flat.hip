@@ -51,6 +51,10 @@ struct FlatK {
   GsClock* clock;
   Plan plan;
   T* pre[3];        // nullable: pre-friction h, hu, hv of active cells (adjoint recomputation)
+  int64_t n_tiles;  // row-sweep tiles (0: chunk traversal)
+  const int64_t* tile_e0;
+  const int32_t* tile_irc;
+  const uint64_t* ent;
 };
 
 template <typename T> __device__ __forceinline__ T fl_upper(T x) { return __shfl_down(x, 1, 64); }
@@ -87,6 +91,9 @@ struct FlatCtx {
   const GS_G int32_t* run_j0;
   const GS_G int32_t* run_d;
   const GS_G int32_t* chunk_run;
+  const GS_G int64_t* tile_e0;
+  const GS_G int32_t* tile_irc;
+  const GS_G uint64_t* ent;
   const GS_G T* frame;   // gridded rain frame of this step (rain_mode 2)
   const T* ntab;         // Manning n per class (LDS copy or the member's table)
   int64_t n, rain_pitch;
@@ -170,6 +177,9 @@ __device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k,
   x.run_j0 = gp(a.run_j0);
   x.run_d = gp(a.run_d);
   x.chunk_run = gp(a.chunk_run);
+  x.tile_e0 = gp(a.tile_e0);
+  x.tile_irc = gp(a.tile_irc);
+  x.ent = gp(a.ent);
   x.frame = k.frame;
   x.ntab = a.ntab;
   x.n = a.n;
@@ -296,6 +306,165 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
   }
 }
 
+// Lane `lane`'s cell in one window row of a sweep tile (entry e = first stored id, stored
+// mask, active mask): the stored cells of a row are consecutive flat ids, so the lane's id
+// is the first id plus the number of stored lanes below it.
+__device__ __forceinline__ int64_t sweep_id(const GS_G uint64_t* e, int lane, bool& st) {
+  const uint64_t base = e[0], sm = e[1];
+  const unsigned below =
+      __builtin_amdgcn_mbcnt_hi((unsigned)(sm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sm, 0u));
+  st = ((sm >> lane) & 1ull) != 0ull;
+  return (int64_t)base + (int64_t)below;
+}
+
+// State and bed of a lane's cell; lanes on unstored cells read nothing and hold zeros
+// (their faces are never used: every face an active cell needs has its stencil stored).
+template <typename T>
+__device__ __forceinline__ void sweep_q(const FlatCtx<T>& x, int64_t k, bool st, T& h, T& hu, T& hv) {
+  h = T(0);
+  hu = T(0);
+  hv = T(0);
+  if (st) {
+    h = x.qi0[k];
+    hu = x.qi1[k];
+    hv = x.qi2[k];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T sweep_b(const FlatCtx<T>& x, int64_t k, bool st) {
+  T b = T(0);
+  if (st) b = x.bed[k];
+  return b;
+}
+
+// One row-sweep tile: the fused update of the active lanes of rows i0 .. i0+nr-1 of one
+// 64-column window. As in the dense sweep, every face is evaluated once: the x-face below
+// a row is carried from the row above, the y-face below a lane comes from the lane below
+// by shuffle, and the residual terms are added in the same order as there (and as in
+// flat_chunk), so all three traversals give the same bits. Every lane of the wavefront
+// must call it.
+template <typename T>
+__device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int lane, uint64_t& lam) {
+  const Phys<T>& P = x.P;
+  const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
+  const GS_G int32_t* irc = x.tile_irc + 3 * t;
+  const int i0 = irc[0], nr = irc[1], c0 = irc[2];
+  const GS_G uint64_t* E = x.ent + 3 * x.tile_e0[t];   // entry of row i0 - 2
+  bool sm2, sm1, sC, sN;
+  const int64_t km2 = sweep_id(E, lane, sm2);
+  const int64_t km1 = sweep_id(E + 3, lane, sm1);
+  int64_t kC = sweep_id(E + 6, lane, sC);
+  int64_t kN = sweep_id(E + 9, lane, sN);
+  const T bm2 = sweep_b(x, km2, sm2);
+  T bm1 = sweep_b(x, km1, sm1), b0 = sweep_b(x, kC, sC), bp1 = sweep_b(x, kN, sN);
+  T hP, huP, hvP, uP, vP;
+  sweep_q(x, km1, sm1, hP, huP, hvP);
+  prims(hP, huP, hvP, hm, uP, vP);
+  const T gzxP = (T(0.5) * (b0 - bm2)) * ix;
+  T hC, huC, hvC, uC, vC;
+  sweep_q(x, kC, sC, hC, huC, hvC);
+  prims(hC, huC, hvC, hm, uC, vC);
+  T gzxC = (T(0.5) * (bp1 - bm1)) * ix;
+  Face<T> fx;
+  face<T>(bm1, b0, gzxP, gzxC, hP, hC, uP, vP, uC, vC, hdx, g, hm, fx);
+  // the x-face below the current row, as the products the current cell adds
+  T xFh = fx.Fh * ix, xFN = fx.FN * ix, xFT = fx.FT * ix, xsR = fx.srcR * ix;
+  bool xg = fx.gate;
+
+  for (int r = 0; r < nr; ++r) {
+    const GS_G uint64_t* Er = E + 3 * (r + 2);   // this row's entry
+    bool sNN;
+    const int64_t kNN = sweep_id(Er + 6, lane, sNN);
+    const T bp2 = sweep_b(x, kNN, sNN);
+    T hN, huN, hvN, uN, vN;
+    sweep_q(x, kN, sN, hN, huN, hvN);
+    prims(hN, huN, hvN, hm, uN, vN);
+    const T gzxN = (T(0.5) * (bp2 - b0)) * ix;
+    Face<T> fn;
+    face<T>(b0, bp1, gzxC, gzxN, hC, hN, uC, vC, uN, vN, hdx, g, hm, fn);
+    const T nFh = fn.Fh * ix, nFN = fn.FN * ix, nFT = fn.FT * ix;
+
+    // y faces of the row: each lane computes its upper face (j, j+1)
+    const T bl = fl_lower(b0), br = fl_upper(b0);
+    const T gzy = (T(0.5) * (br - bl)) * iy;
+    const T hR = fl_upper(hC), uR = fl_upper(uC), vR = fl_upper(vC);
+    const T gzyR = fl_upper(gzy);
+    Face<T> fy;
+    face<T>(b0, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
+    const T yFh = fy.Fh * iy, yFN = fy.FN * iy, yFT = fy.FT * iy;
+    const T yFhB = fl_lower(yFh), yFNB = fl_lower(yFN), yFTB = fl_lower(yFT);
+    const T ysRB = fl_lower(fy.srcR * iy);
+    const int ygB = fl_lower((int)fy.gate);
+
+    T rh = T(0), rhu = T(0), rhv = T(0);
+    if (fn.gate) {
+      rh = rh - nFh;
+      rhu = rhu - nFN;
+      rhv = rhv - nFT;
+      rhu = rhu - fn.srcL * ix;
+    }
+    if (xg) {
+      rh = rh + xFh;
+      rhu = rhu + xFN;
+      rhv = rhv + xFT;
+      rhu = rhu + xsR;
+    }
+    if (fy.gate) {
+      rh = rh - yFh;
+      rhu = rhu - yFT;
+      rhv = rhv - yFN;
+      rhv = rhv - fy.srcL * iy;
+    }
+    if (ygB) {
+      rh = rh + yFhB;
+      rhu = rhu + yFTB;
+      rhv = rhv + yFNB;
+      rhv = rhv + ysRB;
+    }
+
+    if ((Er[2] >> lane) & 1ull) {   // an active cell: lanes 2..61 only
+      const int64_t k = kC;
+      const uint8_t fg = x.flags ? x.flags[k] : (uint8_t)0;
+      T rr0 = rh;
+      if (x.rain_mode == 1) {
+        rr0 = rr0 + x.rain_u;
+      } else if (x.rain_mode == 2) {
+        rr0 = rr0 + x.frame[(int64_t)(i0 + r - x.ngh) * x.rain_pitch + (c0 + lane - x.ngh)];
+      }
+      if (x.pre0) {   // the first three statements of update(), kept for its reverse
+        x.pre0[k] = hC + x.dt * rr0;
+        x.pre1[k] = huC + x.dt * rhu;
+        x.pre2[k] = hvC + x.dt * rhv;
+      }
+      T n = T(0);
+      if (P.friction) n = x.n_field ? x.n_field[k] : x.ntab[fg & GS_FLAG_CLASS_MASK];
+      T h, hu, hv;
+      update<T>(hC, huC, hvC, rr0, rhu, rhv, x.dt, n, P, h, hu, hv);
+      if (fg & GS_FLAG_STAGE) {
+        const T hs = x.eta - b0;
+        h = hs > T(0) ? hs : T(0);
+        hu = T(0);
+        hv = T(0);
+      }
+      if (x.max_h && h > x.max_h[k]) x.max_h[k] = h;
+      x.qo0[k] = h;
+      x.qo1[k] = hu;
+      x.qo2[k] = hv;
+      if (!(fg & GS_FLAG_CFL_SKIP)) {
+        const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
+        lam = lb_ > lam ? lb_ : lam;
+      }
+    }
+
+    // slide the window one row down
+    hC = hN; huC = huN; hvC = hvN; uC = uN; vC = vN; gzxC = gzxN;
+    bm1 = b0; b0 = bp1; bp1 = bp2;
+    xFh = nFh; xFN = nFN; xFT = nFT; xsR = fn.srcR * ix; xg = fn.gate;
+    kC = kN; kN = kNN; sN = sNN;
+  }
+}
+
 // Workgroup b's place in an XCD-major order: workgroups are dealt round-robin to the 8
 // XCDs, so giving each XCD a contiguous range of wavefronts (hence of chunks in every
 // pass) keeps the rows i+-1 that a chunk gathers mostly in that XCD's L2.
@@ -322,6 +491,27 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
        chunk < a.nchunks; chunk += nwaves)
     flat_chunk<T, MODE>(x, chunk, lane, lam);
   if (MODE == 1) grid_reduce_and_plan(lam, a.clock, a.plan, true);
+}
+
+// One step of one flat domain by row-sweep tiles on a resident grid, each wavefront
+// striding over the tiles (band-major order, XCD-major wavefront order).
+template <typename T>
+__global__ __launch_bounds__(kFBlock) void k_flat_sweep(FlatK<T> a) {
+  const FlatClk<T> ck = flat_clock<T, 1>(a);
+  if (!ck.live) return;   // the whole grid sees the same clock
+  FlatCtx<T> x;
+  flat_ctx<T, 1>(a, ck, x);
+  __shared__ T s_ntab[32];
+  if (threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
+  __syncthreads();
+  x.ntab = s_ntab;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
+  uint64_t lam = 0ull;
+  for (int64_t t = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv; t < a.n_tiles; t += nwaves)
+    flat_tile<T>(x, t, lane, lam);
+  grid_reduce_and_plan(lam, a.clock, a.plan, true);
 }
 
 // CFL wave speed of the current flat state over active cells, then plan dt.
@@ -352,6 +542,8 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   if (f->n_runs < 0) return fail(GS_E_ARG, "n_runs < 0");
   if (f->n_runs > 0 && (!f->run_k0 || !f->run_len || !f->run_i || !f->run_j0 || !f->run_d || !f->chunk_run))
     return fail(GS_E_ARG, "run tables are NULL");
+  if (f->n_tiles < 0) return fail(GS_E_ARG, "n_tiles < 0");
+  if (f->n_tiles > 0 && (!f->tile_e0 || !f->tile_irc || !f->ent)) return fail(GS_E_ARG, "tile tables are NULL");
   if (!f->bed) return fail(GS_E_ARG, "bed is NULL");
   for (int s = 0; s < 2; ++s)
     for (int k = 0; k < 3; ++k)
@@ -382,6 +574,10 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   a.run_j0 = f->run_j0;
   a.run_d = f->run_d;
   a.chunk_run = f->chunk_run;
+  a.n_tiles = f->n_tiles;
+  a.tile_e0 = f->tile_e0;
+  a.tile_irc = f->tile_irc;
+  a.ent = f->ent;
   a.P = make_phys<T>(*p, f->dx, f->dy);
   a.rain_mode = fo.n_rain > 0 ? (fo.rain_gridded ? 2 : 1) : 0;
   a.rain_frames = (const T*)fo.rain_frames;
@@ -392,33 +588,59 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   return GS_OK;
 }
 
-// Workgroups of k_flat_step that are resident at once on the current device (cached per
-// device and instantiation); the grid never exceeds the work.
-template <typename T, int MODE>
-static unsigned flat_grid(int64_t nchunks) {
-  static std::atomic<int> cache[64];
+// Workgroups of a one-wavefront-per-item kernel that are resident at once on the current
+// device (cached per device and kernel); the grid never exceeds the work.
+template <typename F>
+static unsigned resident_grid(F kern, std::atomic<int>* cache, int64_t items) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) dev = -1;
   int per = (dev >= 0 && dev < 64) ? cache[dev].load(std::memory_order_relaxed) : 0;
   if (per <= 0) {
     int occ = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_flat_step<T, MODE>, kFBlock, 0) != hipSuccess) occ = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, kFBlock, 0) != hipSuccess) occ = 4;
     if (dev < 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
     per = (occ > 0 ? occ : 1) * (cus > 0 ? cus : 1);
     if (dev >= 0 && dev < 64) cache[dev].store(per, std::memory_order_relaxed);
   }
-  const int64_t need = (nchunks + 3) / 4;
+  const int64_t need = (items + kFBlock / 64 - 1) / (kFBlock / 64);
   return (unsigned)(need < per ? (need > 0 ? need : 1) : per);
 }
 
+template <typename T, int MODE>
+static unsigned flat_grid(int64_t nchunks) {
+  static std::atomic<int> cache[64];
+  return resident_grid(k_flat_step<T, MODE>, cache, nchunks);
+}
+
 template <typename T>
-static int flat_step_t(const GsFlat* f, const GsPhys* p, GsForcing fo, GsClock* clk, hipStream_t s) {
+static unsigned sweep_grid(int64_t ntiles) {
+  static std::atomic<int> cache[64];
+  return resident_grid(k_flat_sweep<T>, cache, ntiles);
+}
+
+// The fused step: row-sweep tiles when the descriptor has them, else chunks of the run table.
+template <typename T>
+static int flat_step_launch(const FlatK<T>& a, hipStream_t s) {
+  if (a.n_tiles > 0) {
+    hipLaunchKernelGGL((k_flat_sweep<T>), dim3(sweep_grid<T>(a.n_tiles)), dim3(kFBlock), 0, s, a);
+    GS_HIP(hipGetLastError(), "k_flat_sweep launch");
+  } else {
+    hipLaunchKernelGGL((k_flat_step<T, 1>), dim3(flat_grid<T, 1>(a.nchunks)), dim3(kFBlock), 0, s, a);
+    GS_HIP(hipGetLastError(), "k_flat_step launch");
+  }
+  return GS_OK;
+}
+
+template <typename T>
+static int flat_step_t(const GsFlat* f, const GsPhys* p, GsForcing fo, GsClock* clk, hipStream_t s,
+                       void* pre0 = nullptr, void* pre1 = nullptr, void* pre2 = nullptr) {
   FlatK<T> a;
   int rc = build_flat<T>(f, p, fo, clk, a, true);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_flat_step<T, 1>), dim3(flat_grid<T, 1>(a.nchunks)), dim3(kFBlock), 0, s, a);
-  GS_HIP(hipGetLastError(), "k_flat_step launch");
-  return GS_OK;
+  a.pre[0] = (T*)pre0;
+  a.pre[1] = (T*)pre1;
+  a.pre[2] = (T*)pre2;
+  return flat_step_launch<T>(a, s);
 }
 
 template <typename T>
@@ -464,15 +686,15 @@ __global__ void k_flat_sample(FlatK<T> a, const int64_t* ids, int n, T* out) {
 }
 
 // ---------------------------------------------------------------------------
-// batched ensemble: every member's chunks in one launch, per-member clocks
+// batched ensemble: every member's sweep tiles in one launch, per-member clocks
 // ---------------------------------------------------------------------------
 template <typename T>
 struct EnsK {
   const FlatK<T>* __restrict__ mem;   // member views (device)
-  const int64_t* __restrict__ chunk0; // [n_members + 1] first global chunk of each member
+  const int64_t* __restrict__ tile0;  // [n_members + 1] first global tile of each member
   int n_members;
-  int cpp;                            // consecutive chunks per wavefront pass
-  int64_t nchunks;                    // total chunks
+  int tpp;                            // consecutive tiles per wavefront pass
+  int64_t ntiles;                     // total tiles
   unsigned* done;                     // workgroups finished in this launch
 };
 
@@ -484,20 +706,20 @@ __device__ __forceinline__ void ens_flush(const EnsK<T>& e, int m, uint64_t lam)
     __hip_atomic_fetch_max(&e.mem[m].clock->lam_acc, lam, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One step of every live member. The global chunk space is the members' chunks end to
-// end; a wavefront takes `cpp` consecutive chunks per pass and strides by the grid, so a
+// One step of every live member. The global tile space is the members' sweep tiles end to
+// end; a wavefront takes `tpp` consecutive tiles per pass and strides by the grid, so a
 // member boundary is crossed at most a few times per wavefront and the member view, its
 // clock values and the wave's speed maximum change only there. The last workgroup to
 // finish plans each member's next dt from its own maximum (plan_finish), as the plain
 // step's last workgroup does for its single domain.
 template <typename T>
 __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>* __restrict__ mem,
-                                                      const int64_t* __restrict__ chunk0) {
+                                                      const int64_t* __restrict__ tile0) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t wave = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv;
-  const int64_t cpp = e.cpp;
+  const int64_t tpp = e.tpp;
   int m = -1;
   int64_t c0 = 0, c1 = 0;
   uint64_t lam = 0ull;
@@ -506,25 +728,25 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>*
   ck.par = 0;
   ck.dt = ck.rain_u = ck.eta = T(0);
   ck.frame = nullptr;
-  for (int64_t base = wave * cpp; base < e.nchunks; base += nwaves * cpp) {
-    const int64_t stop = base + cpp < e.nchunks ? base + cpp : e.nchunks;
+  for (int64_t base = wave * tpp; base < e.ntiles; base += nwaves * tpp) {
+    const int64_t stop = base + tpp < e.ntiles ? base + tpp : e.ntiles;
     for (int64_t gc = base; gc < stop; ++gc) {
       if (gc >= c1) {
         ens_flush(e, m, lam);
         lam = 0ull;
         do {
           ++m;
-          c0 = chunk0[m];
-          c1 = chunk0[m + 1];
+          c0 = tile0[m];
+          c1 = tile0[m + 1];
         } while (gc >= c1);
         ck = flat_clock<T, 1>(mem[m]);   // the member's clock values, once per switch
       }
       if (!ck.live) continue;
-      // the member's layout, rebuilt per chunk from the (scalar-loaded) member table so that
+      // the member's layout, rebuilt per tile from the (scalar-loaded) member table so that
       // it stays a register temporary
       FlatCtx<T> x;
       flat_ctx<T, 1>(mem[m], ck, x);
-      flat_chunk<T, 1>(x, gc - c0, lane, lam);
+      flat_tile<T>(x, gc - c0, lane, lam);
     }
   }
   ens_flush(e, m, lam);
@@ -587,25 +809,11 @@ int gs_flat_step_pre(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsCloc
   using namespace gs;
   if (!pre0 || !pre1 || !pre2) return fail(GS_E_ARG, "pre arrays are NULL");
   if (!f) return fail(GS_E_ARG, "flat descriptor is NULL");
-  if (f->dtype == GS_F64) {
-    FlatK<double> a;
-    int rc = build_flat<double>(f, phys, forc, clock_dev, a, true);
-    if (rc) return rc;
-    a.pre[0] = (double*)pre0; a.pre[1] = (double*)pre1; a.pre[2] = (double*)pre2;
-    hipLaunchKernelGGL((k_flat_step<double, 1>), dim3(flat_grid<double, 1>(a.nchunks)), dim3(kFBlock), 0,
-                       (hipStream_t)stream, a);
-  } else if (f->dtype == GS_F32) {
-    FlatK<float> a;
-    int rc = build_flat<float>(f, phys, forc, clock_dev, a, true);
-    if (rc) return rc;
-    a.pre[0] = (float*)pre0; a.pre[1] = (float*)pre1; a.pre[2] = (float*)pre2;
-    hipLaunchKernelGGL((k_flat_step<float, 1>), dim3(flat_grid<float, 1>(a.nchunks)), dim3(kFBlock), 0,
-                       (hipStream_t)stream, a);
-  } else {
-    return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
-  }
-  GS_HIP(hipGetLastError(), "k_flat_step(pre) launch");
-  return GS_OK;
+  if (f->dtype == GS_F64)
+    return flat_step_t<double>(f, phys, forc, clock_dev, (hipStream_t)stream, pre0, pre1, pre2);
+  if (f->dtype == GS_F32)
+    return flat_step_t<float>(f, phys, forc, clock_dev, (hipStream_t)stream, pre0, pre1, pre2);
+  return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
 }
 
 int gs_flat_sample(const GsFlat* f, const GsClock* clock_dev, const int64_t* ids, int n, void* out, void* stream) {
@@ -644,11 +852,11 @@ struct GsEns {
   int dtype = 0;
   int n = 0;
   void* mem = nullptr;          // device FlatK<T>[n]
-  int64_t* chunk0 = nullptr;    // device [n + 1]
+  int64_t* tile0 = nullptr;     // device [n + 1]
   unsigned* done = nullptr;     // device counter
-  int64_t nchunks = 0;
+  int64_t ntiles = 0;
   int blocks = 0;               // 0: one occupancy wave of the device
-  int cpp = 1;
+  int tpp = 1;
   int cus = 256;
   int occ = 1;                  // resident k_ens_step workgroups per CU
   std::vector<GsFlat> flat;
@@ -657,7 +865,7 @@ struct GsEns {
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int g_batch = 0, g_blocks = -1, g_cpp = -1;
+  int g_batch = 0, g_blocks = -1, g_tpp = -1;
 };
 
 namespace gs {
@@ -673,15 +881,17 @@ static int ens_build_views(GsEns* e, const GsFlat* members, const GsPhys* phys, 
       snprintf(msg, sizeof msg, "%s", g_err);
       return fail(rc, "member %d: %s", m, msg);
     }
-    c0[m + 1] = c0[m] + views[m].nchunks;
+    if (views[m].n_runs > 0 && views[m].n_tiles < 1)
+      return fail(GS_E_ARG, "member %d has active cells but no row-sweep tiles", m);
+    c0[m + 1] = c0[m] + views[m].n_tiles;
   }
-  e->nchunks = c0[e->n];
+  e->ntiles = c0[e->n];
   GS_HIP(hipMalloc(&e->mem, sizeof(FlatK<T>) * (size_t)e->n), "ensemble views");
-  GS_HIP(hipMalloc((void**)&e->chunk0, sizeof(int64_t) * ((size_t)e->n + 1)), "ensemble chunk table");
+  GS_HIP(hipMalloc((void**)&e->tile0, sizeof(int64_t) * ((size_t)e->n + 1)), "ensemble tile table");
   GS_HIP(hipMalloc((void**)&e->done, 256), "ensemble counter");
   GS_HIP(hipMemcpy(e->mem, views.data(), sizeof(FlatK<T>) * (size_t)e->n, hipMemcpyHostToDevice), "views h2d");
-  GS_HIP(hipMemcpy(e->chunk0, c0.data(), sizeof(int64_t) * ((size_t)e->n + 1), hipMemcpyHostToDevice),
-         "chunk table h2d");
+  GS_HIP(hipMemcpy(e->tile0, c0.data(), sizeof(int64_t) * ((size_t)e->n + 1), hipMemcpyHostToDevice),
+         "tile table h2d");
   GS_HIP(hipMemset(e->done, 0, 256), "counter clear");
   return GS_OK;
 }
@@ -690,16 +900,16 @@ template <typename T>
 static int ens_launch(GsEns* e, hipStream_t s) {
   EnsK<T> k;
   k.mem = (const FlatK<T>*)e->mem;
-  k.chunk0 = e->chunk0;
+  k.tile0 = e->tile0;
   k.n_members = e->n;
-  k.cpp = e->cpp;
-  k.nchunks = e->nchunks;
+  k.tpp = e->tpp;
+  k.ntiles = e->ntiles;
   k.done = e->done;
   int64_t blocks = e->blocks > 0 ? e->blocks : (int64_t)e->occ * e->cus;
-  const int64_t need = (e->nchunks + 4LL * e->cpp - 1) / (4LL * e->cpp);
+  const int64_t need = (e->ntiles + 4LL * e->tpp - 1) / (4LL * e->tpp);
   if (blocks > need) blocks = need;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL((k_ens_step<T>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k, k.mem, k.chunk0);
+  hipLaunchKernelGGL((k_ens_step<T>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k, k.mem, k.tile0);
   GS_HIP(hipGetLastError(), "k_ens_step launch");
   return GS_OK;
 }
@@ -732,7 +942,7 @@ static int ens_capture(GsEns* e, int batch) {
   GS_HIP(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0), "graph instantiate");
   e->g_batch = batch;
   e->g_blocks = e->blocks;
-  e->g_cpp = e->cpp;
+  e->g_tpp = e->tpp;
   return GS_OK;
 }
 
@@ -786,23 +996,23 @@ int gs_ens_destroy(GsEns* e) {
   if (e->ev_out) (void)hipEventDestroy(e->ev_out);
   if (e->gstream) (void)hipStreamDestroy(e->gstream);
   if (e->mem) (void)hipFree(e->mem);
-  if (e->chunk0) (void)hipFree(e->chunk0);
+  if (e->tile0) (void)hipFree(e->tile0);
   if (e->done) (void)hipFree(e->done);
   delete e;
   return GS_OK;
 }
 
-int gs_ens_set_launch(GsEns* e, int blocks, int chunks_per_pass) {
+int gs_ens_set_launch(GsEns* e, int blocks, int tiles_per_pass) {
   using namespace gs;
   if (!e) return fail(GS_E_ARG, "ensemble is NULL");
   if (blocks < 0 || blocks > (1 << 24)) return fail(GS_E_RANGE, "blocks must be in 0..2^24");
-  if (chunks_per_pass < 1 || chunks_per_pass > 64) return fail(GS_E_RANGE, "chunks_per_pass must be in 1..64");
+  if (tiles_per_pass < 1 || tiles_per_pass > 64) return fail(GS_E_RANGE, "tiles_per_pass must be in 1..64");
   e->blocks = blocks;
-  e->cpp = chunks_per_pass;
+  e->tpp = tiles_per_pass;
   return GS_OK;
 }
 
-int gs_ens_get_launch(const GsEns* e, int* blocks, int* chunks_per_pass, int64_t* total_chunks) {
+int gs_ens_get_launch(const GsEns* e, int* blocks, int* tiles_per_pass, int64_t* total_tiles) {
   using namespace gs;
   if (!e) return fail(GS_E_ARG, "ensemble is NULL");
   if (blocks) {
@@ -810,8 +1020,8 @@ int gs_ens_get_launch(const GsEns* e, int* blocks, int* chunks_per_pass, int64_t
     if (b == 0) b = e->occ * e->cus;
     *blocks = b;
   }
-  if (chunks_per_pass) *chunks_per_pass = e->cpp;
-  if (total_chunks) *total_chunks = e->nchunks;
+  if (tiles_per_pass) *tiles_per_pass = e->tpp;
+  if (total_tiles) *total_tiles = e->ntiles;
   return GS_OK;
 }
 
@@ -836,7 +1046,7 @@ int gs_ens_run(GsEns* e, int64_t nsteps, int graph_batch, void* stream) {
   if (graph_batch > 100000) return fail(GS_E_RANGE, "graph_batch too large");
   GS_HIP(hipEventRecord(e->ev_in, s), "event record");
   GS_HIP(hipStreamWaitEvent(e->gstream, e->ev_in, 0), "stream wait");
-  if (!e->exec || e->g_batch != graph_batch || e->g_blocks != e->blocks || e->g_cpp != e->cpp) {
+  if (!e->exec || e->g_batch != graph_batch || e->g_blocks != e->blocks || e->g_tpp != e->tpp) {
     const int rc = ens_capture(e, graph_batch);
     if (rc) return rc;
   }
@@ -899,19 +1109,19 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
   };
   const int occ = e->occ;
   int per_cu[4] = {1, 2, occ > 1 ? occ / 2 : 1, occ};
-  const int cpps[4] = {1, 2, 4, 8};
+  const int tpps[2] = {1, 2};
   hipEvent_t t0 = nullptr, t1 = nullptr;
   (void)hipEventCreate(&t0);
   (void)hipEventCreate(&t1);
   double best = 1e300;
-  int best_b = e->blocks, best_c = e->cpp;
+  int best_b = e->blocks, best_c = e->tpp;
   for (int a = 0; a < 4 && rc == GS_OK; ++a) {
     bool dup = false;
     for (int b = 0; b < a; ++b) dup = dup || per_cu[b] == per_cu[a];
     if (dup || per_cu[a] > occ) continue;
-    for (int c = 0; c < 4 && rc == GS_OK; ++c) {
+    for (int c = 0; c < 2 && rc == GS_OK; ++c) {
       e->blocks = per_cu[a] * e->cus;
-      e->cpp = cpps[c];
+      e->tpp = tpps[c];
       rc = restore();
       if (!rc) rc = ens_step(e, s);   // warm-up
       if (!rc && hipEventRecord(t0, s) != hipSuccess) rc = fail(GS_E_STATE, "event record failed");
@@ -922,14 +1132,14 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
       if (!rc && hipEventElapsedTime(&ms, t0, t1) == hipSuccess && ms / trial_steps < best) {
         best = ms / trial_steps;
         best_b = e->blocks;
-        best_c = e->cpp;
+        best_c = e->tpp;
       }
     }
   }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   e->blocks = best_b;
-  e->cpp = best_c;
+  e->tpp = best_c;
   const int rc2 = restore();
   cleanup();
   if (rc) return rc;

@@ -3,7 +3,7 @@
 The reference advances independent scenarios one after another, each through
 ``CompressedSolver.run`` (src/geoswe/compressed_solver.py:3359-3450) with one host-driven
 launch sequence per step. ``Ensemble`` advances all members together: one launch of
-``k_ens_step`` per step walks every member's chunks on a resident grid, each member keeps
+``k_ens_step`` per step walks every member's sweep tiles on a resident grid, each member keeps
 its own device clock (its own CFL dt, rain frame and tide level), the last workgroup plans
 every member's next dt, and the steps are replayed from a captured hipGraph. The cell
 arithmetic is the flat step's own device code, so member k's state is bit for bit the one
@@ -17,6 +17,7 @@ from typing import Optional, Sequence
 
 from . import _lib as L
 from .compressed import CompressedSolver
+from .flatmesh import mesh_sweep_tiles, sweep_rows_for
 
 
 class Ensemble:
@@ -40,6 +41,19 @@ class Ensemble:
             m._open_mode()
         n = len(members)
         self._descs = (L.GsFlat * n)(*[m._desc() for m in members])
+        # Tile height for the whole ensemble: members run side by side, so a small member
+        # needs no short tiles to fill the GPU (its own tables are sized for it alone).
+        self.sweep_rows = sweep_rows_for(sum(int(m.fm.tile_irc[:, 1].sum()) for m in members))
+        self._tiles = []
+        for k, m in enumerate(members):
+            if int(m.fm.tile_irc[:, 1].max(initial=0)) >= self.sweep_rows or m.n_active == 0:
+                continue                       # already at least as tall: keep the member's tiles
+            e0, irc, ent = mesh_sweep_tiles(m.fm, self.sweep_rows)
+            arrs = [L.DeviceArray.from_host(a) for a in (e0, irc.reshape(-1), ent.reshape(-1))]
+            self._tiles.append(arrs)
+            d = self._descs[k]
+            d.n_tiles = len(e0)
+            d.tile_e0, d.tile_irc, d.ent = (a.ptr for a in arrs)
         self._phys = (L.GsPhys * n)(*[m._phys() for m in members])
         self._forc = (L.GsForcing * n)(*[m._fo.s for m in members])
         self._clocks = (C.c_void_p * n)(*[m._clock.ptr for m in members])
@@ -64,10 +78,10 @@ class Ensemble:
     def launch(self) -> dict:
         b, c, t = C.c_int(), C.c_int(), C.c_int64()
         L.call("gs_ens_get_launch", self._h, C.byref(b), C.byref(c), C.byref(t))
-        return {"blocks": b.value, "chunks_per_pass": c.value, "total_chunks": t.value}
+        return {"blocks": b.value, "tiles_per_pass": c.value, "total_tiles": t.value}
 
-    def set_launch(self, blocks: int = 0, chunks_per_pass: int = 1) -> None:
-        L.call("gs_ens_set_launch", self._h, int(blocks), int(chunks_per_pass))
+    def set_launch(self, blocks: int = 0, tiles_per_pass: int = 1) -> None:
+        L.call("gs_ens_set_launch", self._h, int(blocks), int(tiles_per_pass))
 
     # ------------------------------------------------------------------ stepping
     def _check(self) -> None:

@@ -49,6 +49,9 @@ class FlatMesh:
     run_j0: np.ndarray        # int32
     run_d: np.ndarray         # (R, 4) int32 offsets of rows i+1, i-1, i+2, i-2
     chunk_run: np.ndarray     # int32, first run ending past each chunk start
+    tile_e0: np.ndarray = None    # (T,) int64 first window-row entry (row i0 - 2) of each sweep tile
+    tile_irc: np.ndarray = None   # (T, 3) int32 padded row i0, rows swept, padded column of lane 0
+    ent: np.ndarray = None        # (E, 3) uint64 first stored flat id, stored-lane mask, active-lane mask
 
     def pack(self, padded: np.ndarray) -> np.ndarray:
         """Gather a padded (..., nxp, nyp) field into flat order."""
@@ -65,7 +68,7 @@ class FlatMesh:
         return out
 
 
-def build_flat_mesh(inside_padded: np.ndarray, ngh: int, ring: int = 2) -> FlatMesh:
+def build_flat_mesh(inside_padded: np.ndarray, ngh: int, ring: int = 2, sweep_rows: int = None) -> FlatMesh:
     """Stored set, flat order and run table of an active mask given on the padded grid."""
     inside = np.asarray(inside_padded, dtype=bool)
     nxp, nyp = inside.shape
@@ -116,6 +119,107 @@ def build_flat_mesh(inside_padded: np.ndarray, ngh: int, ring: int = 2) -> FlatM
         hi = run_end - 1 + run_d.max(axis=1)
         if lo.min() < 0 or hi.max() >= n:
             raise AssertionError("run offsets leave the stored range")
+    tile_e0, tile_irc, ent = build_sweep_tiles(stored, inside, sweep_rows)
     return FlatMesh(nxp=nxp, nyp=nyp, ngh=ngh, n_stored=n, n_active=int(act.sum()), ij=ij, active=act,
                     run_k0=run_k0.astype(np.int64), run_len=run_len, run_i=run_i, run_j0=run_j0,
-                    run_d=np.ascontiguousarray(run_d), chunk_run=chunk_run)
+                    run_d=np.ascontiguousarray(run_d), chunk_run=chunk_run,
+                    tile_e0=tile_e0, tile_irc=tile_irc, ent=ent)
+
+
+WRITE_LANES = np.uint64(((1 << 62) - 1) & ~3)   # lanes 2..61 of a 64-lane sweep window
+TILE_TARGET = 8192   # tiles a domain should give at least (~1.5 resident wavefronts of an MI355X each)
+
+
+def _window_masks(m: np.ndarray, nwin: int) -> np.ndarray:
+    """(rows, nwin) uint64: bit l of window k = m[:, CHUNK*k + l], l = 0..63."""
+    rows = m.shape[0]
+    v = np.lib.stride_tricks.as_strided(m, shape=(rows, nwin, 64),
+                                        strides=(m.strides[0], CHUNK * m.strides[1], m.strides[1]))
+    return np.packbits(v, axis=2, bitorder="little").view("<u8")[..., 0]
+
+
+def sweep_rows_for(busy: int) -> int:
+    """Tile height for `busy` window rows with an active lane: the largest power of two
+    <= 64 that still gives about TILE_TARGET tiles."""
+    rows = 1
+    while rows < 64 and int(busy) // (2 * rows) >= TILE_TARGET:
+        rows *= 2
+    return rows
+
+
+def mesh_sweep_tiles(fm: "FlatMesh", rows: int):
+    """Sweep tiles of an existing flat mesh at another tile height (same stored set)."""
+    stored = np.zeros((fm.nxp, fm.nyp), bool)
+    stored[fm.ij[:, 0], fm.ij[:, 1]] = True
+    inside = np.zeros((fm.nxp, fm.nyp), bool)
+    a = fm.ij[fm.active]
+    inside[a[:, 0], a[:, 1]] = True
+    return build_sweep_tiles(stored, inside, rows)
+
+
+def build_sweep_tiles(stored: np.ndarray, active: np.ndarray, rows: int = None):
+    """Row-sweep tiles of the flat step.
+
+    Window k covers the padded columns CHUNK*k .. CHUNK*k+63 (64 lanes, lanes 2..61 are
+    written, so every active column 2 .. nyp-3 belongs to exactly one window). For every
+    padded row and window one entry holds the flat id of the row's first stored cell at
+    column >= CHUNK*k and the stored- and active-lane masks: the stored cells of a row are
+    consecutive flat ids, so lane l's id is base + popcount(stored & lanes below l). A tile
+    is a window over at most `rows` consecutive rows (bands of `rows` rows, trimmed to the
+    rows with an active lane); its entries are rows i0-2 .. i0+nr+1 (the stencil's two rows
+    each side), stored contiguously. Tiles are ordered band-major, so the tiles that run
+    side by side read neighbouring columns of the same rows.
+
+    `rows=None` picks the tile height from the domain: the largest power of two <= 64 that
+    still gives about TILE_TARGET tiles, so that a small domain run alone keeps every CU
+    busy (a tile of r rows evaluates 1 + 1/r x-faces per row instead of 2).
+    """
+    stored = np.asarray(stored, dtype=bool)
+    active = np.asarray(active, dtype=bool)
+    nxp, nyp = stored.shape
+    if rows is not None and rows < 1:
+        raise ValueError("rows must be >= 1")
+    nwin = max(1, -(-(nyp - 4) // CHUNK))
+    width = CHUNK * nwin + 4
+    S = np.zeros((nxp, width), bool)
+    S[:, :nyp] = stored
+    smask = _window_masks(S, nwin)
+    A = np.zeros((nxp, width), bool)
+    A[:, :nyp] = active
+    amask = _window_masks(A, nwin) & WRITE_LANES
+    del A
+    if rows is None:
+        rows = sweep_rows_for(np.count_nonzero(amask))        # window rows with an active lane
+    per_block = S[:, :CHUNK * nwin].reshape(nxp, nwin, CHUNK).sum(axis=2, dtype=np.int64)
+    del S
+    row_start = np.zeros(nxp, np.int64)
+    row_start[1:] = np.cumsum(stored.sum(axis=1, dtype=np.int64))[:-1]
+    base = np.zeros((nxp, nwin), np.int64)
+    base[:, 1:] = np.cumsum(per_block, axis=1)[:, :-1]
+    base += row_start[:, None]
+    # tiles: band b x window k, trimmed to its rows with an active lane
+    nb = -(-nxp // rows)
+    live = np.zeros((nb * rows, nwin), bool)
+    live[:nxp] = amask != 0
+    live = live.reshape(nb, rows, nwin)
+    anyl = live.any(axis=1)                                   # (nb, nwin)
+    first = np.argmax(live, axis=1)
+    last = rows - 1 - np.argmax(live[:, ::-1, :], axis=1)
+    bb, kk = np.nonzero(anyl)                                 # band-major order
+    i0 = (bb * rows + first[bb, kk]).astype(np.int64)
+    nr = (last[bb, kk] - first[bb, kk] + 1).astype(np.int64)
+    if i0.size and (i0.min() < 2 or (i0 + nr + 1).max() > nxp - 1):
+        raise AssertionError("a sweep tile's stencil rows leave the padded grid")
+    cnt = nr + 4
+    tile_e0 = np.zeros(i0.size, np.int64)
+    if i0.size:
+        tile_e0[1:] = np.cumsum(cnt)[:-1]
+    ne = int(cnt.sum())
+    erow = np.repeat(i0 - 2 - tile_e0, cnt) + np.arange(ne, dtype=np.int64)
+    ewin = np.repeat(kk.astype(np.int64), cnt)
+    ent = np.empty((ne, 3), np.uint64)
+    ent[:, 0] = base[erow, ewin].astype(np.uint64)
+    ent[:, 1] = smask[erow, ewin]
+    ent[:, 2] = amask[erow, ewin]
+    tile_irc = np.stack([i0, nr, kk.astype(np.int64) * CHUNK], axis=1).astype(np.int32)
+    return tile_e0, np.ascontiguousarray(tile_irc), ent

@@ -163,6 +163,15 @@ typedef struct GsFlat {
   const int32_t* run_j0;     /* [n_runs] padded column of the first cell */
   const int32_t* run_d;      /* [n_runs][4] id offsets of rows i+1, i-1, i+2, i-2 */
   const int32_t* chunk_run;  /* [ceil(N/60)] first run whose end is past the chunk start */
+  /* Row-sweep tiles (the step's traversal; n_tiles = 0 steps by chunks of the run table).
+   * Window w covers padded columns 60w .. 60w+63 (lanes 2..61 written); entry = (flat id
+   * of the row's first stored cell at column >= 60w, stored-lane mask, active-lane mask)
+   * for one padded row of one window. Tile t sweeps rows i0 .. i0+nr-1 of one window;
+   * its entries are rows i0-2 .. i0+nr+1, from ent[3 * tile_e0[t]] on. */
+  int64_t n_tiles;
+  const int64_t* tile_e0;    /* [n_tiles] entry index of row i0 - 2 */
+  const int32_t* tile_irc;   /* [n_tiles][3] i0, nr, padded column of lane 0 */
+  const uint64_t* ent;       /* [entries][3] */
 } GsFlat;
 
 /* ---- library ---- */
@@ -314,7 +323,8 @@ int gs_adj_flat_sweep(const GsAdjFlat* a, const GsPhys* phys, int n, const doubl
 /* ---- batched ensemble of flat domains ---- */
 /*
  * Independent flat domains (members of uneven size, each with its own bed, Manning table,
- * rain, stage and clock) advanced together: one launch per step over all members' chunks,
+ * rain, stage and clock) advanced together: one launch per step over all members' sweep
+ * tiles (members need n_tiles > 0),
  * each member with its own CFL dt planned on the device, steps replayed from a hipGraph.
  * No reference counterpart: the reference runs members one after another through
  * CompressedSolver.run (compressed_solver.py:3359-3450). Member k's results are bitwise
@@ -329,9 +339,9 @@ int gs_ens_create(const GsFlat* members, const GsPhys* phys, const GsForcing* fo
                   GsClock* const* clocks, int n_members, GsEns** out);
 int gs_ens_destroy(GsEns* e);
 /* Launch shape: resident workgroups (0 = one full occupancy wave of the device) and the
- * consecutive chunks of 60 flat ids a wavefront takes per pass (1..64). */
-int gs_ens_set_launch(GsEns* e, int blocks, int chunks_per_pass);
-int gs_ens_get_launch(const GsEns* e, int* blocks, int* chunks_per_pass, int64_t* total_chunks);
+ * consecutive row-sweep tiles a wavefront takes per pass (1..64). */
+int gs_ens_set_launch(GsEns* e, int blocks, int tiles_per_pass);
+int gs_ens_get_launch(const GsEns* e, int* blocks, int* tiles_per_pass, int64_t* total_tiles);
 /* One step of every member that is not done. */
 int gs_ens_step(GsEns* e, void* stream);
 /* nsteps steps; graph_batch > 0 replays a captured hipGraph of graph_batch steps. */

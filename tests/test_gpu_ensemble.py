@@ -59,12 +59,12 @@ def test_ensemble_launch_shapes_and_autotune(cases):
     mem = flats(cases)
     ens = Ensemble(mem, graph_batch=8)
     got = ens.autotune(trial_steps=2)                 # restores the state it timed on
-    assert got["blocks"] >= 1 and 1 <= got["chunks_per_pass"] <= 8
+    assert got["blocks"] >= 1 and 1 <= got["tiles_per_pass"] <= 2
     assert all(m.nsteps == 0 for m in mem)
     ens.advance(10)
-    ens.set_launch(blocks=3, chunks_per_pass=5)       # a small, odd grid: every chunk still visited
+    ens.set_launch(blocks=3, tiles_per_pass=5)        # a small, odd grid: every tile still visited
     ens.advance(10)
-    ens.set_launch(blocks=0, chunks_per_pass=1)
+    ens.set_launch(blocks=0, tiles_per_pass=1)
     ens.advance(10, graph_batch=4)
     assert_same(plain, mem)
     ens.close()

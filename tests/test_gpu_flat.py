@@ -123,3 +123,54 @@ def test_flat_classes_gridded_rain_stage(gpu):
     o.cfl_skip = ~m
     o.run(2.5)
     assert np.array_equal(s.q_interior, o.interior)
+
+
+def _traversal_masks(nx, ny):
+    rng = np.random.default_rng(41)
+    out = [("irregular", irregular_mask(nx, ny, seed=5))]
+    m = np.zeros((nx, ny), bool)          # islands closer together than one 64-column window
+    m[3:9, 5:15] = m[2:12, 30:36] = m[20:40, 50:58] = True
+    m[30, 60:ny - 1] = True
+    m[nx - 6:, 100:110] = True
+    out.append(("islands", m))
+    out.append(("full", np.ones((nx, ny), bool)))
+    m = rng.random((nx, ny)) > 0.35         # speckle: most window rows are partly stored
+    out.append(("speckle", m))
+    return out
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_sweep_tiles_equal_chunk_traversal(gpu, dtype):
+    """The row-sweep traversal and the chunk traversal of the run table give the same
+    state, running maximum, time and step count, for ragged, islanded, full and speckled
+    masks (three windows wide), tiles of 1, 5 and 64 rows, with Manning classes, gridded
+    rain and a tidal stage row."""
+    from geoswe_amd import CompressedSolver, RainfallForcing, StageBoundary, Mesh2D
+    nx, ny = 48, 150
+    q0, bed = rough_state(nx, ny, seed=29, depth=0.6, rough=1.0, vel=0.2)
+    rng = np.random.default_rng(4)
+    frames = rng.random((2, nx, ny)) * 80.0
+    n = np.choose(rng.integers(0, 8, (nx, ny)), np.linspace(0.02, 0.12, 8))
+    sea = np.zeros((nx, ny), bool)
+    sea[0, :] = True
+    sb = StageBoundary.from_mask(sea, Mesh2D(nx, ny, 2.0, 2.0), bed, np.array([0.0, 5.0]), np.array([0.5, 1.0]))
+    for name, m in _traversal_masks(nx, ny):
+        m = m.copy()
+        m[0, :] = True
+        for rows in (1, 5, 64):
+            runs = []
+            for trav in ("tiles", "chunks"):
+                s = dense(q0 * m, bed, friction="manning", dtype=dtype, stage_boundary=sb,
+                          rainfall_forcing=RainfallForcing(np.array([0.0, 1.0]), frames))
+                s.set_manning(n)
+                s.set_inside_mask(m)
+                s.set_cfl_ghost_mask(~m)
+                c = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=True, sweep_rows=rows)
+                c.traversal = trav
+                c.run(1.5)
+                runs.append((c.q_flat(), c.max_depth(), c.t, c.nsteps))
+            (qa, ma, ta, na), (qb, mb, tb, nb) = runs
+            print(name, rows, dtype, na, flush=True)
+            assert na == nb > 3 and ta == tb, (name, rows)
+            assert np.array_equal(qa, qb), (name, rows)
+            assert np.array_equal(ma, mb), (name, rows)

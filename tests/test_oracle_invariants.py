@@ -257,3 +257,71 @@ def test_planted_mistakes_are_caught(monkeypatch):
     monkeypatch.setattr(O, "hllc", lambda hL, uNL, uTL, hR, uNR, uTR, g, hm: orig_h(hL, uTL, uNL, hR, uTR, uNR, g, hm))
     with pytest.raises((AssertionError, FloatingPointError)):
         test_ritter_dry_dam_break_front()
+
+
+def _tile_walk(fm, inside):
+    """Visit every (tile, row, lane) of the sweep tiles by the kernel's own rule and check
+    ids against the row-major numbering; returns how often each active cell was written."""
+    stored = np.zeros((fm.nxp, fm.nyp), bool)
+    stored[fm.ij[:, 0], fm.ij[:, 1]] = True
+    idmap = np.full((fm.nxp, fm.nyp), -1, np.int64)
+    idmap[fm.ij[:, 0], fm.ij[:, 1]] = np.arange(fm.n_stored)
+    written = np.zeros((fm.nxp, fm.nyp), np.int64)
+    for t in range(len(fm.tile_e0)):
+        i0, nr, c0 = (int(v) for v in fm.tile_irc[t])
+        assert 1 <= nr and c0 % 60 == 0
+        for r in range(-2, nr + 2):
+            base, sm, am = (int(v) for v in fm.ent[fm.tile_e0[t] + r + 2])
+            i = i0 + r
+            for lane in range(64):
+                j = c0 + lane
+                st = (sm >> lane) & 1
+                assert st == (j < fm.nyp and stored[i, j])
+                if st:
+                    assert base + bin(sm & ((1 << lane) - 1)).count("1") == idmap[i, j]
+                if 0 <= r < nr and (am >> lane) & 1:   # halo rows' active masks are never read
+                    assert 2 <= lane < 62 and inside[i, j]
+                    written[i, j] += 1
+                    for di, dj in ((1, 0), (-1, 0), (2, 0), (-2, 0), (0, 1), (0, -1), (0, 2), (0, -2)):
+                        assert stored[i + di, j + dj]
+    return written
+
+
+@pytest.mark.parametrize("rows", [1, 3, 8, 64])
+def test_sweep_tiles_cover_every_active_cell_once(rows):
+    """Row-sweep tiles: lane ids from base + popcount(stored lanes below) equal the
+    row-major numbering, stored-lane masks equal the stored set, and every active cell
+    is written by exactly one (tile, row, lane) whose +-2 stencil is stored."""
+    from geoswe_amd.flatmesh import build_flat_mesh
+    rng = np.random.default_rng(31)
+    masks = []
+    m = np.zeros((70, 150), bool)                   # ragged blob wider than two windows
+    m[5:64, 4:140] = rng.random((59, 136)) > 0.25
+    masks.append(m)
+    m = np.zeros((30, 200), bool)                   # islands closer together than a window
+    m[4:8, 10:20] = m[4:9, 40:45] = m[12:26, 58:66] = m[20, 120:196] = True
+    masks.append(m)
+    m = np.zeros((12, 12), bool)                    # single cell
+    m[6, 6] = True
+    masks.append(m)
+    m = np.zeros((40, 67), bool)                    # active up to the last allowed column/row
+    m[2:38, 2:65] = True
+    masks.append(m)
+    for inside in masks:
+        fm = build_flat_mesh(inside, 2 if inside[:, 2].any() else 4, sweep_rows=rows)
+        written = _tile_walk(fm, inside)
+        assert np.array_equal(written, inside.astype(np.int64))
+        assert np.all(fm.tile_irc[:, 1] <= rows)
+        # the same tables rebuilt from the mesh alone (an ensemble's taller tiles)
+        from geoswe_amd.flatmesh import mesh_sweep_tiles
+        e0, irc, ent = mesh_sweep_tiles(build_flat_mesh(inside, fm.ngh, sweep_rows=1), rows)
+        assert np.array_equal(e0, fm.tile_e0) and np.array_equal(irc, fm.tile_irc) and np.array_equal(ent, fm.ent)
+
+
+def test_sweep_rows_for_domain_size():
+    from geoswe_amd.flatmesh import TILE_TARGET, sweep_rows_for
+    assert sweep_rows_for(0) == 1 and sweep_rows_for(2 * TILE_TARGET - 1) == 1
+    assert sweep_rows_for(2 * TILE_TARGET) == 2 and sweep_rows_for(10 ** 9) == 64
+    for busy in (1, 5000, 40000, 300000, 2000000):
+        r = sweep_rows_for(busy)
+        assert r == 64 or busy // r >= TILE_TARGET > busy // (2 * r) or r == 1
