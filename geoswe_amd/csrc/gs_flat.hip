@@ -57,8 +57,6 @@ struct FlatK {
   const uint64_t* ent;
 };
 
-template <typename T> __device__ __forceinline__ T fl_upper(T x) { return __shfl_down(x, 1, 64); }
-template <typename T> __device__ __forceinline__ T fl_lower(T x) { return __shfl_up(x, 1, 64); }
 
 // Everything one chunk pass reads, as values: the state buffers picked for this step (a
 // runtime index into the argument's buffer table would put the table in private memory and
@@ -208,16 +206,16 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
   const T bC = a.bed[kl];
   T uC, vC;
   prims(hC, huC, hvC, hm, uC, vC);
-  const T bl = fl_lower(bC), br = fl_upper(bC);
+  const T bl = lane_lower(bC), br = lane_upper(bC);
   const T gzy = (T(0.5) * (br - bl)) * iy;
   // upper y-face (k, k+1)
-  const T hR = fl_upper(hC), uR = fl_upper(uC), vR = fl_upper(vC);
-  const T gzyR = fl_upper(gzy);
+  const T hR = lane_upper(hC), uR = lane_upper(uC), vR = lane_upper(vC);
+  const T gzyR = lane_upper(gzy);
   Face<T> fy;
   face<T>(bC, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
-  const T yFhB = fl_lower(fy.Fh), yFNB = fl_lower(fy.FN), yFTB = fl_lower(fy.FT);
-  const T ysRB = fl_lower(fy.srcR);
-  const int ygB = fl_lower((int)fy.gate);
+  const T yFhB = lane_lower(fy.Fh), yFNB = lane_lower(fy.FN), yFTB = lane_lower(fy.FT);
+  const T ysRB = lane_lower(fy.srcR);
+  const int ygB = lane_lower((int)fy.gate);
 
   // run of this cell
   int r = -1;
@@ -343,8 +341,9 @@ __device__ __forceinline__ T sweep_b(const FlatCtx<T>& x, int64_t k, bool st) {
 // a row is carried from the row above, the y-face below a lane comes from the lane below
 // by shuffle, and the residual terms are added in the same order as there (and as in
 // flat_chunk), so all three traversals give the same bits. Every lane of the wavefront
-// must call it.
-template <typename T>
+// must call it. PRE: also write the pre-friction state (the adjoint's recomputation); a
+// separate instantiation keeps those pointers out of the plain step's scalar registers.
+template <typename T, bool PRE>
 __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int lane, uint64_t& lam) {
   const Phys<T>& P = x.P;
   const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
@@ -386,16 +385,16 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
     const T nFh = fn.Fh * ix, nFN = fn.FN * ix, nFT = fn.FT * ix;
 
     // y faces of the row: each lane computes its upper face (j, j+1)
-    const T bl = fl_lower(b0), br = fl_upper(b0);
+    const T bl = lane_lower(b0), br = lane_upper(b0);
     const T gzy = (T(0.5) * (br - bl)) * iy;
-    const T hR = fl_upper(hC), uR = fl_upper(uC), vR = fl_upper(vC);
-    const T gzyR = fl_upper(gzy);
+    const T hR = lane_upper(hC), uR = lane_upper(uC), vR = lane_upper(vC);
+    const T gzyR = lane_upper(gzy);
     Face<T> fy;
     face<T>(b0, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
     const T yFh = fy.Fh * iy, yFN = fy.FN * iy, yFT = fy.FT * iy;
-    const T yFhB = fl_lower(yFh), yFNB = fl_lower(yFN), yFTB = fl_lower(yFT);
-    const T ysRB = fl_lower(fy.srcR * iy);
-    const int ygB = fl_lower((int)fy.gate);
+    const T yFhB = lane_lower(yFh), yFNB = lane_lower(yFN), yFTB = lane_lower(yFT);
+    const T ysRB = lane_lower(fy.srcR * iy);
+    const int ygB = lane_lower((int)fy.gate);
 
     T rh = T(0), rhu = T(0), rhv = T(0);
     if (fn.gate) {
@@ -432,7 +431,7 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
       } else if (x.rain_mode == 2) {
         rr0 = rr0 + x.frame[(int64_t)(i0 + r - x.ngh) * x.rain_pitch + (c0 + lane - x.ngh)];
       }
-      if (x.pre0) {   // the first three statements of update(), kept for its reverse
+      if (PRE) {   // the first three statements of update(), kept for its reverse
         x.pre0[k] = hC + x.dt * rr0;
         x.pre1[k] = huC + x.dt * rhu;
         x.pre2[k] = hvC + x.dt * rhv;
@@ -495,7 +494,7 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
 
 // One step of one flat domain by row-sweep tiles on a resident grid, each wavefront
 // striding over the tiles (band-major order, XCD-major wavefront order).
-template <typename T>
+template <typename T, bool PRE>
 __global__ __launch_bounds__(kFBlock) void k_flat_sweep(FlatK<T> a) {
   const FlatClk<T> ck = flat_clock<T, 1>(a);
   if (!ck.live) return;   // the whole grid sees the same clock
@@ -510,7 +509,7 @@ __global__ __launch_bounds__(kFBlock) void k_flat_sweep(FlatK<T> a) {
   const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
   uint64_t lam = 0ull;
   for (int64_t t = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv; t < a.n_tiles; t += nwaves)
-    flat_tile<T>(x, t, lane, lam);
+    flat_tile<T, PRE>(x, t, lane, lam);
   grid_reduce_and_plan(lam, a.clock, a.plan, true);
 }
 
@@ -612,17 +611,20 @@ static unsigned flat_grid(int64_t nchunks) {
   return resident_grid(k_flat_step<T, MODE>, cache, nchunks);
 }
 
-template <typename T>
+template <typename T, bool PRE>
 static unsigned sweep_grid(int64_t ntiles) {
   static std::atomic<int> cache[64];
-  return resident_grid(k_flat_sweep<T>, cache, ntiles);
+  return resident_grid(k_flat_sweep<T, PRE>, cache, ntiles);
 }
 
 // The fused step: row-sweep tiles when the descriptor has them, else chunks of the run table.
 template <typename T>
 static int flat_step_launch(const FlatK<T>& a, hipStream_t s) {
-  if (a.n_tiles > 0) {
-    hipLaunchKernelGGL((k_flat_sweep<T>), dim3(sweep_grid<T>(a.n_tiles)), dim3(kFBlock), 0, s, a);
+  if (a.n_tiles > 0 && a.pre[0]) {
+    hipLaunchKernelGGL((k_flat_sweep<T, true>), dim3(sweep_grid<T, true>(a.n_tiles)), dim3(kFBlock), 0, s, a);
+    GS_HIP(hipGetLastError(), "k_flat_sweep(pre) launch");
+  } else if (a.n_tiles > 0) {
+    hipLaunchKernelGGL((k_flat_sweep<T, false>), dim3(sweep_grid<T, false>(a.n_tiles)), dim3(kFBlock), 0, s, a);
     GS_HIP(hipGetLastError(), "k_flat_sweep launch");
   } else {
     hipLaunchKernelGGL((k_flat_step<T, 1>), dim3(flat_grid<T, 1>(a.nchunks)), dim3(kFBlock), 0, s, a);
@@ -746,7 +748,7 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>*
       // it stays a register temporary
       FlatCtx<T> x;
       flat_ctx<T, 1>(mem[m], ck, x);
-      flat_tile<T>(x, gc - c0, lane, lam);
+      flat_tile<T, false>(x, gc - c0, lane, lam);
     }
   }
   ens_flush(e, m, lam);

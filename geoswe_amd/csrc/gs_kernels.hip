@@ -145,8 +145,6 @@ __device__ __forceinline__ T load_b(const DenseK<T>& a, int i, const ColMap& cm)
   return a.bed[(int64_t)mi * a.pitch + cm.mjb];
 }
 
-template <typename T> __device__ __forceinline__ T from_upper(T x) { return __shfl_down(x, 1, 64); }
-template <typename T> __device__ __forceinline__ T from_lower(T x) { return __shfl_up(x, 1, 64); }
 
 // MODE 0: residual into a.rhs; MODE 1: fused step.
 template <typename T, int MODE>
@@ -185,7 +183,8 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
   const unsigned qd = nb / 8u, rm = nb % 8u, xcd = b % 8u, kk = b / 8u;
   const unsigned lb = (xcd < rm) ? xcd * (qd + 1u) + kk : rm * (qd + 1u) + (xcd - rm) * qd + kk;
   const int lane = threadIdx.x & 63;
-  const int tile = (int)(lb * 4u + (threadIdx.x >> 6));
+  // the wavefront index as a scalar: tile, rows and row addresses are then uniform values
+  const int tile = (int)(lb * 4u) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   uint64_t lam = 0ull;
 
   if (tile < a.ntiles) {
@@ -224,16 +223,16 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
       const T nFh = fn.Fh * ix, nFN = fn.FN * ix, nFT = fn.FT * ix;
 
       // y faces of row i: lane computes its upper face (c, c+1)
-      const T bl = from_lower(b0), br = from_upper(b0);
+      const T bl = lane_lower(b0), br = lane_upper(b0);
       const T gzy = (T(0.5) * (br - bl)) * iy;
-      const T hR = from_upper(hC), uR = from_upper(uC), vR = from_upper(vC);
-      const T gzyR = from_upper(gzy);
+      const T hR = lane_upper(hC), uR = lane_upper(uC), vR = lane_upper(vC);
+      const T gzyR = lane_upper(gzy);
       Face<T> fy;
       face<T>(b0, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
       const T yFh = fy.Fh * iy, yFN = fy.FN * iy, yFT = fy.FT * iy;
-      const T yFhB = from_lower(yFh), yFNB = from_lower(yFN), yFTB = from_lower(yFT);
-      const T ysRB = from_lower(fy.srcR * iy);
-      const int ygB = from_lower((int)fy.gate);
+      const T yFhB = lane_lower(yFh), yFNB = lane_lower(yFN), yFTB = lane_lower(yFT);
+      const T ysRB = lane_lower(fy.srcR * iy);
+      const int ygB = lane_lower((int)fy.gate);
 
       T rh = T(0), rhu = T(0), rhv = T(0);
       if (fn.gate) {
