@@ -78,6 +78,43 @@ def cpu_baseline(bed_spec, crop=(1024, 1000), steps=80):
                       f"{steps} steps in {el:.1f} s, one thread; host has {os.cpu_count()} logical CPUs"}
 
 
+def valu_calibration(reps: int = 2) -> dict:
+    """VALU issue rate of the device (gs_bench_valu): wave64 instructions per second for
+    independent float32 multiply/add chains (mode 0) and for chains in which half the
+    instructions are hardware square roots and reciprocals (mode 1); from the two, the issue
+    cost of one transcendental instruction in simple-instruction slots. Each figure is the
+    best of `reps` bursts of 10 back-to-back launches (~1-3 ms each) after a warm-up burst."""
+    from geoswe_amd import _lib as L
+    blocks, iters = 8 * 256, 4096                  # 8 workgroups per CU of the MI355X's 256
+    out = L.DeviceArray((blocks * 256,), np.float32)
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    L.call("gs_event_create", C.byref(e0))
+    L.call("gs_event_create", C.byref(e1))
+    rate = {}
+    burst = 10
+    for mode, per_step in ((0, 16), (1, 32)):
+        # a sustained burst, as the step runs: back-to-back launches after a warm-up burst
+        # (the shader clock ramps up under load; isolated launches run at a lower clock)
+        best = None
+        for _ in range(reps):
+            for _ in range(burst):
+                L.call("gs_bench_valu", C.c_void_p(out.ptr), blocks, iters, mode, None)
+            L.call("gs_event_record", e0, None)
+            for _ in range(burst):
+                L.call("gs_bench_valu", C.c_void_p(out.ptr), blocks, iters, mode, None)
+            L.call("gs_event_record", e1, None)
+            L.sync()
+            ms = C.c_float()
+            L.call("gs_event_elapsed_ms", C.byref(ms), e0, e1)
+            per = ms.value / burst
+            best = per if best is None else min(best, per)
+        rate[mode] = blocks * 4 * (iters - iters % 8) * per_step / (best * 1e-3)
+    # mode 1 per step: 16 simple + 16 transcendental; time = (16 + 16 x) / rate0 = 32 / rate1
+    slots = (32.0 * rate[0] / rate[1] - 16.0) / 16.0
+    return {"simple_wave_insts_per_s": rate[0], "mixed_wave_insts_per_s": rate[1],
+            "transcendental_slots": slots, "workgroups": blocks, "iters": iters}
+
+
 def copy_calibration(nbytes: int, reps: int):
     """Time gs_bench_copy (read n + write n bytes per launch); prints one JSON line."""
     from geoswe_amd import _lib as L
@@ -501,6 +538,9 @@ def main():
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="instead of the step, time the plain copy kernel over this many bytes (HBM calibration)")
     ap.add_argument("--copy-reps", type=int, default=10)
+    ap.add_argument("--valu-peak", action="store_true",
+                    help="instead of the step, time the VALU calibration kernel (issue rate of simple and "
+                         "transcendental instructions)")
     ap.add_argument("--workload", choices=["rain-on-slope", "coastal", "ensemble", "adjoint", "dambreak"], default="rain-on-slope",
                     help="coastal: the compressed coastal case (flat and dense layouts, bitwise cross-check); "
                          "ensemble: batched uneven members vs sequential runs; adjoint: gauge-misfit gradients")
@@ -513,6 +553,11 @@ def main():
     args = ap.parse_args()
     if args.copy_bytes:
         return copy_calibration(args.copy_bytes, args.copy_reps)
+    if args.valu_peak:
+        v = valu_calibration()
+        v["simple_wave_insts_per_cu_per_ns"] = v["simple_wave_insts_per_s"] / 256 / 1e9   # 256 CUs
+        print(json.dumps(v), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build (a multi-GPU run "
               f"would split the domain into strips along y with a 2-cell halo exchange per step)", file=sys.stderr)
@@ -561,6 +606,14 @@ def main():
             # over this run's launch time
             valu = {"insts_per_cell": v["valu_insts_per_cell"], "wave_insts_per_launch": v["valu_wave_insts_per_launch"],
                     "wave_insts_per_s": v["valu_wave_insts_per_launch"] / (kern_ms * 1e-3), "source": v["source"]}
+            # the device's measured float32 multiply/add issue rate (gs_bench_valu) and the
+            # step's VALU rate against it: at or above 1 the step is bound by instruction issue
+            # (its compares, selects and moves issue faster than float32 arithmetic)
+            cal = valu_calibration()
+            valu.update({"fp32_peak_wave_insts_per_s": cal["simple_wave_insts_per_s"],
+                         "rate_over_fp32_peak": valu["wave_insts_per_s"] / cal["simple_wave_insts_per_s"],
+                         "transcendental_slots": cal["transcendental_slots"],
+                         "peak_source": "gs_bench_valu (bench.py --valu-peak)"})
     log(f"{ms_step:.3f} ms/step wall, {kern_ms:.3f} ms/launch (events), {value / 1e9:.2f} G cell-updates/s, "
         f"{achieved:.0f} GB/s algorithmic")
     h32 = None

@@ -119,6 +119,34 @@ static unsigned grid_for(int64_t n, int64_t cap = 65536) {
 
 using namespace gs;
 
+// VALU issue-rate calibration: each lane runs 8 independent float32 chains of `iters` steps
+// (MODE 0: a*m then +d, 16 VALU instructions per step; MODE 1: hardware square root then *m,
+// hardware reciprocal then +d, 32 per step, half of them transcendental), 8 steps per loop
+// trip so that loop control is negligible. The chains are stored so that nothing is dead;
+// `out` holds gridDim.x * 256 floats. `iters` is rounded down to a multiple of 8.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_valu(float* out, int iters, float m, float d) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  float a[8];
+  for (int k = 0; k < 8; ++k) a[k] = (float)(gid + k) * 1.0e-3f;
+  for (int i = 0; i < iters / 8; ++i) {
+#pragma unroll
+    for (int kk = 0; kk < 64; ++kk) {
+      const int k = kk & 7;
+      if (MODE == 0) {
+        a[k] = a[k] * m;
+        a[k] = a[k] + d;
+      } else {
+        a[k] = __builtin_amdgcn_sqrtf(a[k]) * m;
+        a[k] = __builtin_amdgcn_rcpf(a[k]) + d;
+      }
+    }
+  }
+  float sum = 0.0f;
+  for (int k = 0; k < 8; ++k) sum = sum + a[k];
+  out[gid] = sum;
+}
+
 extern "C" {
 
 int gs_fill(void* dst, int dtype, int64_t rows, int64_t cols, int64_t pitch, double value, void* stream) {
@@ -173,6 +201,17 @@ int gs_bench_copy(void* dst, const void* src, size_t bytes, void* stream) {
   hipLaunchKernelGGL(k_copy, dim3(grid_for(n16, 1 << 20)), dim3(256), 0, (hipStream_t)stream, (const uint4*)src,
                      (uint4*)dst, n16);
   GS_HIP(hipGetLastError(), "k_copy launch");
+  return GS_OK;
+}
+
+int gs_bench_valu(float* out, int blocks, int iters, int mode, void* stream) {
+  if (!out || blocks < 1 || iters < 1 || (mode != 0 && mode != 1))
+    return fail(GS_E_ARG, "gs_bench_valu: out, blocks >= 1, iters >= 1, mode 0 or 1 required");
+  if (mode == 0)
+    hipLaunchKernelGGL(k_valu<0>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out, iters, 0.9999f, 1.0e-4f);
+  else
+    hipLaunchKernelGGL(k_valu<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out, iters, 0.9999f, 1.0e-4f);
+  GS_HIP(hipGetLastError(), "k_valu launch");
   return GS_OK;
 }
 

@@ -377,6 +377,11 @@ int gs_gen_voronoi(uint8_t* dst, int nx, int ny, int64_t pitch, int64_t i0, int6
 /* Plain streaming copy kernel (16 B per lane), used to calibrate HBM bandwidth and the
  * rocprofv3 FETCH_SIZE/WRITE_SIZE counters. No reference counterpart. */
 int gs_bench_copy(void* dst, const void* src, size_t bytes, void* stream);
+/* VALU issue-rate calibration: `blocks` workgroups of 256 lanes, each lane 8 independent
+ * float32 chains of `iters` steps; mode 0: multiply + add (16 VALU instructions per step),
+ * mode 1: hardware square root, multiply, hardware reciprocal, add (32 per step). `out`
+ * holds blocks * 256 floats. No reference counterpart. */
+int gs_bench_valu(float* out, int blocks, int iters, int mode, void* stream);
 
 #ifdef __cplusplus
 }
