@@ -98,3 +98,27 @@ def test_ensemble_float64_equals_plain_runs(gpu):
     assert all(m.dtype == np.float64 for m in mem)
     assert_same(plain, mem)
     ens.close()
+
+
+@pytest.mark.parametrize("target", [16, 1])
+def test_ensemble_with_taller_tiles_than_members(cases, monkeypatch, target):
+    """The ensemble sizes its tile height for all members together and builds its own tile
+    tables where a member's are shorter; members' own tables stay as they are, and every
+    member is still bitwise its plain run."""
+    import geoswe_amd.flatmesh as FM
+    import geoswe_amd.ensemble as EN
+    from geoswe_amd.ensemble import Ensemble
+    plain = flats(cases)
+    for p in plain:
+        p.advance(25)
+    mem = flats(cases)
+    own = [m.fm.tile_irc.copy() for m in mem]
+    monkeypatch.setattr(FM, "TILE_TARGET", target)          # few tiles wanted -> tall tiles
+    ens = Ensemble(mem, graph_batch=8)
+    assert ens.sweep_rows == EN.sweep_rows_for(sum(int(t[:, 1].sum()) for t in own))
+    if target == 1:
+        assert ens.sweep_rows == 64 and len(ens._tiles) == len(mem)   # every member rebuilt
+    assert all(np.array_equal(o, m.fm.tile_irc) for o, m in zip(own, mem))
+    ens.advance(25)
+    assert_same(plain, mem)
+    ens.close()
