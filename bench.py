@@ -31,7 +31,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL = {"dense": 28.0, "flat": 28.0}   # q read 12 + bed 4 + q write 12 (uniform n, uniform rain)
+# q read 12 + bed 4 + q write 12 (uniform n, uniform rain, no flags plane in either layout)
+BYTES_PER_CELL = {"dense": 28.0, "flat": 28.0}
 
 
 def log(msg):
@@ -115,8 +116,8 @@ def valu_calibration(reps: int = 2) -> dict:
             "transcendental_slots": slots, "workgroups": blocks, "iters": iters}
 
 
-def copy_calibration(nbytes: int, reps: int):
-    """Time gs_bench_copy (read n + write n bytes per launch); prints one JSON line."""
+def copy_calibration(nbytes: int, reps: int, quiet: bool = False) -> dict:
+    """Time gs_bench_copy (read n + write n bytes per launch); prints one JSON line unless quiet."""
     from geoswe_amd import _lib as L
     nbytes -= nbytes % 16
     a = L.DeviceArray((nbytes,), np.uint8, zero=True)
@@ -134,8 +135,11 @@ def copy_calibration(nbytes: int, reps: int):
     ms = C.c_float()
     L.call("gs_event_elapsed_ms", C.byref(ms), e0, e1)
     per = ms.value / reps
-    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ms_per_copy": per,
-                      "GBps_read_plus_write": 2 * nbytes / (per * 1e-3) / 1e9}), flush=True)
+    out = {"copy_bytes": nbytes, "reps": reps, "ms_per_copy": per,
+           "GBps_read_plus_write": 2 * nbytes / (per * 1e-3) / 1e9}
+    if not quiet:
+        print(json.dumps(out), flush=True)
+    return out
 
 
 def coastal(args):
@@ -149,7 +153,7 @@ def coastal(args):
     log(f"building the coastal case {n}x{n}")
     case = coastal_c3(n=n)
     d = case.dense(track_max_depth=False)
-    f = case.flat(track_max_depth=False)
+    f = case.flat(track_max_depth=False, sweep_rows=args.sweep_rows)
     fm = f.fm
     act = fm.active
     ij = fm.ij[act] - case.mesh.ngh
@@ -333,7 +337,7 @@ def adjoint(args):
     for _ in range(S):
         dts.append(ref.step())
     del ref
-    f = case.flat(track_max_depth=False)
+    f = case.flat(track_max_depth=False, sweep_rows=args.sweep_rows)
     fm = f.fm
     h0 = f.q_flat()[0]
     rng = np.random.default_rng(20261205)
@@ -531,6 +535,8 @@ def main():
     ap.add_argument("--nx", type=int, default=25600)
     ap.add_argument("--ny", type=int, default=25000)
     ap.add_argument("--layout", choices=["dense", "flat"], default="dense")
+    ap.add_argument("--sweep-rows", type=int, default=None,
+                    help="flat layout: rows per row-sweep tile (default: chosen from the domain size)")
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -552,7 +558,8 @@ def main():
     ap.add_argument("--n", type=int, default=16384, help="coastal box side")
     args = ap.parse_args()
     if args.copy_bytes:
-        return copy_calibration(args.copy_bytes, args.copy_reps)
+        copy_calibration(args.copy_bytes, args.copy_reps)
+        return
     if args.valu_peak:
         v = valu_calibration()
         v["simple_wave_insts_per_cu_per_ns"] = v["simple_wave_insts_per_s"] / 256 / 1e9   # 256 CUs
@@ -582,7 +589,10 @@ def main():
     bed = FbmBed(seed=C2_SEED).normalise(nx, ny)
     s = rain_on_slope_c2(nx, ny, "float32", tile_rows=args.tile_rows, bed=bed)
     if args.layout == "flat":
-        s = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=False)
+        log("packing the flat layout (every cell active)")
+        s = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=False, sweep_rows=args.sweep_rows)
+        log(f"flat layout: {s.n_active / 1e6:.1f} M active, {s.n_stored / 1e6:.1f} M stored, "
+            f"{len(s.fm.tile_e0)} sweep tiles")
     log(f"warm-up {args.warmup} steps")
     s.advance(args.warmup)
     log(f"timing {args.steps} steps")
@@ -614,8 +624,10 @@ def main():
                          "rate_over_fp32_peak": valu["wave_insts_per_s"] / cal["simple_wave_insts_per_s"],
                          "transcendental_slots": cal["transcendental_slots"],
                          "peak_source": "gs_bench_valu (bench.py --valu-peak)"})
+    # measured copy-kernel bandwidth of this device (2 GiB read + 2 GiB written per launch)
+    copy = copy_calibration(1 << 31, 10, quiet=True)
     log(f"{ms_step:.3f} ms/step wall, {kern_ms:.3f} ms/launch (events), {value / 1e9:.2f} G cell-updates/s, "
-        f"{achieved:.0f} GB/s algorithmic")
+        f"{achieved:.0f} GB/s algorithmic; copy kernel {copy['GBps_read_plus_write']:.0f} GB/s")
     h32 = None
     err64 = None
     if not args.no_fp64:
@@ -657,7 +669,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "gs::k_dense_sweep<float, 1>" if args.layout == "dense" else "gs::k_flat_sweep<float>",
-                     "kernel_ms": kern_ms, "bytes_per_cell": BYTES_PER_CELL[args.layout], "valu": valu},
+                     "kernel_ms": kern_ms, "bytes_per_cell": BYTES_PER_CELL[args.layout], "valu": valu,
+                     "copy_GBps": copy["GBps_read_plus_write"],
+                     "frac_of_copy": achieved / copy["GBps_read_plus_write"]},
         "max_depth_err_vs_fp64_m": err64,
         "cpu_baseline": cpu,
     }

@@ -205,7 +205,7 @@ class GaugeMisfit:
         a.n_stored, a.dtype, a.n_classes = N, dcode, self.n_classes
         a.dx, a.dy = float(s.mesh.dx), float(s.mesh.dy)
         a.active, a.nbr, a.bed = self._active.ptr, self._nbr.ptr, s._bed.ptr
-        a.flags, a.ntab = s._flags.ptr, s._ntab.ptr
+        a.flags, a.ntab = (s._flags.ptr if s._flags is not None else None), s._ntab.ptr
         for k in range(3):
             a.lam[k], a.w[k] = lam[k].ptr, w[k].ptr
         a.xf, a.yf, a.gz, a.bbar, a.nbar, a.npart = xf.ptr, yf.ptr, gz.ptr, bbar.ptr, nbar.ptr, npart.ptr

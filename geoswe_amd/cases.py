@@ -121,9 +121,10 @@ class CoastalCase:
         s.set_cfl_ghost_mask(~self.active)
         return s
 
-    def flat(self, track_max_depth: bool = False):
+    def flat(self, track_max_depth: bool = False, sweep_rows: int = None):
         from .compressed import CompressedSolver
-        return CompressedSolver.from_dense(self.dense(), cfl_linf=True, track_max_depth=track_max_depth)
+        return CompressedSolver.from_dense(self.dense(), cfl_linf=True, track_max_depth=track_max_depth,
+                                           sweep_rows=sweep_rows)
 
 
 def _taxicab_dilate(mask: np.ndarray, r: int) -> np.ndarray:

@@ -35,7 +35,10 @@ class CompressedSolver:
         qf = fm.pack(q_pad).astype(self.dtype)
         self._bufs = [[L.DeviceArray.from_host(np.ascontiguousarray(qf[k])) for k in range(3)] for _ in range(2)]
         self._bed = L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(b_pad).astype(self.dtype)))
-        self._flags = L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(flags_pad).astype(np.uint8)))
+        # per-cell flags (Manning class, stage, CFL skip); none at all when every byte is zero
+        # (uniform n, no stage cells, no CFL mask): the kernels then read no flags plane
+        fl = np.ascontiguousarray(fm.pack(flags_pad).astype(np.uint8))
+        self._flags = L.DeviceArray.from_host(fl) if fl.any() else None
         t = np.zeros(L.MAX_CLASSES, self.dtype)
         t[:len(ntab)] = ntab
         self._ntab = L.DeviceArray.from_host(t)
@@ -117,7 +120,8 @@ class CompressedSolver:
         for s in range(2):
             for k in range(3):
                 f.q[s][k] = self._bufs[s][k].ptr
-        f.bed, f.flags, f.ntab = self._bed.ptr, self._flags.ptr, self._ntab.ptr
+        f.bed, f.ntab = self._bed.ptr, self._ntab.ptr
+        f.flags = self._flags.ptr if self._flags is not None else None
         f.n_field = self._n_field.ptr if self._n_field is not None else None
         f.max_h = self._max_h.ptr if self._max_h is not None else None
         f.run_k0, f.run_len, f.run_i, f.run_j0, f.run_d, f.chunk_run = (r.ptr for r in self._runs)

@@ -305,34 +305,48 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
 }
 
 // Lane `lane`'s cell in one window row of a sweep tile (entry e = first stored id, stored
-// mask, active mask): the stored cells of a row are consecutive flat ids, so the lane's id
-// is the first id plus the number of stored lanes below it.
-__device__ __forceinline__ int64_t sweep_id(const GS_G uint64_t* e, int lane, bool& st) {
-  const uint64_t base = e[0], sm = e[1];
+// mask, active mask): the stored cells of a row are consecutive flat ids, so the lane's cell
+// is the row's first id (uniform across the wavefront) plus the number of stored lanes below
+// it. The byte offset of that count is returned; accesses add it to the row's start.
+template <typename T>
+__device__ __forceinline__ uint32_t sweep_lane(const GS_G uint64_t* e, int lane, int64_t& base, bool& st) {
+  const uint64_t sm = e[1];
+  base = (int64_t)e[0];
   const unsigned below =
       __builtin_amdgcn_mbcnt_hi((unsigned)(sm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sm, 0u));
   st = ((sm >> lane) & 1ull) != 0ull;
-  return (int64_t)base + (int64_t)below;
+  return below * (uint32_t)sizeof(T);
+}
+
+// Element at byte offset `off` of an array segment that starts at `row` (uniform).
+template <typename P>
+__device__ __forceinline__ const GS_G P& gat(const GS_G P* row, uint32_t off) {
+  return *(const GS_G P*)((const GS_G char*)row + off);
+}
+template <typename P>
+__device__ __forceinline__ GS_G P& gat(GS_G P* row, uint32_t off) {
+  return *(GS_G P*)((GS_G char*)row + off);
 }
 
 // State and bed of a lane's cell; lanes on unstored cells read nothing and hold zeros
 // (their faces are never used: every face an active cell needs has its stencil stored).
 template <typename T>
-__device__ __forceinline__ void sweep_q(const FlatCtx<T>& x, int64_t k, bool st, T& h, T& hu, T& hv) {
+__device__ __forceinline__ void sweep_q(const FlatCtx<T>& x, int64_t base, uint32_t off, bool st, T& h, T& hu,
+                                        T& hv) {
   h = T(0);
   hu = T(0);
   hv = T(0);
   if (st) {
-    h = x.qi0[k];
-    hu = x.qi1[k];
-    hv = x.qi2[k];
+    h = gat(x.qi0 + base, off);
+    hu = gat(x.qi1 + base, off);
+    hv = gat(x.qi2 + base, off);
   }
 }
 
 template <typename T>
-__device__ __forceinline__ T sweep_b(const FlatCtx<T>& x, int64_t k, bool st) {
+__device__ __forceinline__ T sweep_b(const FlatCtx<T>& x, int64_t base, uint32_t off, bool st) {
   T b = T(0);
-  if (st) b = x.bed[k];
+  if (st) b = gat(x.bed + base, off);
   return b;
 }
 
@@ -351,18 +365,19 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
   const int i0 = irc[0], nr = irc[1], c0 = irc[2];
   const GS_G uint64_t* E = x.ent + 3 * x.tile_e0[t];   // entry of row i0 - 2
   bool sm2, sm1, sC, sN;
-  const int64_t km2 = sweep_id(E, lane, sm2);
-  const int64_t km1 = sweep_id(E + 3, lane, sm1);
-  int64_t kC = sweep_id(E + 6, lane, sC);
-  int64_t kN = sweep_id(E + 9, lane, sN);
-  const T bm2 = sweep_b(x, km2, sm2);
-  T bm1 = sweep_b(x, km1, sm1), b0 = sweep_b(x, kC, sC), bp1 = sweep_b(x, kN, sN);
+  int64_t gm2, gm1, gC, gN;   // first flat id of rows i-2, i-1, i, i+1 (uniform)
+  const uint32_t om2 = sweep_lane<T>(E, lane, gm2, sm2);
+  const uint32_t om1 = sweep_lane<T>(E + 3, lane, gm1, sm1);
+  uint32_t oC = sweep_lane<T>(E + 6, lane, gC, sC);
+  uint32_t oN = sweep_lane<T>(E + 9, lane, gN, sN);
+  const T bm2 = sweep_b(x, gm2, om2, sm2);
+  T bm1 = sweep_b(x, gm1, om1, sm1), b0 = sweep_b(x, gC, oC, sC), bp1 = sweep_b(x, gN, oN, sN);
   T hP, huP, hvP, uP, vP;
-  sweep_q(x, km1, sm1, hP, huP, hvP);
+  sweep_q(x, gm1, om1, sm1, hP, huP, hvP);
   prims(hP, huP, hvP, hm, uP, vP);
   const T gzxP = (T(0.5) * (b0 - bm2)) * ix;
   T hC, huC, hvC, uC, vC;
-  sweep_q(x, kC, sC, hC, huC, hvC);
+  sweep_q(x, gC, oC, sC, hC, huC, hvC);
   prims(hC, huC, hvC, hm, uC, vC);
   T gzxC = (T(0.5) * (bp1 - bm1)) * ix;
   Face<T> fx;
@@ -374,10 +389,11 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
   for (int r = 0; r < nr; ++r) {
     const GS_G uint64_t* Er = E + 3 * (r + 2);   // this row's entry
     bool sNN;
-    const int64_t kNN = sweep_id(Er + 6, lane, sNN);
-    const T bp2 = sweep_b(x, kNN, sNN);
+    int64_t gNN;
+    const uint32_t oNN = sweep_lane<T>(Er + 6, lane, gNN, sNN);
+    const T bp2 = sweep_b(x, gNN, oNN, sNN);
     T hN, huN, hvN, uN, vN;
-    sweep_q(x, kN, sN, hN, huN, hvN);
+    sweep_q(x, gN, oN, sN, hN, huN, hvN);
     prims(hN, huN, hvN, hm, uN, vN);
     const T gzxN = (T(0.5) * (bp2 - b0)) * ix;
     Face<T> fn;
@@ -423,8 +439,8 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
     }
 
     if ((Er[2] >> lane) & 1ull) {   // an active cell: lanes 2..61 only
-      const int64_t k = kC;
-      const uint8_t fg = x.flags ? x.flags[k] : (uint8_t)0;
+      const uint32_t ob = oC / (uint32_t)sizeof(T);   // byte offset in a 1-byte plane
+      const uint8_t fg = x.flags ? gat(x.flags + gC, ob) : (uint8_t)0;
       T rr0 = rh;
       if (x.rain_mode == 1) {
         rr0 = rr0 + x.rain_u;
@@ -432,12 +448,12 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
         rr0 = rr0 + x.frame[(int64_t)(i0 + r - x.ngh) * x.rain_pitch + (c0 + lane - x.ngh)];
       }
       if (PRE) {   // the first three statements of update(), kept for its reverse
-        x.pre0[k] = hC + x.dt * rr0;
-        x.pre1[k] = huC + x.dt * rhu;
-        x.pre2[k] = hvC + x.dt * rhv;
+        gat(x.pre0 + gC, oC) = hC + x.dt * rr0;
+        gat(x.pre1 + gC, oC) = huC + x.dt * rhu;
+        gat(x.pre2 + gC, oC) = hvC + x.dt * rhv;
       }
       T n = T(0);
-      if (P.friction) n = x.n_field ? x.n_field[k] : x.ntab[fg & GS_FLAG_CLASS_MASK];
+      if (P.friction) n = x.n_field ? gat(x.n_field + gC, oC) : x.ntab[fg & GS_FLAG_CLASS_MASK];
       T h, hu, hv;
       update<T>(hC, huC, hvC, rr0, rhu, rhv, x.dt, n, P, h, hu, hv);
       if (fg & GS_FLAG_STAGE) {
@@ -446,10 +462,10 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
         hu = T(0);
         hv = T(0);
       }
-      if (x.max_h && h > x.max_h[k]) x.max_h[k] = h;
-      x.qo0[k] = h;
-      x.qo1[k] = hu;
-      x.qo2[k] = hv;
+      if (x.max_h && h > gat(x.max_h + gC, oC)) gat(x.max_h + gC, oC) = h;
+      gat(x.qo0 + gC, oC) = h;
+      gat(x.qo1 + gC, oC) = hu;
+      gat(x.qo2 + gC, oC) = hv;
       if (!(fg & GS_FLAG_CFL_SKIP)) {
         const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
         lam = lb_ > lam ? lb_ : lam;
@@ -460,7 +476,7 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
     hC = hN; huC = huN; hvC = hvN; uC = uN; vC = vN; gzxC = gzxN;
     bm1 = b0; b0 = bp1; bp1 = bp2;
     xFh = nFh; xFN = nFN; xFT = nFT; xsR = fn.srcR * ix; xg = fn.gate;
-    kC = kN; kN = kNN; sN = sNN;
+    gC = gN; oC = oN; gN = gNN; oN = oNN; sN = sNN;
   }
 }
 

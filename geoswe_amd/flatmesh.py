@@ -128,6 +128,7 @@ def build_flat_mesh(inside_padded: np.ndarray, ngh: int, ring: int = 2, sweep_ro
 
 WRITE_LANES = np.uint64(((1 << 62) - 1) & ~3)   # lanes 2..61 of a 64-lane sweep window
 TILE_TARGET = 8192   # tiles a domain should give at least (~1.5 resident wavefronts of an MI355X each)
+MAX_SWEEP_ROWS = 64  # tallest tile: each tile re-reads 4 stencil rows and re-evaluates one x-face row
 
 
 def _window_masks(m: np.ndarray, nwin: int) -> np.ndarray:
@@ -140,9 +141,9 @@ def _window_masks(m: np.ndarray, nwin: int) -> np.ndarray:
 
 def sweep_rows_for(busy: int) -> int:
     """Tile height for `busy` window rows with an active lane: the largest power of two
-    <= 64 that still gives about TILE_TARGET tiles."""
+    <= MAX_SWEEP_ROWS that still gives about TILE_TARGET tiles."""
     rows = 1
-    while rows < 64 and int(busy) // (2 * rows) >= TILE_TARGET:
+    while rows < MAX_SWEEP_ROWS and int(busy) // (2 * rows) >= TILE_TARGET:
         rows *= 2
     return rows
 
