@@ -566,7 +566,7 @@ def main():
         print(json.dumps(v), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build (a multi-GPU run "
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (a multi-GPU run "
               f"would split the domain into strips along y with a 2-cell halo exchange per step)", file=sys.stderr)
         sys.exit(2)
 

@@ -142,7 +142,7 @@ class Solver2D:
         ``q0`` and ``b`` are then None (used for grids too large to stage through the host).
         """
         if comm is not None and getattr(comm, "size", 1) > 1:
-            raise NotImplementedError("multi-GPU domain decomposition is not part of this build; one GPU "
+            raise NotImplementedError("multi-GPU domain decomposition is not implemented; one GPU "
                                       "runs the whole domain")
         cfg.require_production_scheme("Solver2D")
         self.mesh, self.cfg = mesh, cfg

@@ -16,7 +16,7 @@ Imports GeoSWE from /root/reference/src with GEOSWE_BACKEND=numpy and records, a
   small irregular mask after the ring-2 dilation of CompressedSWE.
 * forcing_samples.npz       -- RainfallForcing.rate_at_time / StageBoundary.stage_at_time.
 
-The reference cannot travel to the GPU box; only these data files do.
+The reference is not available where the GPU tests run; only these data files are.
 Usage: python tests/golden/make_golden.py
 """
 import os
