@@ -325,3 +325,25 @@ def test_sweep_rows_for_domain_size():
     for busy in (1, 5000, 40000, 300000, 2000000):
         r = sweep_rows_for(busy)
         assert r == 64 or busy // r >= TILE_TARGET > busy // (2 * r) or r == 1
+
+
+def test_sweep_tiles_random_masks():
+    """Property check of the sweep-tile builder over random masks, sizes and tile heights
+    (hypothesis): every active cell written exactly once, ids and stored masks exact."""
+    hyp = pytest.importorskip("hypothesis")
+    from hypothesis import given, settings, strategies as st
+    from geoswe_amd.flatmesh import build_flat_mesh
+
+    @settings(max_examples=25, deadline=None, derandomize=True)
+    @given(nx=st.integers(1, 30), ny=st.integers(1, 140), density=st.floats(0.05, 1.0),
+           rows=st.sampled_from([1, 2, 7, 64]), seed=st.integers(0, 2 ** 16))
+    def check(nx, ny, density, rows, seed):
+        rng = np.random.default_rng(seed)
+        inside = np.zeros((nx + 8, ny + 8), bool)
+        inside[4:4 + nx, 4:4 + ny] = rng.random((nx, ny)) < density
+        if not inside.any():
+            inside[4, 4] = True
+        fm = build_flat_mesh(inside, 4, sweep_rows=rows)
+        assert np.array_equal(_tile_walk(fm, inside), inside.astype(np.int64))
+
+    check()
