@@ -65,6 +65,14 @@ class GsFlat(C.Structure):
                 ("tile_irc", C.c_void_p), ("ent", C.c_void_p)]
 
 
+class GsFlatMeshHost(C.Structure):
+    _fields_ = [("n_stored", C.c_int64), ("n_active", C.c_int64), ("n_runs", C.c_int64), ("n_chunks", C.c_int64),
+                ("n_tiles", C.c_int64), ("n_entries", C.c_int64), ("sweep_rows", C.c_int32), ("pad_", C.c_int32),
+                ("ij", C.c_void_p), ("active", C.c_void_p), ("run_k0", C.c_void_p), ("run_len", C.c_void_p),
+                ("run_i", C.c_void_p), ("run_j0", C.c_void_p), ("run_d", C.c_void_p), ("chunk_run", C.c_void_p),
+                ("tile_e0", C.c_void_p), ("tile_irc", C.c_void_p), ("ent", C.c_void_p)]
+
+
 class GsAdjFlat(C.Structure):
     _fields_ = [("n_stored", C.c_int64), ("dtype", C.c_int32), ("n_classes", C.c_int32), ("dx", C.c_double),
                 ("dy", C.c_double), ("active", C.c_void_p), ("nbr", C.c_void_p), ("bed", C.c_void_p),
@@ -130,6 +138,8 @@ _SIGS = {
                              C.c_int, _P]),
     "gs_bench_copy": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "gs_bench_valu": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P]),
+    "gs_flat_mesh_build": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
+    "gs_flat_mesh_free": (C.c_int, [_P]),
     "gs_gen_voronoi": (C.c_int, [_P, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, _P, _P, _P, C.c_int, _P]),
 }
 

@@ -68,8 +68,13 @@ class FlatMesh:
         return out
 
 
-def build_flat_mesh(inside_padded: np.ndarray, ngh: int, ring: int = 2, sweep_rows: int = None) -> FlatMesh:
-    """Stored set, flat order and run table of an active mask given on the padded grid."""
+def build_flat_mesh(inside_padded: np.ndarray, ngh: int, ring: int = 2, sweep_rows: int = None,
+                    native: bool = True) -> FlatMesh:
+    """Stored set, flat order, run table and row-sweep tiles of an active mask given on the
+    padded grid. `native`: the library's host builder (gs_flat_mesh_build, one pass with
+    rolling rows); otherwise the NumPy restatement below, which produces the same arrays."""
+    if native:
+        return _build_native(inside_padded, ngh, ring, sweep_rows)
     inside = np.asarray(inside_padded, dtype=bool)
     nxp, nyp = inside.shape
     if ring < 2:
@@ -124,6 +129,49 @@ def build_flat_mesh(inside_padded: np.ndarray, ngh: int, ring: int = 2, sweep_ro
                     run_k0=run_k0.astype(np.int64), run_len=run_len, run_i=run_i, run_j0=run_j0,
                     run_d=np.ascontiguousarray(run_d), chunk_run=chunk_run,
                     tile_e0=tile_e0, tile_irc=tile_irc, ent=ent)
+
+
+def _build_native(inside_padded: np.ndarray, ngh: int, ring: int, sweep_rows) -> FlatMesh:
+    import ctypes as C
+    from . import _lib as L
+    inside = np.ascontiguousarray(np.asarray(inside_padded, dtype=bool), dtype=np.uint8)
+    if inside.ndim != 2:
+        raise ValueError("inside mask must be 2-D (padded grid)")
+    nxp, nyp = inside.shape
+    if sweep_rows is not None and sweep_rows < 1:
+        raise ValueError("rows must be >= 1")
+    out = L.GsFlatMeshHost()
+    lib = L.lib()
+    rc = lib.gs_flat_mesh_build(C.c_void_p(inside.ctypes.data), nxp, nyp, int(ring),
+                                int(sweep_rows) if sweep_rows else -1, TILE_TARGET, MAX_SWEEP_ROWS, C.byref(out))
+    if rc:
+        msg = lib.gs_last_error().decode()
+        raise (AssertionError if rc == 4 else ValueError)(msg)
+
+    def arr(ptr, ctype, shape, dtype):
+        count = int(np.prod(shape))
+        if count == 0:
+            return np.zeros(shape, dtype)
+        a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ctype)), shape=(count,))
+        return a.copy().view(dtype).reshape(shape)
+
+    try:
+        n, nr, nc, nt, ne = out.n_stored, out.n_runs, out.n_chunks, out.n_tiles, out.n_entries
+        fm = FlatMesh(nxp=nxp, nyp=nyp, ngh=ngh, n_stored=n, n_active=out.n_active,
+                      ij=arr(out.ij, C.c_int32, (n, 2), np.int32),
+                      active=arr(out.active, C.c_uint8, (n,), np.uint8).astype(bool),
+                      run_k0=arr(out.run_k0, C.c_int64, (nr,), np.int64),
+                      run_len=arr(out.run_len, C.c_int32, (nr,), np.int32),
+                      run_i=arr(out.run_i, C.c_int32, (nr,), np.int32),
+                      run_j0=arr(out.run_j0, C.c_int32, (nr,), np.int32),
+                      run_d=arr(out.run_d, C.c_int32, (nr, 4), np.int32),
+                      chunk_run=arr(out.chunk_run, C.c_int32, (nc,), np.int32),
+                      tile_e0=arr(out.tile_e0, C.c_int64, (nt,), np.int64),
+                      tile_irc=arr(out.tile_irc, C.c_int32, (nt, 3), np.int32),
+                      ent=arr(out.ent, C.c_uint64, (ne, 3), np.uint64))
+    finally:
+        lib.gs_flat_mesh_free(C.byref(out))
+    return fm
 
 
 WRITE_LANES = np.uint64(((1 << 62) - 1) & ~3)   # lanes 2..61 of a 64-lane sweep window

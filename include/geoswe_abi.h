@@ -174,6 +174,25 @@ typedef struct GsFlat {
   const uint64_t* ent;       /* [entries][3] */
 } GsFlat;
 
+/* Host arrays of a flat mesh built by gs_flat_mesh_build (allocated by the library with
+ * malloc; release them with gs_flat_mesh_free). Same meaning as the GsFlat tables. */
+typedef struct GsFlatMeshHost {
+  int64_t n_stored, n_active, n_runs, n_chunks, n_tiles, n_entries;
+  int32_t sweep_rows;        /* tile height used */
+  int32_t pad_;
+  int32_t* ij;               /* [n_stored][2] padded (i, j) of each stored cell, row-major order */
+  uint8_t* active;           /* [n_stored] 1 for an active cell, 0 for the frozen ring */
+  int64_t* run_k0;           /* [n_runs] */
+  int32_t* run_len;          /* [n_runs] */
+  int32_t* run_i;            /* [n_runs] */
+  int32_t* run_j0;           /* [n_runs] */
+  int32_t* run_d;            /* [n_runs][4] */
+  int32_t* chunk_run;        /* [n_chunks] */
+  int64_t* tile_e0;          /* [n_tiles] */
+  int32_t* tile_irc;         /* [n_tiles][3] */
+  uint64_t* ent;             /* [n_entries][3] */
+} GsFlatMeshHost;
+
 /* ---- library ---- */
 int gs_version(void);
 const char* gs_last_error(void);
@@ -319,6 +338,19 @@ int gs_flat_replay_pre(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsCl
 int gs_adj_flat_sweep(const GsAdjFlat* a, const GsPhys* phys, int n, const double* dts, const void* const* states,
                       const void* const* pre, const int64_t* gauge_ids, const void* const* coef, int n_gauges,
                       void* gz_pair, int64_t first_step, void* stream);
+
+/* ---- host mesh builder ---- */
+/*
+ * Replaces: the CompressedMesh2D / CompressedSWE construction (compressed_mesh.py:108-187,
+ * compressed_rhs.py:241-369) that CompressedSolver.from_dense (compressed_solver.py:2903-3124)
+ * runs on the host. `inside` is the (nxp, nyp) active mask of the padded grid (row-major,
+ * 0/1); the stored set is its `ring`-fold 4-neighbour dilation (ring >= 2), numbered
+ * row-major. sweep_rows > 0 fixes the tile height; -1 picks the largest power of two <=
+ * max_sweep_rows that still gives ~tile_target tiles. Host memory only, no device call.
+ */
+int gs_flat_mesh_build(const uint8_t* inside, int nxp, int nyp, int ring, int sweep_rows, int tile_target,
+                       int max_sweep_rows, GsFlatMeshHost* out);
+int gs_flat_mesh_free(GsFlatMeshHost* m);
 
 /* ---- batched ensemble of flat domains ---- */
 /*

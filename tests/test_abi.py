@@ -37,7 +37,7 @@ def test_version_and_error_string():
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
 def test_struct_layout_matches_header(tmp_path):
     structs = {"GsClock": L.GsClock, "GsPhys": L.GsPhys, "GsForcing": L.GsForcing, "GsDense": L.GsDense,
-               "GsFlat": L.GsFlat, "GsAdjFlat": L.GsAdjFlat}
+               "GsFlat": L.GsFlat, "GsAdjFlat": L.GsAdjFlat, "GsFlatMeshHost": L.GsFlatMeshHost}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HDR}"', "int main(void){"]
     for sname, cls in structs.items():
         lines.append(f'printf("{sname} %zu\\n", sizeof({sname}));')

@@ -347,3 +347,50 @@ def test_sweep_tiles_random_masks():
         assert np.array_equal(_tile_walk(fm, inside), inside.astype(np.int64))
 
     check()
+
+
+def _same_mesh(a, b):
+    for f in ("nxp", "nyp", "ngh", "n_stored", "n_active"):
+        assert getattr(a, f) == getattr(b, f), f
+    for f in ("ij", "active", "run_k0", "run_len", "run_i", "run_j0", "run_d", "chunk_run", "tile_e0", "tile_irc",
+              "ent"):
+        x, y = getattr(a, f), getattr(b, f)
+        assert x.dtype == y.dtype and x.shape == y.shape and np.array_equal(x, y), f
+
+
+def test_native_mesh_builder_equals_numpy_builder():
+    """gs_flat_mesh_build (host C++) gives every array of the NumPy builder exactly, for
+    ragged, islanded, single-cell, edge-touching and random masks, automatic and fixed tile
+    heights, and rejects the same inputs."""
+    from geoswe_amd.flatmesh import build_flat_mesh
+    rng = np.random.default_rng(77)
+    masks = []
+    m = np.zeros((70, 150), bool)
+    m[5:64, 4:140] = rng.random((59, 136)) > 0.25
+    masks.append((m, 4))
+    m = np.zeros((30, 200), bool)
+    m[4:8, 10:20] = m[4:9, 40:45] = m[12:26, 58:66] = m[20, 120:196] = True
+    masks.append((m, 4))
+    m = np.zeros((12, 12), bool)
+    m[6, 6] = True
+    masks.append((m, 4))
+    m = np.zeros((40, 67), bool)
+    m[2:38, 2:65] = True
+    masks.append((m, 2))
+    for k in range(6):
+        nx, ny = rng.integers(1, 90), rng.integers(1, 260)
+        m = np.zeros((nx + 8, ny + 8), bool)
+        m[4:4 + nx, 4:4 + ny] = rng.random((nx, ny)) < rng.uniform(0.05, 1.0)
+        m[4, 4] = True
+        masks.append((m, 4))
+    for inside, g in masks:
+        for rows in (None, 1, 3, 64):
+            _same_mesh(build_flat_mesh(inside, g, sweep_rows=rows, native=True),
+                       build_flat_mesh(inside, g, sweep_rows=rows, native=False))
+    bad = np.zeros((10, 10), bool)
+    bad[1, 5] = True                                  # closer than two cells to the edge
+    for native in (True, False):
+        with pytest.raises(ValueError):
+            build_flat_mesh(bad, 4, native=native)
+        with pytest.raises(ValueError):
+            build_flat_mesh(masks[2][0], 4, ring=1, native=native)
