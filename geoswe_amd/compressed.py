@@ -9,7 +9,9 @@ dense one at a fixed state.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import math
+import os
 import time
 from typing import Optional
 
@@ -203,13 +205,83 @@ class CompressedSolver:
             L.sync(self.stream)
             self._raise_if_bad(self._clockv())
 
-    def run(self, t_end: float, dt_max: Optional[float] = None, max_steps: int = 10 ** 9, batch: int = 64) -> int:
-        """Step to t_end (compressed_solver.py:3359-3450); dt planned on the device."""
+    # ------------------------------------------------------------------ checkpoints
+    def save_checkpoint(self, ckpt_dir: str) -> None:
+        """Write the current state to ``ckpt_dir`` in the reference's single-rank layout
+        (compressed_solver.py:1789-1880): ``ckpt_r00.npz`` holding ``t``, ``steps``, ``q0``,
+        ``q1``, ``q2`` (flat arrays in the mesh's stored order) and ``max_h`` when the
+        running maximum is tracked, published by an atomic rename, then ``ckpt_meta.json``
+        with ``t``, ``steps``, ``fidx``, ``next_frame`` and ``nranks``."""
+        os.makedirs(ckpt_dir, exist_ok=True)
+        c = self._clockv()
+        q = self.q_flat()
+        arrays = {"t": np.float64(c.t), "steps": np.int64(c.nsteps), "q0": q[0], "q1": q[1], "q2": q[2]}
+        if self._max_h is not None:
+            arrays["max_h"] = self._max_h.download()
+        tmp = os.path.join(ckpt_dir, "ckpt_r00_tmp")
+        np.savez(tmp, **arrays)
+        os.replace(tmp + ".npz", os.path.join(ckpt_dir, "ckpt_r00.npz"))
+        meta = dict(t=float(c.t), steps=int(c.nsteps), fidx=0, next_frame=float(c.t), nranks=1)
+        mt = os.path.join(ckpt_dir, "ckpt_meta.json.tmp")
+        with open(mt, "w") as f:
+            json.dump(meta, f)
+        os.replace(mt, os.path.join(ckpt_dir, "ckpt_meta.json"))
+
+    def load_checkpoint(self, ckpt_dir: str) -> None:
+        """Restore the state written by :meth:`save_checkpoint` (or by the reference's run on
+        one rank for the same mesh; compressed_solver.py:1881-1932): the same rank count, the
+        array's (t, steps) stamp equal to the meta's (else a torn checkpoint), the state in
+        the buffer that the restored step count selects. Stepping on continues bit for bit as
+        the run that wrote it."""
+        with open(os.path.join(ckpt_dir, "ckpt_meta.json")) as f:
+            meta = json.load(f)
+        if int(meta.get("nranks", 1)) != 1:
+            raise RuntimeError(f"checkpoint nranks={meta.get('nranks')} != 1; resume with the same rank count")
+        z = np.load(os.path.join(ckpt_dir, "ckpt_r00.npz"))
+        t, steps = float(meta["t"]), int(meta["steps"])
+        if "t" in z.files and (abs(float(z["t"]) - t) > 1e-9 or int(z["steps"]) != steps):
+            raise RuntimeError(f"checkpoint state is at t={float(z['t']):.3f}s/step {int(z['steps'])}, meta says "
+                               f"t={t:.3f}s/step {steps}: torn checkpoint")
+        N = self.fm.n_stored
+        for k in range(3):
+            a = z[f"q{k}"]
+            if a.shape != (N,):
+                raise ValueError(f"checkpoint q{k} has shape {a.shape}, the mesh stores {N} cells")
+        L.call("gs_clock_reset", C.c_void_p(self._clock.ptr), C.c_double(t), self.stream)
+        c = self._clockv()
+        c.nsteps = steps
+        L.call("gs_memcpy_h2d", C.c_void_p(self._clock.ptr), C.byref(c), C.c_size_t(C.sizeof(c)), self.stream)
+        par = steps & 1
+        for k in range(3):
+            self._bufs[par][k].upload(np.ascontiguousarray(z[f"q{k}"], dtype=self.dtype), self.stream)
+        if self._max_h is not None:
+            if "max_h" in z.files:
+                self._max_h.upload(np.ascontiguousarray(z["max_h"], dtype=self.dtype), self.stream)
+            else:
+                import warnings
+                warnings.warn("load_checkpoint: the checkpoint has no max_h; the running maximum covers only "
+                              "the steps after the resume", RuntimeWarning)
+        L.sync(self.stream)
+        self._lam_valid = False
+        self._target = None
+        self._any_step = self._any_step or steps > 0
+
+    def run(self, t_end: float, dt_max: Optional[float] = None, max_steps: int = 10 ** 9, batch: int = 64,
+            checkpoint_every_s: float = 0.0, ckpt_dir: Optional[str] = None, resume: bool = False) -> int:
+        """Step to t_end (compressed_solver.py:3359-3450); dt planned on the device.
+        ``checkpoint_every_s`` > 0 with ``ckpt_dir`` writes a checkpoint whenever simulated
+        time has advanced that far since the last one (checked every ``batch`` steps) and
+        at the end; ``resume`` first restores ``ckpt_dir`` when it holds one."""
         if dt_max is not None:
             check_numeric_inputs(where="CompressedSolver.run", cfl=None)
+        if (checkpoint_every_s or resume) and not ckpt_dir:
+            raise ValueError("checkpoint_every_s and resume need ckpt_dir")
+        if resume and os.path.exists(os.path.join(ckpt_dir, "ckpt_meta.json")):
+            self.load_checkpoint(ckpt_dir)
         t0 = time.perf_counter()
         self._ensure_lam()
         n0 = self.nsteps
+        next_ckpt = self.t + checkpoint_every_s if checkpoint_every_s else math.inf
         L.call("gs_clock_set_target", C.c_void_p(self._clock.ptr), C.c_double(float(t_end)),
                C.c_double(float(dt_max) if dt_max else 0.0), C.c_int64(int(max_steps)), C.byref(self._phys()),
                self._fo.s, self.stream)
@@ -219,9 +291,14 @@ class CompressedSolver:
             self._raise_if_bad(c)
             if c.status & L.ST_DONE:
                 break
+            if c.t >= next_ckpt:
+                self.save_checkpoint(ckpt_dir)
+                next_ckpt = c.t + checkpoint_every_s
             for _ in range(batch):
                 L.call("gs_flat_step", d, p, self._fo.s, clk, self.stream)
             self._any_step = True
+        if checkpoint_every_s:
+            self.save_checkpoint(ckpt_dir)          # the final checkpoint: a planned stop loses nothing
         self._target = None
         steps = self.nsteps - n0
         self.diagnostics["wallclock"] = time.perf_counter() - t0

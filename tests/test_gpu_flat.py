@@ -174,3 +174,49 @@ def test_sweep_tiles_equal_chunk_traversal(gpu, dtype):
             assert na == nb > 3 and ta == tb, (name, rows)
             assert np.array_equal(qa, qb), (name, rows)
             assert np.array_equal(ma, mb), (name, rows)
+
+
+def test_checkpoint_resume_is_bitwise(gpu, tmp_path):
+    """A run stopped by max_steps after writing checkpoints, resumed from the last one in a
+    fresh solver, ends bit for bit where the uninterrupted run ends (state, running maximum,
+    time, step count); gridded rain, Manning classes, a tidal stage row and an irregular
+    mask; a torn checkpoint (array stamp and meta disagree) is refused."""
+    import json
+    from geoswe_amd import CompressedSolver, RainfallForcing, StageBoundary, Mesh2D
+    nx, ny = 48, 70
+    q0, bed = rough_state(nx, ny, seed=31, depth=0.6, rough=1.0, vel=0.2)
+    rng = np.random.default_rng(6)
+    frames = rng.random((3, nx, ny)) * 80.0
+    n = np.choose(rng.integers(0, 8, (nx, ny)), np.linspace(0.02, 0.12, 8))
+    m = irregular_mask(nx, ny, seed=12)
+    m[0, :] = True
+    sea = np.zeros((nx, ny), bool)
+    sea[0, :] = True
+    sb = StageBoundary.from_mask(sea, Mesh2D(nx, ny, 2.0, 2.0), bed, np.array([0.0, 40.0]), np.array([0.5, 1.0]))
+
+    def mk():
+        s = dense(q0 * m, bed, friction="manning", stage_boundary=sb,
+                  rainfall_forcing=RainfallForcing(np.array([0.0, 6.0, 14.0]), frames))
+        s.set_manning(n)
+        s.set_inside_mask(m)
+        s.set_cfl_ghost_mask(~m)
+        return CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=True)
+
+    T = 30.0
+    a = mk()
+    a.run(T, batch=8)
+    d = str(tmp_path / "ckpt")
+    b = mk()
+    b.run(T, max_steps=37, batch=8, ckpt_dir=d, checkpoint_every_s=3.0)
+    assert b.nsteps == 37 and b.t < T and a.nsteps > 37
+    meta = json.load(open(tmp_path / "ckpt" / "ckpt_meta.json"))
+    assert meta["steps"] == 37 and meta["t"] == b.t and meta["nranks"] == 1
+    c = mk()
+    c.run(T, batch=8, ckpt_dir=d, resume=True)
+    assert c.nsteps == a.nsteps and c.t == a.t
+    assert np.array_equal(c.q_flat(), a.q_flat())
+    assert np.array_equal(c.max_depth(), a.max_depth())
+    meta["steps"] = 36
+    json.dump(meta, open(tmp_path / "ckpt" / "ckpt_meta.json", "w"))
+    with pytest.raises(RuntimeError, match="torn"):
+        mk().load_checkpoint(d)
