@@ -206,7 +206,7 @@ class CompressedSolver:
             self._raise_if_bad(self._clockv())
 
     # ------------------------------------------------------------------ checkpoints
-    def save_checkpoint(self, ckpt_dir: str) -> None:
+    def save_checkpoint(self, ckpt_dir: str, fidx: int = 0, next_frame: Optional[float] = None) -> None:
         """Write the current state to ``ckpt_dir`` in the reference's single-rank layout
         (compressed_solver.py:1789-1880): ``ckpt_r00.npz`` holding ``t``, ``steps``, ``q0``,
         ``q1``, ``q2`` (flat arrays in the mesh's stored order) and ``max_h`` when the
@@ -221,7 +221,8 @@ class CompressedSolver:
         tmp = os.path.join(ckpt_dir, "ckpt_r00_tmp")
         np.savez(tmp, **arrays)
         os.replace(tmp + ".npz", os.path.join(ckpt_dir, "ckpt_r00.npz"))
-        meta = dict(t=float(c.t), steps=int(c.nsteps), fidx=0, next_frame=float(c.t), nranks=1)
+        meta = dict(t=float(c.t), steps=int(c.nsteps), fidx=int(fidx),
+                    next_frame=float(c.t if next_frame is None else next_frame), nranks=1)
         mt = os.path.join(ckpt_dir, "ckpt_meta.json.tmp")
         with open(mt, "w") as f:
             json.dump(meta, f)
@@ -265,19 +266,48 @@ class CompressedSolver:
         self._lam_valid = False
         self._target = None
         self._any_step = self._any_step or steps > 0
+        return meta
+
+    def _save_frame(self, fdir: str, idx: int, t: float, fp32: bool) -> None:
+        """One depth frame as the reference writes it (compressed_solver.py:1762-1768): the
+        interior (nx, ny) depth, float16 unless `fp32`, in a compressed .npz holding `h`."""
+        h = self.depth().astype(np.float32 if fp32 else np.float16)
+        np.savez_compressed(os.path.join(fdir, f"depth_{idx:05d}_t{int(round(t)):07d}_r00.npz"), h=h)
 
     def run(self, t_end: float, dt_max: Optional[float] = None, max_steps: int = 10 ** 9, batch: int = 64,
-            checkpoint_every_s: float = 0.0, ckpt_dir: Optional[str] = None, resume: bool = False) -> int:
+            checkpoint_every_s: float = 0.0, ckpt_dir: Optional[str] = None, resume: bool = False,
+            out_dir: Optional[str] = None, frame_every_s: float = 0.0, frame_fp32: bool = False) -> int:
         """Step to t_end (compressed_solver.py:3359-3450); dt planned on the device.
         ``checkpoint_every_s`` > 0 with ``ckpt_dir`` writes a checkpoint whenever simulated
         time has advanced that far since the last one (checked every ``batch`` steps) and
-        at the end; ``resume`` first restores ``ckpt_dir`` when it holds one."""
+        at the end; ``resume`` first restores ``ckpt_dir`` when it holds one.
+        ``frame_every_s`` > 0 with ``out_dir`` writes depth frames and their manifest in the
+        reference's layout (``<out_dir>/frames_parallel/depth_<index:05d>_t<seconds:07d>_r00.npz``,
+        compressed_solver.py:1734-1768, 2395-2406, 2450-2451): a frame after the first step
+        that reaches each frame time, and one at the end; the clock is then checked after
+        every step, as the reference's loop does. Frames do not change the trajectory."""
         if dt_max is not None:
             check_numeric_inputs(where="CompressedSolver.run", cfl=None)
         if (checkpoint_every_s or resume) and not ckpt_dir:
             raise ValueError("checkpoint_every_s and resume need ckpt_dir")
+        frames_on = frame_every_s > 0
+        if frames_on and not out_dir:
+            raise ValueError("frame_every_s needs out_dir")
+        fidx, next_frame, last_ft = 0, (0.0 if frames_on else math.inf), -1.0
         if resume and os.path.exists(os.path.join(ckpt_dir, "ckpt_meta.json")):
-            self.load_checkpoint(ckpt_dir)
+            meta = self.load_checkpoint(ckpt_dir)
+            if frames_on:
+                fidx, next_frame, last_ft = int(meta["fidx"]), float(meta["next_frame"]), float(meta["t"])
+        fdir = None
+        if frames_on:
+            fdir = os.path.join(out_dir, "frames_parallel")
+            os.makedirs(fdir, exist_ok=True)
+            nx, ny = self.mesh.nx, self.mesh.ny
+            with open(os.path.join(fdir, "manifest.json"), "w") as mf:
+                json.dump({"nx_orig": nx, "ny_orig": ny, "nx_glob": nx, "ny_glob": ny, "dx": float(self.mesh.dx),
+                           "x0": 0.0, "y0": 0.0, "crs_wkt": "", "nranks": 1, "frame_every_s": float(frame_every_s),
+                           "ranks": [{"rank": 0, "cx": 0, "cy": 0, "i0": 0, "j0": 0, "nx": nx, "ny": ny}]}, mf)
+            batch = 1
         t0 = time.perf_counter()
         self._ensure_lam()
         n0 = self.nsteps
@@ -291,14 +321,23 @@ class CompressedSolver:
             self._raise_if_bad(c)
             if c.status & L.ST_DONE:
                 break
+            if frames_on and c.nsteps > n0 and c.t >= next_frame - 1e-9:
+                self._save_frame(fdir, fidx, c.t, frame_fp32)
+                fidx, next_frame, last_ft = fidx + 1, next_frame + frame_every_s, c.t
             if c.t >= next_ckpt:
-                self.save_checkpoint(ckpt_dir)
+                self.save_checkpoint(ckpt_dir, fidx, next_frame if frames_on else None)
                 next_ckpt = c.t + checkpoint_every_s
             for _ in range(batch):
                 L.call("gs_flat_step", d, p, self._fo.s, clk, self.stream)
             self._any_step = True
+        c = self._clockv()
+        if frames_on and c.nsteps > n0 and c.t >= next_frame - 1e-9:     # the last step reached a frame time
+            self._save_frame(fdir, fidx, c.t, frame_fp32)
+            fidx, next_frame, last_ft = fidx + 1, next_frame + frame_every_s, c.t
+        if frames_on and c.t > last_ft + 1e-6:
+            self._save_frame(fdir, fidx, c.t, frame_fp32)
         if checkpoint_every_s:
-            self.save_checkpoint(ckpt_dir)          # the final checkpoint: a planned stop loses nothing
+            self.save_checkpoint(ckpt_dir, fidx, next_frame if frames_on else None)   # final: a planned stop loses nothing
         self._target = None
         steps = self.nsteps - n0
         self.diagnostics["wallclock"] = time.perf_counter() - t0

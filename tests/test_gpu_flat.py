@@ -220,3 +220,47 @@ def test_checkpoint_resume_is_bitwise(gpu, tmp_path):
     json.dump(meta, open(tmp_path / "ckpt" / "ckpt_meta.json", "w"))
     with pytest.raises(RuntimeError, match="torn"):
         mk().load_checkpoint(d)
+
+
+def test_frames_in_reference_layout(gpu, tmp_path):
+    """run(out_dir=, frame_every_s=) writes the reference's frames: a float16 interior depth
+    after the first step reaching each frame time and one at the end, named by index and
+    rounded time, with a manifest; the framed run ends bit for bit where an unframed run
+    ends."""
+    import json
+    import os
+    from geoswe_amd import CompressedSolver
+    q0, bed = rough_state(40, 56, seed=37, depth=0.5, rough=0.8, vel=0.2)
+    m = irregular_mask(40, 56, seed=14)
+
+    def mk():
+        s = dense(q0 * m, bed, friction="manning", manning_n=0.03, rainfall=30.0 / 3.6e6)
+        s.set_inside_mask(m)
+        s.set_cfl_ghost_mask(~m)
+        return CompressedSolver.from_dense(s, cfl_linf=True)
+
+    T, every = 20.0, 3.0
+    a = mk()
+    a.run(T)
+    b = mk()
+    b.run(T, out_dir=str(tmp_path), frame_every_s=every)
+    assert b.nsteps == a.nsteps and b.t == a.t and np.array_equal(b.q_flat(), a.q_flat())
+    # the expected frames, one step at a time (each call plans dt exactly as the run did)
+    c = mk()
+    want, nxt, last = [], 0.0, -1.0
+    while c.t < T - 1e-12:
+        c.run(T, max_steps=1)
+        if c.t >= nxt - 1e-9:
+            want.append((len(want), c.t, c.depth().astype(np.float16)))
+            nxt, last = nxt + every, c.t
+    if c.t > last + 1e-6:
+        want.append((len(want), c.t, c.depth().astype(np.float16)))
+    assert c.nsteps == a.nsteps and np.array_equal(c.q_flat(), a.q_flat())
+    fdir = tmp_path / "frames_parallel"
+    names = sorted(f for f in os.listdir(fdir) if f.startswith("depth_"))
+    assert names == [f"depth_{i:05d}_t{int(round(t)):07d}_r00.npz" for i, t, _ in want]
+    for (i, t, h), name in zip(want, names):
+        z = np.load(fdir / name)
+        assert z["h"].dtype == np.float16 and np.array_equal(z["h"], h), name
+    man = json.load(open(fdir / "manifest.json"))
+    assert man["nranks"] == 1 and man["frame_every_s"] == every and man["ranks"][0]["nx"] == 40
