@@ -19,6 +19,7 @@ LIB_PATH = _HERE / "_lib" / "libgeoswe_hip.so"
 GS_BC = {"extrapolate": 0, "periodic": 1, "wall": 2, "fall": 3}
 GS_F32, GS_F64 = 0, 1
 FLAG_CLASS_MASK, FLAG_CFL_SKIP, FLAG_INACTIVE, FLAG_STAGE = 0x1F, 0x20, 0x40, 0x80
+FLAG_FORCE = 0x40          # flat layout: sponge / clamp cell (the dense layout's inactive bit)
 MAX_CLASSES = 32
 ST_DONE, ST_NONFINITE = 1, 2
 ADJ_MAX_BLOCKS = 8192
@@ -62,7 +63,8 @@ class GsFlat(C.Structure):
                 ("max_h", C.c_void_p), ("run_k0", C.c_void_p), ("run_len", C.c_void_p),
                 ("run_i", C.c_void_p), ("run_j0", C.c_void_p), ("run_d", C.c_void_p),
                 ("chunk_run", C.c_void_p), ("n_tiles", C.c_int64), ("tile_e0", C.c_void_p),
-                ("tile_irc", C.c_void_p), ("ent", C.c_void_p)]
+                ("tile_irc", C.c_void_p), ("ent", C.c_void_p), ("sponge", C.c_void_p), ("clamp", C.c_void_p),
+                ("n_force_ring", C.c_int64), ("force_ring", C.c_void_p)]
 
 
 class GsFlatMeshHost(C.Structure):

@@ -41,6 +41,9 @@ class CompressedSolver:
         # (uniform n, no stage cells, no CFL mask): the kernels then read no flags plane
         fl = np.ascontiguousarray(fm.pack(flags_pad).astype(np.uint8))
         self._flags = L.DeviceArray.from_host(fl) if fl.any() else None
+        self._flags_base = fl              # without the coastal-forcing bit (set_sponge / set_clamp)
+        self._sponge = self._clamp = self._force_ring = None
+        self._tables_taken = False         # an Ensemble or GaugeMisfit holds this solver's device pointers
         t = np.zeros(L.MAX_CLASSES, self.dtype)
         t[:len(ntab)] = ntab
         self._ntab = L.DeviceArray.from_host(t)
@@ -105,6 +108,79 @@ class CompressedSolver:
         self._lam_valid = False
         self._target = None
 
+    # ------------------------------------------------------------------ coastal forcings
+    def _require_unbuilt(self, who: str) -> None:
+        if self._any_step or self._tables_taken:
+            raise RuntimeError(f"CompressedSolver.{who}: the solver has already stepped or been taken into an "
+                               f"Ensemble/GaugeMisfit, which would not see the new forcing. Attach the forcings "
+                               f"before the first run, or build a fresh solver.")
+
+    def set_sponge(self, sponge):
+        """Open-boundary sponge (compressed_solver.py:3148-3156): ``{"keep": k, "amb": a}``, two
+        fields on the padded grid (the shape of the dense solver's ``q[0]``). After each update,
+        ``h <- keep*h + amb`` (two roundings) and ``hu, hv <- keep*hu, keep*hv`` on every stored
+        cell with ``keep != 1`` or ``amb != 0``, ring cells included, as the reference's band
+        kernel (compressed_solver.py:1320-1326, 1641-1645). ``None`` removes it."""
+        self._require_unbuilt("set_sponge")
+        new = None
+        if sponge is not None:
+            kf = self.fm.pack(np.asarray(sponge["keep"])).astype(self.dtype)
+            af = self.fm.pack(np.asarray(sponge["amb"])).astype(self.dtype)
+            band = (kf != 1) | (af != 0)
+            if band.any():
+                new = (L.DeviceArray.from_host(np.ascontiguousarray(np.stack([kf, af], axis=1))), band)
+        self._sponge = new
+        self._rebuild_force()
+        return self
+
+    def set_clamp(self, b):
+        """Stage clamp (compressed_solver.py:3216-3217, 3293-3298): ``{"rows", "cols", "hmax"}``,
+        cells in padded coordinates with their depth cap. After the update and the sponge, a
+        listed cell with ``h > hmax`` gets ``h = hmax`` and its momentum scaled by ``hmax/h``
+        (stage_clamp_flat, compressed_solver.py:424-440). Cells that are not stored are
+        skipped, as there. ``None`` removes it."""
+        self._require_unbuilt("set_clamp")
+        new = None
+        if b is not None:
+            fm = self.fm
+            rows = np.asarray(b["rows"], np.int64).ravel()
+            cols = np.asarray(b["cols"], np.int64).ravel()
+            hmax = np.asarray(b["hmax"], np.float64).ravel()
+            if not (len(rows) == len(cols) == len(hmax)):
+                raise ValueError(f"set_clamp: rows, cols, hmax have lengths {len(rows)}, {len(cols)}, {len(hmax)}")
+            if len(rows) and ((rows < 0).any() or (rows >= fm.nxp).any() or (cols < 0).any()
+                              or (cols >= fm.nyp).any()):
+                raise ValueError(f"set_clamp: a cell lies outside the padded grid ({fm.nxp}, {fm.nyp})")
+            # stored cells are numbered row by row, columns ascending: their keys are sorted
+            key = fm.ij[:, 0].astype(np.int64) * fm.nyp + fm.ij[:, 1]
+            want = rows * fm.nyp + cols
+            pos = np.minimum(np.searchsorted(key, want), fm.n_stored - 1)
+            hit = key[pos] == want
+            hx = np.full(fm.n_stored, np.inf, self.dtype)
+            hx[pos[hit]] = hmax[hit].astype(self.dtype)
+            cells = ~(np.isnan(hx) | (hx == np.inf))        # h > +inf or NaN never clamps
+            if cells.any():
+                new = (L.DeviceArray.from_host(hx), cells)
+        self._clamp = new
+        self._rebuild_force()
+        return self
+
+    def _rebuild_force(self) -> None:
+        force = np.zeros(self.fm.n_stored, bool)
+        for f in (self._sponge, self._clamp):
+            if f is not None:
+                force |= f[1]
+        fl = self._flags_base
+        if force.any():
+            fl = fl | (force.astype(np.uint8) * np.uint8(L.FLAG_FORCE))
+        self._flags = L.DeviceArray.from_host(np.ascontiguousarray(fl)) if fl.any() else None
+        ring = np.nonzero(force & ~self.fm.active)[0].astype(np.int64)
+        self._force_ring = L.DeviceArray.from_host(ring) if len(ring) else None
+
+    @property
+    def has_coastal_forcing(self) -> bool:
+        return self._sponge is not None or self._clamp is not None
+
     # ------------------------------------------------------------------ structs
     def _phys(self) -> L.GsPhys:
         cfg = self.cfg
@@ -131,6 +207,10 @@ class CompressedSolver:
             raise ValueError(f"traversal must be 'tiles' or 'chunks', got {self.traversal!r}")
         f.n_tiles = len(fm.tile_e0) if self.traversal == "tiles" else 0
         f.tile_e0, f.tile_irc, f.ent = (t.ptr for t in self._tiles)
+        f.sponge = self._sponge[0].ptr if self._sponge is not None else None
+        f.clamp = self._clamp[0].ptr if self._clamp is not None else None
+        f.n_force_ring = self._force_ring.shape[0] if self._force_ring is not None else 0
+        f.force_ring = self._force_ring.ptr if self._force_ring is not None else None
         return f
 
     # ------------------------------------------------------------------ state
@@ -232,7 +312,7 @@ class CompressedSolver:
         """Restore the state written by :meth:`save_checkpoint` (or by the reference's run on
         one rank for the same mesh; compressed_solver.py:1881-1932): the same rank count, the
         array's (t, steps) stamp equal to the meta's (else a torn checkpoint), the state in
-        the buffer that the restored step count selects. Stepping on continues bit for bit as
+        both step buffers. Stepping on continues bit for bit as
         the run that wrote it."""
         with open(os.path.join(ckpt_dir, "ckpt_meta.json")) as f:
             meta = json.load(f)
@@ -252,9 +332,12 @@ class CompressedSolver:
         c = self._clockv()
         c.nsteps = steps
         L.call("gs_memcpy_h2d", C.c_void_p(self._clock.ptr), C.byref(c), C.c_size_t(C.sizeof(c)), self.stream)
-        par = steps & 1
+        # both buffers: the ring cells a step does not write then hold the checkpoint's values
+        # whichever buffer a later step reads (the active cells of the other one are rewritten)
         for k in range(3):
-            self._bufs[par][k].upload(np.ascontiguousarray(z[f"q{k}"], dtype=self.dtype), self.stream)
+            a = np.ascontiguousarray(z[f"q{k}"], dtype=self.dtype)
+            for par in (0, 1):
+                self._bufs[par][k].upload(a, self.stream)
         if self._max_h is not None:
             if "max_h" in z.files:
                 self._max_h.upload(np.ascontiguousarray(z["max_h"], dtype=self.dtype), self.stream)

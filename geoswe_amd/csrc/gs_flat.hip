@@ -55,6 +55,10 @@ struct FlatK {
   const int64_t* tile_e0;
   const int32_t* tile_irc;
   const uint64_t* ent;
+  const T* sponge;  // nullable [N][2] (keep, amb) of GS_FLAG_FORCE cells
+  const T* clamp;   // nullable [N] h_max of GS_FLAG_FORCE cells
+  int64_t n_fring;  // GS_FLAG_FORCE cells of the ring (stored, not active)
+  const int64_t* fring;
 };
 
 
@@ -92,6 +96,8 @@ struct FlatCtx {
   const GS_G int64_t* tile_e0;
   const GS_G int32_t* tile_irc;
   const GS_G uint64_t* ent;
+  const GS_G T* sponge;
+  const GS_G T* clamp;
   const GS_G T* frame;   // gridded rain frame of this step (rain_mode 2)
   const T* ntab;         // Manning n per class (LDS copy or the member's table)
   int64_t n, rain_pitch;
@@ -178,6 +184,8 @@ __device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k,
   x.tile_e0 = gp(a.tile_e0);
   x.tile_irc = gp(a.tile_irc);
   x.ent = gp(a.ent);
+  x.sponge = gp(a.sponge);
+  x.clamp = gp(a.clamp);
   x.frame = k.frame;
   x.ntab = a.ntab;
   x.n = a.n;
@@ -194,6 +202,24 @@ __device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k,
 // One wavefront pass over flat ids chunk*60-2 .. chunk*60+61: the residual (MODE 0) or the
 // fused update (MODE 1) of the 60 middle ids, folding their CFL speed bits into lam.
 // Every lane of the wavefront must call it (cross-lane shuffles).
+// Sponge band then stage clamp on a GS_FLAG_FORCE cell, after the update, the stage and the
+// running max (sponge_band, compressed_solver.py:1320-1326: h*keep then + amb, two roundings;
+// stage_clamp_flat, 424-440). keep = 1, amb = 0, h_max = +inf leave a cell unchanged.
+template <typename T>
+__device__ __forceinline__ void coastal_force(T kp, T am, T hx, bool sp, bool cl, T& h, T& hu, T& hv) {
+  if (sp) {
+    h = h * kp + am;
+    hu = hu * kp;
+    hv = hv * kp;
+  }
+  if (cl && h > hx) {
+    const T ratio = hx / h;
+    h = hx;
+    hu = hu * ratio;
+    hv = hv * ratio;
+  }
+}
+
 template <typename T, int MODE>
 __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, int lane, uint64_t& lam) {
   const FlatCtx<T>& a = x;
@@ -295,6 +321,11 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
     hv = T(0);
   }
   if (a.max_h && h > a.max_h[k]) a.max_h[k] = h;
+  if (fg & GS_FLAG_FORCE) {
+    const bool sp = x.sponge != nullptr, cl = x.clamp != nullptr;
+    coastal_force<T>(sp ? x.sponge[2 * k] : T(1), sp ? x.sponge[2 * k + 1] : T(0), cl ? x.clamp[k] : T(0), sp, cl,
+                     h, hu, hv);
+  }
   x.qo0[k] = h;
   x.qo1[k] = hu;
   x.qo2[k] = hv;
@@ -463,6 +494,11 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
         hv = T(0);
       }
       if (x.max_h && h > gat(x.max_h + gC, oC)) gat(x.max_h + gC, oC) = h;
+      if (fg & GS_FLAG_FORCE) {
+        const bool sp = x.sponge != nullptr, cl = x.clamp != nullptr;
+        coastal_force<T>(sp ? gat(x.sponge + 2 * gC, 2 * oC) : T(1), sp ? gat(x.sponge + 2 * gC + 1, 2 * oC) : T(0),
+                         cl ? gat(x.clamp + gC, oC) : T(0), sp, cl, h, hu, hv);
+      }
       gat(x.qo0 + gC, oC) = h;
       gat(x.qo1 + gC, oC) = hu;
       gat(x.qo2 + gC, oC) = hv;
@@ -531,6 +567,29 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(sizeof(
   grid_reduce_and_plan(lam, a.clock, a.plan, true);
 }
 
+// The forcings on the ring cells of the band (stored, not active): out = force(in), one
+// thread per listed id. Launched just before the step kernel on the same stream, it reads
+// the clock the step will read (its input buffer, its done state); the step kernel only
+// reads these cells, from the input buffer, so the two never touch the same word.
+template <typename T>
+__global__ __launch_bounds__(kFBlock) void k_flat_ring_force(FlatK<T> a) {
+  const FlatClk<T> ck = flat_clock<T, 1>(a);
+  if (!ck.live) return;
+  FlatCtx<T> x;
+  flat_ctx<T, 1>(a, ck, x);
+  const bool sp = x.sponge != nullptr, cl = x.clamp != nullptr;
+  const GS_G int64_t* ids = gp(a.fring);
+  for (int64_t i = (int64_t)blockIdx.x * kFBlock + threadIdx.x; i < a.n_fring; i += (int64_t)gridDim.x * kFBlock) {
+    const int64_t k = ids[i];
+    T h = x.qi0[k], hu = x.qi1[k], hv = x.qi2[k];
+    coastal_force<T>(sp ? x.sponge[2 * k] : T(1), sp ? x.sponge[2 * k + 1] : T(0), cl ? x.clamp[k] : T(0), sp, cl,
+                     h, hu, hv);
+    x.qo0[k] = h;
+    x.qo1[k] = hu;
+    x.qo2[k] = hv;
+  }
+}
+
 // CFL wave speed of the current flat state over active cells, then plan dt.
 template <typename T>
 __global__ __launch_bounds__(kFBlock) void k_flat_cfl(FlatK<T> a) {
@@ -561,6 +620,10 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
     return fail(GS_E_ARG, "run tables are NULL");
   if (f->n_tiles < 0) return fail(GS_E_ARG, "n_tiles < 0");
   if (f->n_tiles > 0 && (!f->tile_e0 || !f->tile_irc || !f->ent)) return fail(GS_E_ARG, "tile tables are NULL");
+  if (f->n_force_ring < 0 || f->n_force_ring > f->n_stored) return fail(GS_E_RANGE, "n_force_ring out of range");
+  if (f->n_force_ring > 0 && !f->force_ring) return fail(GS_E_ARG, "force_ring is NULL");
+  if (f->n_force_ring > 0 && !f->sponge && !f->clamp)
+    return fail(GS_E_ARG, "force_ring lists cells but neither sponge nor clamp is set");
   if (!f->bed) return fail(GS_E_ARG, "bed is NULL");
   for (int s = 0; s < 2; ++s)
     for (int k = 0; k < 3; ++k)
@@ -595,6 +658,10 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   a.tile_e0 = f->tile_e0;
   a.tile_irc = f->tile_irc;
   a.ent = f->ent;
+  a.sponge = (const T*)f->sponge;
+  a.clamp = (const T*)f->clamp;
+  a.n_fring = f->n_force_ring;
+  a.fring = f->force_ring;
   a.P = make_phys<T>(*p, f->dx, f->dy);
   a.rain_mode = fo.n_rain > 0 ? (fo.rain_gridded ? 2 : 1) : 0;
   a.rain_frames = (const T*)fo.rain_frames;
@@ -638,6 +705,11 @@ static unsigned sweep_grid(int64_t ntiles) {
 // The fused step: row-sweep tiles when the descriptor has them, else chunks of the run table.
 template <typename T>
 static int flat_step_launch(const FlatK<T>& a, hipStream_t s) {
+  if (a.n_fring > 0) {
+    const int64_t nb = (a.n_fring + kFBlock - 1) / kFBlock;
+    hipLaunchKernelGGL((k_flat_ring_force<T>), dim3((unsigned)(nb < 1024 ? nb : 1024)), dim3(kFBlock), 0, s, a);
+    GS_HIP(hipGetLastError(), "k_flat_ring_force launch");
+  }
   if (a.n_tiles > 0 && a.pre[0]) {
     hipLaunchKernelGGL((k_flat_sweep<T, true>), dim3(sweep_grid<T, true>(a.n_tiles)), dim3(kFBlock), 0, s, a);
     GS_HIP(hipGetLastError(), "k_flat_sweep(pre) launch");
@@ -903,6 +975,9 @@ static int ens_build_views(GsEns* e, const GsFlat* members, const GsPhys* phys, 
     }
     if (views[m].n_runs > 0 && views[m].n_tiles < 1)
       return fail(GS_E_ARG, "member %d has active cells but no row-sweep tiles", m);
+    if (views[m].n_fring > 0)
+      return fail(GS_E_ARG, "member %d forces ring cells (sponge or clamp outside the active set); the ensemble "
+                  "step updates active cells only", m);
     c0[m + 1] = c0[m] + views[m].n_tiles;
   }
   e->ntiles = c0[e->n];

@@ -34,7 +34,12 @@ class Ensemble:
                 raise TypeError(f"member {k} is {type(m).__name__}, not CompressedSolver")
             if m.dtype != dt0:
                 raise ValueError(f"member {k} has dtype {m.dtype}, member 0 {dt0}")
+            if m._force_ring is not None:
+                raise NotImplementedError(f"member {k} forces ring cells (sponge or clamp outside its active set); "
+                                          f"the ensemble step updates active cells only")
         self.members = members
+        for m in members:
+            m._tables_taken = True
         self.graph_batch = int(graph_batch)
         for m in members:                      # first dt exactly as a plain open run plans it
             m._ensure_lam()

@@ -51,7 +51,9 @@ extern "C" {
 /* per-cell flags byte (dense "flags" plane, flat "flags" array) */
 #define GS_FLAG_CLASS_MASK 0x1F  /* Manning class id (0..31) into the ntab array */
 #define GS_FLAG_CFL_SKIP 0x20    /* left out of the CFL wave-speed maximum (set_cfl_ghost_mask) */
-#define GS_FLAG_INACTIVE 0x40    /* outside the inside mask: zero residual, no rain */
+#define GS_FLAG_INACTIVE 0x40    /* dense layout: outside the inside mask, zero residual, no rain */
+#define GS_FLAG_FORCE 0x40       /* flat layout (which stores no inactive cell to update): a sponge
+                                    or clamp cell, which reads GsFlat.sponge / GsFlat.clamp */
 #define GS_FLAG_STAGE 0x80       /* stage-boundary cell: h = max(eta(t)-b, 0), hu = hv = 0 */
 
 /* GsClock.status bits */
@@ -172,6 +174,17 @@ typedef struct GsFlat {
   const int64_t* tile_e0;    /* [n_tiles] entry index of row i0 - 2 */
   const int32_t* tile_irc;   /* [n_tiles][3] i0, nr, padded column of lane 0 */
   const uint64_t* ent;       /* [entries][3] */
+  /* Coastal forcings on GS_FLAG_FORCE cells, applied after the update, the stage and the
+   * running max (compressed_solver.py:1320-1326 sponge_band, 424-440 stage_clamp_flat):
+   * h <- keep*h + amb, hu, hv <- keep*hu, keep*hv; then where h > h_max:
+   * r = h_max/h, h <- h_max, hu, hv <- r*hu, r*hv. */
+  const void* sponge;        /* nullable [N][2] (keep, amb), state dtype */
+  const void* clamp;         /* nullable [N] h_max, state dtype (+inf: no clamp) */
+  /* GS_FLAG_FORCE cells of the ring (stored, not active): the step writes
+   * force(state) for them, as the reference's gathered kernels touch every stored cell
+   * of the band; the residual and the CFL never see them. */
+  int64_t n_force_ring;
+  const int64_t* force_ring; /* [n_force_ring] flat ids */
 } GsFlat;
 
 /* Host arrays of a flat mesh built by gs_flat_mesh_build (allocated by the library with

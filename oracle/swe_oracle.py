@@ -21,6 +21,7 @@ Sources followed (paths relative to the reference checkout):
 * dt rule and rain step limit ......... solver.py:2657-2666, 2975-3027
 * ghost-cell fill ..................... bc.py:50-98; ghost bed solver.py:1466-1484
 * rain lookup / stage boundary ........ forcing.py:170-216, 315-377
+* open-boundary sponge, stage clamp .... compressed_solver.py:1320-1326, 424-440
 
 Documented departures (shared with the HIP kernels, which implement the same rule):
 
@@ -493,6 +494,38 @@ def stage_at_time(time_s, stage_m, t: float) -> float:
 # a dense run (Solver2D.run/step, solver.py:2695-3003)
 # ---------------------------------------------------------------------------
 
+def coastal_force(q_int: np.ndarray, sponge=None, clamp=None) -> None:
+    """Open-boundary sponge then stage clamp, in place on a (3, nx, ny) state.
+
+    Sponge (compressed_solver.py:1320-1326): ``h <- h*keep + amb`` as a multiply then an
+    add (two roundings), ``hu, hv <- hu*keep, hv*keep``, on the band (``keep != 1`` or
+    ``amb != 0``). Clamp (compressed_solver.py:424-440): where ``h > h_max``,
+    ``r = h_max/h``, ``h <- h_max``, ``hu, hv <- hu*r, hv*r``; clamp cells are those with an
+    ``h_max`` that is neither +inf nor NaN. A cell named by either list gets both (the
+    identity sponge ``h*1 + 0`` included), as the fused step applies them.
+    """
+    dt = q_int.dtype
+    cells = np.zeros(q_int.shape[1:], bool)
+    if sponge is not None:
+        kp = np.asarray(sponge[0]).astype(dt)
+        am = np.asarray(sponge[1]).astype(dt)
+        cells |= (kp != 1) | (am != 0)
+    if clamp is not None:
+        hx = np.asarray(clamp).astype(dt)
+        cells |= ~(np.isnan(hx) | (hx == np.inf))
+    h, hu, hv = q_int[0], q_int[1], q_int[2]
+    if sponge is not None:
+        h[cells] = h[cells] * kp[cells] + am[cells]
+        hu[cells] = hu[cells] * kp[cells]
+        hv[cells] = hv[cells] * kp[cells]
+    if clamp is not None:
+        over = cells & (h > hx)
+        r = hx[over] / h[over]
+        h[over] = hx[over]
+        hu[over] = hu[over] * r
+        hv[over] = hv[over] * r
+
+
 @dataclass
 class DenseCase:
     """Inputs of a dense run on an unpadded ``(nx, ny)`` grid."""
@@ -509,11 +542,13 @@ class DenseCase:
     rain: object = None                # None, scalar m/s, or (time_s, rate_mm_h[, frames]) tuple
     active: Optional[np.ndarray] = None   # (nx, ny) bool inside mask
     stage: Optional[dict] = None       # {"cells": (M,2) unpadded ij, "time_s", "stage_m"}
+    sponge: Optional[tuple] = None     # (keep, amb), (nx, ny) each
+    clamp: Optional[np.ndarray] = None  # (nx, ny) h_max, +inf where none
 
 
 class DenseOracle:
     """Dense time stepping in the reference order: BC fill -> residual -> update -> t += dt
-    -> stage -> (next) CFL. Works in float32 or float64."""
+    -> stage -> sponge -> clamp -> (next) CFL. Works in float32 or float64."""
 
     def __init__(self, case: DenseCase, ph: Physics, dtype=np.float32):
         self.case, self.ph = case, ph
@@ -580,6 +615,8 @@ class DenseOracle:
             self.q[0, ii + g, jj + g] = np.maximum(eta - bed, self.dtype.type(0))
             self.q[1, ii + g, jj + g] = 0
             self.q[2, ii + g, jj + g] = 0
+        if c.sponge is not None or c.clamp is not None:
+            coastal_force(self.q[I], c.sponge, c.clamp)
         self.nsteps += 1
 
     def next_dt(self, t_end=None, dt_max=None) -> float:

@@ -1,0 +1,208 @@
+"""Open-boundary sponge and stage clamp fused into the flat step, against the oracle.
+
+The flat solver's state on every stored interior cell (active and ring), its running
+maximum and its step sequence must equal the dense oracle's with the same forcings bit for
+bit, for both traversals and both precisions; the ensemble carries active-cell forcings
+and refuses ring forcings, and the adjoint refuses both.
+"""
+import numpy as np
+import pytest
+
+from oracle import swe_oracle as O
+
+from test_gpu_flat import dense, irregular_mask, rough_state
+
+pytestmark = pytest.mark.gpu
+
+NX, NY = 56, 70
+T_END = 2.0
+
+
+def _forcings(m, band_rows=14, h_amb=0.55, alpha=0.3):
+    """Sponge over the first `band_rows` rows (active, ring and outer cells alike) ramping to
+    `alpha` at row 0; clamp on a patch of active cells, a few ring cells and cells that are not
+    stored (skipped by the flat layout, inert in the oracle)."""
+    nx, ny = m.shape
+    ramp = np.clip((band_rows - np.arange(nx)) / band_rows, 0.0, 1.0)[:, None] * np.ones((1, ny))
+    keep = 1.0 - alpha * ramp
+    amb = alpha * ramp * h_amb
+    rows, cols = np.meshgrid(np.arange(20, 32), np.arange(25, 45, 2), indexing="ij")
+    rows, cols = list(rows.ravel()), list(cols.ravel())
+    ring = np.argwhere(~m & (np.roll(m, 1, 0) | np.roll(m, -1, 0)))[:6]
+    outer = np.argwhere(~m)[:4]
+    for i, j in list(ring) + list(outer):
+        rows.append(int(i))
+        cols.append(int(j))
+    rows, cols = np.array(rows), np.array(cols)
+    hmax = np.full(len(rows), 0.42)
+    grid = np.full((nx, ny), np.inf)
+    grid[rows, cols] = hmax
+    return keep, amb, rows, cols, hmax, grid
+
+
+def _padded(a, g, fill):
+    out = np.full((a.shape[0] + 2 * g, a.shape[1] + 2 * g), fill)
+    out[g:-g, g:-g] = a
+    return out
+
+
+def _physics(cfg):
+    """The oracle's scalars from the solver's Config (h_min is 1e-6 only for float32)."""
+    return O.Physics(h_min=cfg.h_min, h_cfl=cfg.h_cfl(), cfl=cfg.cfl, friction=True,
+                     vcap=cfg.friction_velocity_cap_ms)
+
+
+def _flat(q0, bed, m, dtype, trav, keep, amb, rows, cols, hmax, sweep_rows=None):
+    from geoswe_amd import CompressedSolver
+    s = dense(q0, bed, friction="manning", manning_n=0.03, rainfall=30.0 / 3.6e6, dtype=dtype)
+    s.set_inside_mask(m)
+    s.set_cfl_ghost_mask(~m)
+    c = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=True, sweep_rows=sweep_rows)
+    g = s.mesh.ngh
+    c.set_sponge({"keep": _padded(keep, g, 1.0), "amb": _padded(amb, g, 0.0)})
+    c.set_clamp({"rows": rows + g, "cols": cols + g, "hmax": hmax})
+    c.traversal = trav
+    return c, g
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("trav", ["tiles", "chunks"])
+def test_sponge_and_clamp_match_oracle(gpu, dtype, trav, monkeypatch):
+    q0, bed = rough_state(NX, NY, seed=31, depth=0.7, rough=1.0, vel=0.3)
+    m = irregular_mask(NX, NY, seed=9)
+    q0 = q0 * m                                  # inactive cells at rest
+    keep, amb, rows, cols, hmax, grid = _forcings(m)
+    c, g = _flat(q0, bed, m, dtype, trav, keep, amb, rows, cols, hmax, sweep_rows=5)
+    assert c._force_ring is not None and c._force_ring.shape[0] > 0       # ring cells are forced
+    n = c.run(T_END)
+
+    # oracle, recording the running maximum where the step records it (before the forcings)
+    mx = np.zeros((NX, NY), np.dtype(dtype))
+    orig = O.coastal_force
+
+    def recording(q, sponge=None, clamp=None):
+        np.maximum(mx, q[0], out=mx)
+        orig(q, sponge, clamp)
+
+    monkeypatch.setattr(O, "coastal_force", recording)
+    case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=bed, dx=2.0, dy=2.0, bc_x="fall", bc_y="fall",
+                       manning=0.03, rain=30.0 / 3.6e6, active=m, sponge=(keep, amb), clamp=grid)
+    o = O.DenseOracle(case, _physics(c.cfg), np.dtype(dtype))
+    o.cfl_skip = ~m
+    o.run(T_END)
+    print(dtype, trav, n, flush=True)
+    assert o.nsteps == c.nsteps == n > 5 and o.t == c.t
+
+    ij = c.fm.ij - g
+    inner = (ij[:, 0] >= 0) & (ij[:, 0] < NX) & (ij[:, 1] >= 0) & (ij[:, 1] < NY)
+    qf = c.q_flat()
+    want = o.interior[:, ij[inner, 0], ij[inner, 1]]
+    assert np.array_equal(qf[:, inner], want)
+    act = c.fm.active
+    ia = ij[act]
+    assert np.array_equal(c.max_depth()[ia[:, 0], ia[:, 1]], mx[ia[:, 0], ia[:, 1]])
+    # the forcings acted: band rows relaxed, clamp patch capped, ring cells of the band moved
+    band_ring = inner & ~act & (ij[:, 0] < 5) & (keep[np.clip(ij[:, 0], 0, NX - 1), 0] < 1)
+    if band_ring.any():
+        assert not np.array_equal(qf[0, band_ring], c.fm.pack(np.pad(q0[0], g))[band_ring].astype(dtype))
+    patch = act & (ij[:, 0] >= 20) & (ij[:, 0] < 32)
+    assert np.all(qf[0, patch & np.isin(ij[:, 1], np.arange(25, 45, 2))] <= 0.42)
+
+
+def test_forcing_with_identity_fields_is_the_plain_step(gpu):
+    """keep = 1, amb = 0 and no finite cap flag no cell: the run is bitwise the plain run."""
+    from geoswe_amd import CompressedSolver
+    q0, bed = rough_state(NX, NY, seed=32, depth=0.7, rough=1.0, vel=0.3)
+    runs = []
+    for forced in (False, True):
+        s = dense(q0, bed, friction="manning", manning_n=0.03)
+        c = CompressedSolver.from_dense(s, cfl_linf=True)
+        if forced:
+            g = s.mesh.ngh
+            c.set_sponge({"keep": np.ones((NX + 2 * g, NY + 2 * g)), "amb": np.zeros((NX + 2 * g, NY + 2 * g))})
+            c.set_clamp({"rows": np.array([g + 3]), "cols": np.array([g + 3]), "hmax": np.array([np.inf])})
+            assert not c.has_coastal_forcing and c._flags is None
+        c.run(1.5)
+        runs.append((c.q_flat(), c.nsteps))
+    assert runs[0][1] == runs[1][1] and np.array_equal(runs[0][0], runs[1][0])
+
+
+def test_ensemble_carries_active_forcings_and_refuses_ring_ones(gpu):
+    from geoswe_amd.ensemble import Ensemble
+    q0, bed = rough_state(NX, NY, seed=33, depth=0.7, rough=1.0, vel=0.3)
+    m = np.zeros((NX, NY), bool)
+    m[2:NX - 2, 2:NY - 2] = True
+    q0 = q0 * m
+    keep, amb, rows, cols, hmax, _ = _forcings(m)
+    keep[~m] = 1.0                               # band on active cells only
+    amb[~m] = 0.0
+    inside = m[rows, cols]
+    mk = lambda: _flat(q0, bed, m, "float32", "tiles", keep, amb, rows[inside], cols[inside], hmax[inside])[0]
+    plain = [mk(), mk()]
+    assert plain[0]._force_ring is None and plain[0].has_coastal_forcing
+    for p in plain:
+        p.advance(20)
+    mem = [mk(), mk()]
+    ens = Ensemble(mem, graph_batch=8)
+    ens.advance(20)
+    for p, e in zip(plain, mem):
+        assert p.nsteps == e.nsteps == 20 and p.t == e.t
+        assert np.array_equal(p.q_flat(), e.q_flat())
+    ens.close()
+    ring = _flat(q0, bed, m, "float32", "tiles", *_forcings(m)[:5])[0]
+    assert ring._force_ring is not None
+    with pytest.raises(NotImplementedError, match="ring cells"):
+        Ensemble([ring])
+
+
+def test_forcing_setters_refuse_after_stepping_and_adjoint_refuses(gpu):
+    from geoswe_amd import CompressedSolver
+    from geoswe_amd.adjoint import GaugeMisfit
+    q0, bed = rough_state(40, 40, seed=34)
+    s = dense(q0, bed, friction="manning", manning_n=0.03)
+    c = CompressedSolver.from_dense(s, cfl_linf=True)
+    g = s.mesh.ngh
+    c.set_clamp({"rows": np.array([g + 5]), "cols": np.array([g + 5]), "hmax": np.array([0.1])})
+    with pytest.raises(NotImplementedError, match="no reverse"):
+        GaugeMisfit(c, np.array([int(np.nonzero(c.fm.active)[0][0])]), np.full(3, 0.01))
+    with pytest.raises(ValueError, match="outside the padded grid"):
+        c.set_clamp({"rows": np.array([-1]), "cols": np.array([0]), "hmax": np.array([0.1])})
+    c.step()
+    with pytest.raises(RuntimeError, match="already stepped"):
+        c.set_sponge(None)
+
+
+def test_resume_with_forced_ring_cells_is_bitwise(gpu, tmp_path):
+    """A run with forced ring cells stopped at an odd step count and resumed in a fresh
+    solver ends bitwise where the uninterrupted run ends; an unforced ring cell whose value in
+    the checkpoint differs from the solver's own (a checkpoint written by a code whose ring
+    moved) keeps that value on every later step, whichever buffer the step reads."""
+    q0, bed = rough_state(NX, NY, seed=35, depth=0.7, rough=1.0, vel=0.3)
+    m = irregular_mask(NX, NY, seed=9)
+    q0 = q0 * m
+    f = _forcings(m)[:5]
+    mk = lambda: _flat(q0, bed, m, "float32", "tiles", *f)[0]
+    T = 6.0
+    a = mk()
+    a.run(T, batch=8)
+    d = tmp_path / "ckpt"
+    b = mk()
+    b.run(T, max_steps=13, batch=4, ckpt_dir=str(d), checkpoint_every_s=1.0)
+    assert b.nsteps == 13 and a.nsteps > 13
+    c = mk()
+    c.run(T, batch=8, ckpt_dir=str(d), resume=True)
+    assert c.nsteps == a.nsteps and c.t == a.t
+    assert np.array_equal(c.q_flat(), a.q_flat()) and np.array_equal(c.max_depth(), a.max_depth())
+
+    forced = np.zeros(c.n_stored, bool)
+    forced[c._force_ring.download()] = True
+    k = int(np.nonzero(~c.fm.active & ~forced)[0][0])        # an unforced ring cell
+    z = dict(np.load(d / "ckpt_r00.npz"))
+    z["q0"] = z["q0"].copy()
+    z["q0"][k] = np.float32(0.3125)
+    np.savez(d / "ckpt_r00.npz", **z)
+    e = mk()
+    e.load_checkpoint(str(d))
+    for _ in range(2):
+        e.step()
+        assert e.q_flat()[0, k] == np.float32(0.3125)

@@ -1,0 +1,140 @@
+"""Pins the oracle's open-boundary sponge and stage clamp (coastal_force) on the CPU.
+
+The reference's tests hold no vectors for these two forcings, so each check is one the
+mathematics fixes: the identity fields, the full-absorption limit, the geometric series of
+a repeated relaxation, the clamp's invariants (depth cap, velocity kept, untouched cells
+bitwise), the sponge-before-clamp order, and the place of both in the step (after the
+update and the stage).
+"""
+import numpy as np
+import pytest
+
+from oracle import swe_oracle as O
+
+
+def _state(nx=9, ny=7, seed=3, dtype=np.float32):
+    rng = np.random.default_rng(seed)
+    h = (rng.random((nx, ny)) * 2.0).astype(dtype)
+    hu = ((rng.random((nx, ny)) - 0.5) * h).astype(dtype)
+    hv = ((rng.random((nx, ny)) - 0.5) * h).astype(dtype)
+    return np.stack([h, hu, hv])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_identity_fields_change_nothing(dtype):
+    q = _state(dtype=dtype)
+    q0 = q.copy()
+    ones, zeros = np.ones(q.shape[1:]), np.zeros(q.shape[1:])
+    O.coastal_force(q, sponge=(ones, zeros), clamp=np.full(q.shape[1:], np.inf))
+    assert np.array_equal(q, q0) and q.dtype == dtype
+
+
+def test_full_absorption_sets_ambient_depth_at_rest():
+    """keep = 0: the cell becomes the ambient state exactly (h = amb, no momentum)."""
+    q = _state()
+    keep = np.ones(q.shape[1:])
+    amb = np.zeros(q.shape[1:])
+    keep[:2] = 0.0
+    amb[:2] = 0.75
+    q_out = q.copy()
+    O.coastal_force(q_out, sponge=(keep, amb))
+    assert np.all(q_out[0, :2] == np.float32(0.75))
+    assert np.all(q_out[1:, :2] == 0)
+    assert np.array_equal(q_out[:, 2:], q[:, 2:])          # off the band: untouched
+
+
+def test_repeated_relaxation_is_the_geometric_series():
+    """n applications of h <- k h + a from h0 give k^n h0 + a (1 - k^n) / (1 - k), momentum k^n."""
+    q = _state(dtype=np.float64)
+    h0, hu0 = q[0].copy(), q[1].copy()
+    k, a, n = 0.9, 0.02, 25
+    keep = np.full(q.shape[1:], k)
+    amb = np.full(q.shape[1:], a)
+    for _ in range(n):
+        O.coastal_force(q, sponge=(keep, amb))
+    want = k ** n * h0 + a * (1 - k ** n) / (1 - k)
+    assert np.allclose(q[0], want, rtol=1e-13, atol=1e-15)
+    assert np.allclose(q[1], k ** n * hu0, rtol=1e-13, atol=1e-15)
+
+
+def test_sponge_rounds_multiply_then_add():
+    """float32: h*keep is rounded before amb is added (no fused multiply-add)."""
+    h = np.float32(1.0000001)
+    kp = np.float32(0.99999994)
+    am = np.float32(-0.99999994)
+    two = np.float32(h * kp) + am
+    fused = np.float32(np.float64(h) * np.float64(kp) + np.float64(am))
+    assert two != fused                                      # the case tells the two apart
+    q = np.array([[[h]], [[0.0]], [[0.0]]], np.float32)
+    O.coastal_force(q, sponge=(np.array([[kp]]), np.array([[am]])))
+    assert q[0, 0, 0] == two
+
+
+def test_clamp_caps_depth_and_keeps_velocity():
+    q = _state(dtype=np.float64)
+    q0 = q.copy()
+    hmax = np.full(q.shape[1:], np.inf)
+    hmax[:, :3] = 0.8
+    O.coastal_force(q, clamp=hmax)
+    over = (q0[0] > 0.8) & (hmax < np.inf)
+    assert over.any() and (~over & (hmax < np.inf)).any()
+    assert np.all(q[0][over] == 0.8)
+    for m in (1, 2):    # velocity hu/h unchanged to rounding
+        assert np.allclose(q[m][over] / q[0][over], q0[m][over] / q0[0][over], rtol=1e-14)
+    assert np.array_equal(q[:, ~over], q0[:, ~over])         # at or under the cap: bitwise
+
+
+def test_sponge_comes_before_clamp():
+    """A sponge that lifts a cell over its cap leaves it exactly at the cap; the other order
+    would leave h_max*keep + amb."""
+    q = np.array([[[0.5]], [[0.1]], [[0.0]]], np.float32)
+    O.coastal_force(q, sponge=(np.array([[0.5]]), np.array([[1.0]])), clamp=np.array([[1.1]]))
+    assert q[0, 0, 0] == np.float32(1.1)
+    assert q[1, 0, 0] == np.float32(np.float32(0.1 * np.float32(0.5)) * np.float32(np.float32(1.1) / np.float32(1.25)))
+
+
+def test_clamp_only_cell_gets_the_identity_sponge():
+    """A clamp cell outside the band still takes h*1 + 0, which turns a -0 depth into +0."""
+    q = np.array([[[-0.0, -0.0]], [[0.0, 0.0]], [[0.0, 0.0]]], np.float32)
+    O.coastal_force(q, sponge=(np.ones((1, 2)), np.zeros((1, 2))), clamp=np.array([[1.0, np.inf]]))
+    assert not np.signbit(q[0, 0, 0]) and np.signbit(q[0, 0, 1])
+
+
+def _case(**kw):
+    nx, ny = 12, 10
+    rng = np.random.default_rng(5)
+    bed = rng.random((nx, ny)) * 0.3
+    h = np.maximum(0.6 - bed, 0.0)
+    return O.DenseCase(h=h, hu=0.05 * h, hv=np.zeros_like(h), bed=bed, dx=2.0, dy=2.0, bc_x="fall",
+                       bc_y="fall", **kw)
+
+
+def test_step_applies_forcing_after_update_and_stage():
+    """Full absorption on rows 0-1 holds them at the ambient depth after every step, also on
+    a stage cell (the stage is written first, then the sponge)."""
+    nx, ny = 12, 10
+    keep = np.ones((nx, ny))
+    amb = np.zeros((nx, ny))
+    keep[:2] = 0.0
+    amb[:2] = 0.4
+    stage = {"cells": np.array([[0, 3], [5, 5]]), "time_s": np.array([0.0, 10.0]), "stage_m": np.array([1.0, 1.0])}
+    o = O.DenseOracle(_case(sponge=(keep, amb), stage=stage), O.Physics(friction=False), np.float32)
+    plain = O.DenseOracle(_case(stage=stage), O.Physics(friction=False), np.float32)
+    for _ in range(4):
+        dt = o.next_dt()
+        o.step(dt)
+        plain.step(dt)
+        q = o.interior
+        assert np.all(q[0, :2] == np.float32(0.4)) and np.all(q[1:, :2] == 0)
+    assert q[0, 5, 5] == plain.interior[0, 5, 5]     # a stage cell off the band keeps the stage depth
+    assert not np.array_equal(q[:, 2:4], plain.interior[:, 2:4])   # the band's neighbours feel it
+
+
+def test_step_clamp_caps_the_updated_depth():
+    nx, ny = 12, 10
+    hmax = np.full((nx, ny), np.inf)
+    hmax[4:8, 4:8] = 0.45
+    o = O.DenseOracle(_case(clamp=hmax), O.Physics(friction=False), np.float32)
+    for _ in range(3):
+        o.step(o.next_dt())
+        assert np.all(o.interior[0, 4:8, 4:8] <= np.float32(0.45))

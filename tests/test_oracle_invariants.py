@@ -385,8 +385,11 @@ def test_native_mesh_builder_equals_numpy_builder():
         masks.append((m, 4))
     for inside, g in masks:
         for rows in (None, 1, 3, 64):
-            _same_mesh(build_flat_mesh(inside, g, sweep_rows=rows, native=True),
-                       build_flat_mesh(inside, g, sweep_rows=rows, native=False))
+            fm = build_flat_mesh(inside, g, sweep_rows=rows, native=True)
+            _same_mesh(fm, build_flat_mesh(inside, g, sweep_rows=rows, native=False))
+            # stored cells in row-major order (the (row, col) -> id lookup of set_clamp)
+            key = fm.ij[:, 0].astype(np.int64) * fm.nyp + fm.ij[:, 1]
+            assert np.all(np.diff(key) > 0)
     bad = np.zeros((10, 10), bool)
     bad[1, 5] = True                                  # closer than two cells to the edge
     for native in (True, False):
