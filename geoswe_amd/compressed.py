@@ -42,7 +42,7 @@ class CompressedSolver:
         fl = np.ascontiguousarray(fm.pack(flags_pad).astype(np.uint8))
         self._flags = L.DeviceArray.from_host(fl) if fl.any() else None
         self._flags_base = fl              # without the coastal-forcing bit (set_sponge / set_clamp)
-        self._sponge = self._clamp = self._force_ring = None
+        self._sponge = self._clamp = self._force_ring = self._force_ring_row = self._ring = None
         self._tables_taken = False         # an Ensemble or GaugeMisfit holds this solver's device pointers
         t = np.zeros(L.MAX_CLASSES, self.dtype)
         t[:len(ntab)] = ntab
@@ -151,19 +151,71 @@ class CompressedSolver:
             if len(rows) and ((rows < 0).any() or (rows >= fm.nxp).any() or (cols < 0).any()
                               or (cols >= fm.nyp).any()):
                 raise ValueError(f"set_clamp: a cell lies outside the padded grid ({fm.nxp}, {fm.nyp})")
-            # stored cells are numbered row by row, columns ascending: their keys are sorted
-            key = fm.ij[:, 0].astype(np.int64) * fm.nyp + fm.ij[:, 1]
-            want = rows * fm.nyp + cols
-            pos = np.minimum(np.searchsorted(key, want), fm.n_stored - 1)
-            hit = key[pos] == want
+            ids = self._flat_ids(rows, cols)
+            hit = ids >= 0
             hx = np.full(fm.n_stored, np.inf, self.dtype)
-            hx[pos[hit]] = hmax[hit].astype(self.dtype)
+            hx[ids[hit]] = hmax[hit].astype(self.dtype)
             cells = ~(np.isnan(hx) | (hx == np.inf))        # h > +inf or NaN never clamps
             if cells.any():
                 new = (L.DeviceArray.from_host(hx), cells)
         self._clamp = new
         self._rebuild_force()
         return self
+
+    def set_ring(self, ring):
+        """IDW gauge ring (compressed_solver.py:3146-3147, 3265-3268): ``{"i", "j", "bed", "wg",
+        "NG", "n", "t_common", "stage_all"}`` as the run driver builds it: ring cells in padded
+        coordinates with their bed, inverse-distance weights ``wg`` (``n`` rows of ``NG``, flat
+        or 2-D), the common time grid and the ``(NG, len(t_common))`` gauge stages. Each step,
+        after the sponge and before the clamp, a ring cell gets still water at the weighted
+        stage of the step's start time, ``h = max(0, sum_g w_g s_g - bed)``, ``hu = hv = 0``
+        (ring_bc_flat, 287-307; stage interpolation, 2338-2348). Ring cells must lie outside
+        the active set (the reference's ring is the stored ring around it); cells that are not
+        stored are skipped, as there. ``None`` (or ``n == 0``) removes it."""
+        self._require_unbuilt("set_ring")
+        new = None
+        if ring and int(ring.get("n", 0)) > 0:
+            fm = self.fm
+            n, ng = int(ring["n"]), int(ring["NG"])
+            ri = np.asarray(ring["i"], np.int64).ravel()
+            rj = np.asarray(ring["j"], np.int64).ravel()
+            bed = np.asarray(ring["bed"], np.float64).ravel()
+            w = np.asarray(ring["wg"], np.float64).reshape(-1)
+            tc = np.ascontiguousarray(ring["t_common"], np.float64).ravel()
+            st = np.asarray(ring["stage_all"], np.float64)
+            if ng < 1 or len(ri) != n or len(rj) != n or len(bed) != n or w.size != n * ng:
+                raise ValueError(f"set_ring: n={n}, NG={ng} but i, j, bed, wg have {len(ri)}, {len(rj)}, "
+                                 f"{len(bed)}, {w.size} entries")
+            if len(tc) < 2 or st.shape != (ng, len(tc)):
+                raise ValueError(f"set_ring: stage_all has shape {st.shape}, expected ({ng}, {len(tc)}) with "
+                                 f"at least two times")
+            if n and ((ri < 0).any() or (ri >= fm.nxp).any() or (rj < 0).any() or (rj >= fm.nyp).any()):
+                raise ValueError(f"set_ring: a ring cell lies outside the padded grid ({fm.nxp}, {fm.nyp})")
+            ids = self._flat_ids(ri, rj)
+            rows = np.nonzero(ids >= 0)[0]
+            ids = ids[rows]
+            if fm.active[ids].any():
+                raise ValueError("set_ring: ring cells must lie outside the active set")
+            if len(np.unique(ids)) != len(ids):
+                raise ValueError("set_ring: a cell is listed twice")
+            dt = self.dtype
+            new = None if not len(ids) else dict(ids=ids, rows=rows.astype(np.int32), ng=ng, nt=len(tc),
+                       bed=L.DeviceArray.from_host(np.ascontiguousarray(bed.astype(dt))),
+                       w=L.DeviceArray.from_host(np.ascontiguousarray(w.astype(dt))),
+                       t=L.DeviceArray.from_host(tc),
+                       stage=L.DeviceArray.from_host(np.ascontiguousarray(st.astype(dt))))
+        self._ring = new
+        self._rebuild_force()
+        return self
+
+    def _flat_ids(self, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
+        """Flat id of each padded (row, col), -1 where the cell is not stored."""
+        fm = self.fm
+        # stored cells are numbered row by row, columns ascending: their keys are sorted
+        key = fm.ij[:, 0].astype(np.int64) * fm.nyp + fm.ij[:, 1]
+        want = rows * fm.nyp + cols
+        pos = np.minimum(np.searchsorted(key, want), fm.n_stored - 1)
+        return np.where(key[pos] == want, pos, -1)
 
     def _rebuild_force(self) -> None:
         force = np.zeros(self.fm.n_stored, bool)
@@ -174,12 +226,20 @@ class CompressedSolver:
         if force.any():
             fl = fl | (force.astype(np.uint8) * np.uint8(L.FLAG_FORCE))
         self._flags = L.DeviceArray.from_host(np.ascontiguousarray(fl)) if fl.any() else None
-        ring = np.nonzero(force & ~self.fm.active)[0].astype(np.int64)
-        self._force_ring = L.DeviceArray.from_host(ring) if len(ring) else None
+        listed = force & ~self.fm.active
+        if self._ring is not None:
+            listed[self._ring["ids"]] = True
+        ids = np.nonzero(listed)[0].astype(np.int64)
+        self._force_ring = L.DeviceArray.from_host(ids) if len(ids) else None
+        self._force_ring_row = None
+        if self._ring is not None and len(ids):
+            row = np.full(self.fm.n_stored, -1, np.int32)
+            row[self._ring["ids"]] = self._ring["rows"]
+            self._force_ring_row = L.DeviceArray.from_host(np.ascontiguousarray(row[ids]))
 
     @property
     def has_coastal_forcing(self) -> bool:
-        return self._sponge is not None or self._clamp is not None
+        return self._sponge is not None or self._clamp is not None or self._ring is not None
 
     # ------------------------------------------------------------------ structs
     def _phys(self) -> L.GsPhys:
@@ -211,6 +271,11 @@ class CompressedSolver:
         f.clamp = self._clamp[0].ptr if self._clamp is not None else None
         f.n_force_ring = self._force_ring.shape[0] if self._force_ring is not None else 0
         f.force_ring = self._force_ring.ptr if self._force_ring is not None else None
+        r = self._ring
+        if r is not None and self._force_ring_row is not None:
+            f.force_ring_row = self._force_ring_row.ptr
+            f.ring_ng, f.ring_nt = r["ng"], r["nt"]
+            f.ring_bed, f.ring_w, f.ring_t, f.ring_stage = r["bed"].ptr, r["w"].ptr, r["t"].ptr, r["stage"].ptr
         return f
 
     # ------------------------------------------------------------------ state

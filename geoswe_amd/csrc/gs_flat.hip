@@ -57,8 +57,14 @@ struct FlatK {
   const uint64_t* ent;
   const T* sponge;  // nullable [N][2] (keep, amb) of GS_FLAG_FORCE cells
   const T* clamp;   // nullable [N] h_max of GS_FLAG_FORCE cells
-  int64_t n_fring;  // GS_FLAG_FORCE cells of the ring (stored, not active)
+  int64_t n_fring;  // forced cells of the ring (stored, not active): sponge, clamp, gauge ring
   const int64_t* fring;
+  const int32_t* fring_row;  // nullable: gauge-ring row of each entry, -1 for none
+  int ring_ng, ring_nt;
+  const T* ring_bed;
+  const T* ring_w;
+  const double* ring_t;
+  const T* ring_stage;
 };
 
 
@@ -206,18 +212,24 @@ __device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k,
 // running max (sponge_band, compressed_solver.py:1320-1326: h*keep then + amb, two roundings;
 // stage_clamp_flat, 424-440). keep = 1, amb = 0, h_max = +inf leave a cell unchanged.
 template <typename T>
-__device__ __forceinline__ void coastal_force(T kp, T am, T hx, bool sp, bool cl, T& h, T& hu, T& hv) {
-  if (sp) {
-    h = h * kp + am;
-    hu = hu * kp;
-    hv = hv * kp;
-  }
-  if (cl && h > hx) {
+__device__ __forceinline__ void sponge_cell(T kp, T am, T& h, T& hu, T& hv) {
+  h = h * kp + am;
+  hu = hu * kp;
+  hv = hv * kp;
+}
+template <typename T>
+__device__ __forceinline__ void clamp_cell(T hx, T& h, T& hu, T& hv) {
+  if (h > hx) {
     const T ratio = hx / h;
     h = hx;
     hu = hu * ratio;
     hv = hv * ratio;
   }
+}
+template <typename T>
+__device__ __forceinline__ void coastal_force(T kp, T am, T hx, bool sp, bool cl, T& h, T& hu, T& hv) {
+  if (sp) sponge_cell<T>(kp, am, h, hu, hv);
+  if (cl) clamp_cell<T>(hx, h, hu, hv);
 }
 
 template <typename T, int MODE>
@@ -579,11 +591,47 @@ __global__ __launch_bounds__(kFBlock) void k_flat_ring_force(FlatK<T> a) {
   flat_ctx<T, 1>(a, ck, x);
   const bool sp = x.sponge != nullptr, cl = x.clamp != nullptr;
   const GS_G int64_t* ids = gp(a.fring);
+  const GS_G int32_t* rows = gp(a.fring_row);
+  // gauge stage weights at the step's start time (uniform: every thread takes the same path)
+  T wa = T(0), wb = T(0);
+  int ti = 0;
+  if (rows) {
+    const GS_G double* tc = gp(a.ring_t);
+    const double t = gp(a.clock)->t;
+    int lo = 0, hi = a.ring_nt;   // bisect_left: the number of samples below t
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (tc[mid] < t) lo = mid + 1; else hi = mid;
+    }
+    ti = lo - 1;
+    if (ti > a.ring_nt - 2) ti = a.ring_nt - 2;
+    if (ti < 0) ti = 0;
+    double wt = (t - tc[ti]) / (tc[ti + 1] - tc[ti] + 1.0e-12);
+    wt = wt < 1.0 ? wt : 1.0;
+    wt = wt > 0.0 ? wt : 0.0;
+    wa = (T)(1.0 - wt);
+    wb = (T)wt;
+  }
   for (int64_t i = (int64_t)blockIdx.x * kFBlock + threadIdx.x; i < a.n_fring; i += (int64_t)gridDim.x * kFBlock) {
     const int64_t k = ids[i];
     T h = x.qi0[k], hu = x.qi1[k], hv = x.qi2[k];
-    coastal_force<T>(sp ? x.sponge[2 * k] : T(1), sp ? x.sponge[2 * k + 1] : T(0), cl ? x.clamp[k] : T(0), sp, cl,
-                     h, hu, hv);
+    const bool fc = x.flags != nullptr && (x.flags[k] & GS_FLAG_FORCE);
+    if (fc && sp) sponge_cell<T>(x.sponge[2 * k], x.sponge[2 * k + 1], h, hu, hv);
+    const int r = rows ? rows[i] : -1;
+    if (r >= 0) {
+      const GS_G T* w = gp(a.ring_w) + (int64_t)r * a.ring_ng;
+      const GS_G T* S = gp(a.ring_stage);
+      T eta = T(0);
+      for (int g = 0; g < a.ring_ng; ++g) {
+        const T sg = S[(int64_t)g * a.ring_nt + ti] * wa + S[(int64_t)g * a.ring_nt + ti + 1] * wb;
+        eta = eta + w[g] * sg;
+      }
+      const T d = eta - gp(a.ring_bed)[r];
+      h = d > T(0) ? d : T(0);   // fmaxf(0, d): NaN gives 0
+      hu = T(0);
+      hv = T(0);
+    }
+    if (fc && cl) clamp_cell<T>(x.clamp[k], h, hu, hv);
     x.qo0[k] = h;
     x.qo1[k] = hu;
     x.qo2[k] = hv;
@@ -622,8 +670,12 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   if (f->n_tiles > 0 && (!f->tile_e0 || !f->tile_irc || !f->ent)) return fail(GS_E_ARG, "tile tables are NULL");
   if (f->n_force_ring < 0 || f->n_force_ring > f->n_stored) return fail(GS_E_RANGE, "n_force_ring out of range");
   if (f->n_force_ring > 0 && !f->force_ring) return fail(GS_E_ARG, "force_ring is NULL");
-  if (f->n_force_ring > 0 && !f->sponge && !f->clamp)
-    return fail(GS_E_ARG, "force_ring lists cells but neither sponge nor clamp is set");
+  if (f->n_force_ring > 0 && !f->sponge && !f->clamp && !f->force_ring_row)
+    return fail(GS_E_ARG, "force_ring lists cells but no sponge, clamp or gauge ring is set");
+  if (f->force_ring_row && (f->ring_ng < 1 || f->ring_nt < 2 || !f->ring_bed || !f->ring_w || !f->ring_t ||
+                            !f->ring_stage))
+    return fail(GS_E_ARG, "gauge ring: needs >= 1 gauge, >= 2 time samples and its bed, weight, time and stage "
+                "arrays");
   if (!f->bed) return fail(GS_E_ARG, "bed is NULL");
   for (int s = 0; s < 2; ++s)
     for (int k = 0; k < 3; ++k)
@@ -662,6 +714,13 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   a.clamp = (const T*)f->clamp;
   a.n_fring = f->n_force_ring;
   a.fring = f->force_ring;
+  a.fring_row = f->force_ring_row;
+  a.ring_ng = f->ring_ng;
+  a.ring_nt = f->ring_nt;
+  a.ring_bed = (const T*)f->ring_bed;
+  a.ring_w = (const T*)f->ring_w;
+  a.ring_t = f->ring_t;
+  a.ring_stage = (const T*)f->ring_stage;
   a.P = make_phys<T>(*p, f->dx, f->dy);
   a.rain_mode = fo.n_rain > 0 ? (fo.rain_gridded ? 2 : 1) : 0;
   a.rain_frames = (const T*)fo.rain_frames;
@@ -976,7 +1035,7 @@ static int ens_build_views(GsEns* e, const GsFlat* members, const GsPhys* phys, 
     if (views[m].n_runs > 0 && views[m].n_tiles < 1)
       return fail(GS_E_ARG, "member %d has active cells but no row-sweep tiles", m);
     if (views[m].n_fring > 0)
-      return fail(GS_E_ARG, "member %d forces ring cells (sponge or clamp outside the active set); the ensemble "
+      return fail(GS_E_ARG, "member %d forces ring cells (gauge ring, or sponge or clamp outside the active set); the ensemble "
                   "step updates active cells only", m);
     c0[m + 1] = c0[m] + views[m].n_tiles;
   }

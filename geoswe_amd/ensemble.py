@@ -35,7 +35,7 @@ class Ensemble:
             if m.dtype != dt0:
                 raise ValueError(f"member {k} has dtype {m.dtype}, member 0 {dt0}")
             if m._force_ring is not None:
-                raise NotImplementedError(f"member {k} forces ring cells (sponge or clamp outside its active set); "
+                raise NotImplementedError(f"member {k} forces ring cells (gauge ring, or sponge or clamp outside its active set); "
                                           f"the ensemble step updates active cells only")
         self.members = members
         for m in members:

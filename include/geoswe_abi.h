@@ -185,6 +185,18 @@ typedef struct GsFlat {
    * of the band; the residual and the CFL never see them. */
   int64_t n_force_ring;
   const int64_t* force_ring; /* [n_force_ring] flat ids */
+  /* IDW gauge ring on ring cells (ring_bc_flat, compressed_solver.py:287-307, with the
+   * per-step stage interpolation of 2338-2348), between the sponge and the clamp, at the
+   * step's start time t: ti = clamp(#{t_common < t} - 1, 0, nt - 2),
+   * wt = clamp((t - t_common[ti]) / (t_common[ti+1] - t_common[ti] + 1e-12), 0, 1) in
+   * float64, s_g = S[g][ti]*(T)(1 - wt) + S[g][ti+1]*(T)wt, eta = sum over g ascending of
+   * w[r][g]*s_g from 0, then h = max(0, eta - ring_bed[r]), hu = hv = 0. */
+  const int32_t* force_ring_row; /* nullable [n_force_ring]: ring row r of each entry, -1 for none */
+  int32_t ring_ng, ring_nt;  /* gauges, samples of the common time grid (>= 2) */
+  const void* ring_bed;      /* [rows] state dtype */
+  const void* ring_w;        /* [rows][ring_ng] state dtype */
+  const double* ring_t;      /* [ring_nt] ascending */
+  const void* ring_stage;    /* [ring_ng][ring_nt] state dtype */
 } GsFlat;
 
 /* Host arrays of a flat mesh built by gs_flat_mesh_build (allocated by the library with

@@ -22,6 +22,7 @@ Sources followed (paths relative to the reference checkout):
 * ghost-cell fill ..................... bc.py:50-98; ghost bed solver.py:1466-1484
 * rain lookup / stage boundary ........ forcing.py:170-216, 315-377
 * open-boundary sponge, stage clamp .... compressed_solver.py:1320-1326, 424-440
+* IDW gauge ring ....................... compressed_solver.py:287-307, 2338-2348
 
 Documented departures (shared with the HIP kernels, which implement the same rule):
 
@@ -494,15 +495,41 @@ def stage_at_time(time_s, stage_m, t: float) -> float:
 # a dense run (Solver2D.run/step, solver.py:2695-3003)
 # ---------------------------------------------------------------------------
 
-def coastal_force(q_int: np.ndarray, sponge=None, clamp=None) -> None:
-    """Open-boundary sponge then stage clamp, in place on a (3, nx, ny) state.
+def ring_depth(ring: dict, t: float, dtype) -> np.ndarray:
+    """Still-water depth of each gauge-ring cell at time ``t`` (the step's start time).
+
+    Stage interpolation (compressed_solver.py:2338-2348): ``ti = searchsorted(t_common, t) -
+    1`` clamped to ``[0, nt - 2]``; ``wt = (t - t_common[ti]) / (t_common[ti+1] - t_common[ti]
+    + 1e-12)`` in float64, clamped to ``[0, 1]``; per gauge ``s = S[ti]*(1 - wt) + S[ti+1]*wt``
+    in the state dtype. IDW depth (ring_bc_flat, 287-307): ``eta = sum_g w_g s_g`` from 0 in
+    gauge order, ``h = max(0, eta - bed)`` (``fmax``: a NaN gives 0).
+    """
+    dt = np.dtype(dtype)
+    tc = np.asarray(ring["t_common"], np.float64)
+    ti = int(np.searchsorted(tc, t) - 1)
+    ti = max(0, min(len(tc) - 2, ti))
+    wt = float((t - tc[ti]) / (tc[ti + 1] - tc[ti] + 1e-12))
+    wt = min(1.0, max(0.0, wt))
+    S = np.asarray(ring["stage_all"]).astype(dt)
+    s = S[:, ti] * dt.type(1.0 - wt) + S[:, ti + 1] * dt.type(wt)
+    w = np.asarray(ring["w"]).astype(dt).reshape(len(ring["cells"]), S.shape[0])
+    eta = np.zeros(w.shape[0], dt)
+    for g in range(w.shape[1]):
+        eta = eta + w[:, g] * s[g]
+    return np.fmax(dt.type(0), eta - np.asarray(ring["bed"]).astype(dt))
+
+
+def coastal_force(q_int: np.ndarray, sponge=None, clamp=None, ring=None, t: float = 0.0) -> None:
+    """Open-boundary sponge, gauge ring, stage clamp, in place on a (3, nx, ny) state.
 
     Sponge (compressed_solver.py:1320-1326): ``h <- h*keep + amb`` as a multiply then an
     add (two roundings), ``hu, hv <- hu*keep, hv*keep``, on the band (``keep != 1`` or
     ``amb != 0``). Clamp (compressed_solver.py:424-440): where ``h > h_max``,
     ``r = h_max/h``, ``h <- h_max``, ``hu, hv <- hu*r, hv*r``; clamp cells are those with an
     ``h_max`` that is neither +inf nor NaN. A cell named by either list gets both (the
-    identity sponge ``h*1 + 0`` included), as the fused step applies them.
+    identity sponge ``h*1 + 0`` included), as the fused step applies them. ``ring``
+    (``{"cells": (n, 2) ij, "bed", "w": (n, NG), "t_common", "stage_all": (NG, nt)}``) sets
+    its cells to still water at :func:`ring_depth` of time ``t`` between the two.
     """
     dt = q_int.dtype
     cells = np.zeros(q_int.shape[1:], bool)
@@ -518,6 +545,11 @@ def coastal_force(q_int: np.ndarray, sponge=None, clamp=None) -> None:
         h[cells] = h[cells] * kp[cells] + am[cells]
         hu[cells] = hu[cells] * kp[cells]
         hv[cells] = hv[cells] * kp[cells]
+    if ring is not None:
+        ii, jj = np.asarray(ring["cells"])[:, 0], np.asarray(ring["cells"])[:, 1]
+        h[ii, jj] = ring_depth(ring, t, dt)
+        hu[ii, jj] = 0
+        hv[ii, jj] = 0
     if clamp is not None:
         over = cells & (h > hx)
         r = hx[over] / h[over]
@@ -544,11 +576,13 @@ class DenseCase:
     stage: Optional[dict] = None       # {"cells": (M,2) unpadded ij, "time_s", "stage_m"}
     sponge: Optional[tuple] = None     # (keep, amb), (nx, ny) each
     clamp: Optional[np.ndarray] = None  # (nx, ny) h_max, +inf where none
+    ring: Optional[dict] = None         # gauge ring: see coastal_force
 
 
 class DenseOracle:
     """Dense time stepping in the reference order: BC fill -> residual -> update -> t += dt
-    -> stage -> sponge -> clamp -> (next) CFL. Works in float32 or float64."""
+    -> stage -> sponge -> gauge ring (at the step's start time) -> clamp -> (next) CFL.
+    Works in float32 or float64."""
 
     def __init__(self, case: DenseCase, ph: Physics, dtype=np.float32):
         self.case, self.ph = case, ph
@@ -606,6 +640,7 @@ class DenseOracle:
             n = np.asarray(n, dtype=np.float32)
         act = None if c.active is None else np.asarray(c.active)
         self.q[I] = update(self.q[I], r[I], dt, ph, n_cell=n, rain=rain, active=act)
+        t0 = self.t
         self.t += dt
         if c.stage is not None:
             eta = self.dtype.type(stage_at_time(c.stage["time_s"], c.stage["stage_m"], self.t))
@@ -615,8 +650,8 @@ class DenseOracle:
             self.q[0, ii + g, jj + g] = np.maximum(eta - bed, self.dtype.type(0))
             self.q[1, ii + g, jj + g] = 0
             self.q[2, ii + g, jj + g] = 0
-        if c.sponge is not None or c.clamp is not None:
-            coastal_force(self.q[I], c.sponge, c.clamp)
+        if c.sponge is not None or c.clamp is not None or c.ring is not None:
+            coastal_force(self.q[I], c.sponge, c.clamp, c.ring, t0)
         self.nsteps += 1
 
     def next_dt(self, t_end=None, dt_max=None) -> float:

@@ -80,9 +80,9 @@ def test_sponge_and_clamp_match_oracle(gpu, dtype, trav, monkeypatch):
     mx = np.zeros((NX, NY), np.dtype(dtype))
     orig = O.coastal_force
 
-    def recording(q, sponge=None, clamp=None):
+    def recording(q, *args):
         np.maximum(mx, q[0], out=mx)
-        orig(q, sponge, clamp)
+        orig(q, *args)
 
     monkeypatch.setattr(O, "coastal_force", recording)
     case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=bed, dx=2.0, dy=2.0, bc_x="fall", bc_y="fall",
@@ -206,3 +206,45 @@ def test_resume_with_forced_ring_cells_is_bitwise(gpu, tmp_path):
     for _ in range(2):
         e.step()
         assert e.q_flat()[0, k] == np.float32(0.3125)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("trav", ["tiles", "chunks"])
+def test_gauge_ring_matches_oracle(gpu, dtype, trav):
+    """The IDW gauge ring on every first-layer ring cell, with the sponge and the clamp: the
+    flat state of every stored interior cell, the step count and the time equal the oracle's
+    bit for bit; the common time grid is fine enough that the run crosses several samples."""
+    q0, bed = rough_state(NX, NY, seed=36, depth=0.7, rough=1.0, vel=0.3)
+    m = irregular_mask(NX, NY, seed=9)
+    q0 = q0 * m
+    keep, amb, rows, cols, hmax, grid = _forcings(m)
+    c, g = _flat(q0, bed, m, dtype, trav, keep, amb, rows, cols, hmax)
+    near = np.zeros_like(m)
+    for ax in (0, 1):
+        for sh in (1, -1):
+            near |= np.roll(m, sh, ax)
+    cells = np.argwhere(~m & near)                      # the ring's first layer, inside the box
+    rng = np.random.default_rng(3)
+    ng = 3
+    w = rng.random((len(cells), ng))
+    w /= w.sum(axis=1, keepdims=True)
+    tc = np.array([-1.0, 0.0, 0.35, 0.8, 1.3, 1.9, 4.0])
+    S = 0.45 + 0.5 * rng.random((ng, len(tc)))
+    c.set_ring({"i": cells[:, 0] + g, "j": cells[:, 1] + g, "bed": bed[cells[:, 0], cells[:, 1]], "wg": w.ravel(),
+                "NG": ng, "n": len(cells), "t_common": tc, "stage_all": S})
+    n = c.run(T_END)
+    case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=bed, dx=2.0, dy=2.0, bc_x="fall", bc_y="fall",
+                       manning=0.03, rain=30.0 / 3.6e6, active=m, sponge=(keep, amb), clamp=grid,
+                       ring={"cells": cells, "bed": bed[cells[:, 0], cells[:, 1]], "w": w, "t_common": tc,
+                             "stage_all": S})
+    o = O.DenseOracle(case, _physics(c.cfg), np.dtype(dtype))
+    o.cfl_skip = ~m
+    o.run(T_END)
+    print(dtype, trav, n, flush=True)
+    assert o.nsteps == c.nsteps == n > 5 and o.t == c.t
+    ij = c.fm.ij - g
+    inner = (ij[:, 0] >= 0) & (ij[:, 0] < NX) & (ij[:, 1] >= 0) & (ij[:, 1] < NY)
+    qf = c.q_flat()
+    assert np.array_equal(qf[:, inner], o.interior[:, ij[inner, 0], ij[inner, 1]])
+    k = c._flat_ids(cells[:, 0] + g, cells[:, 1] + g)
+    assert (k >= 0).all() and (qf[0, k] > 0).any() and np.all(qf[1:, k] == 0)   # wet still-water ring

@@ -1,9 +1,12 @@
-"""Pins the oracle's open-boundary sponge and stage clamp (coastal_force) on the CPU.
+"""Pins the oracle's coastal forcings (open-boundary sponge, IDW gauge ring, stage clamp:
+coastal_force, ring_depth) on the CPU.
 
-The reference's tests hold no vectors for these two forcings, so each check is one the
+The reference's tests hold no vectors for these forcings, so each check is one the
 mathematics fixes: the identity fields, the full-absorption limit, the geometric series of
 a repeated relaxation, the clamp's invariants (depth cap, velocity kept, untouched cells
-bitwise), the sponge-before-clamp order, and the place of both in the step (after the
+bitwise), the sponge-before-clamp order, the ring's stage at and outside the sample times
+and midway, its inverse-distance weights, its use of the step's start time and its place
+between the sponge and the clamp, and the place of all of them in the step (after the
 update and the stage).
 """
 import numpy as np
@@ -138,3 +141,55 @@ def test_step_clamp_caps_the_updated_depth():
     for _ in range(3):
         o.step(o.next_dt())
         assert np.all(o.interior[0, 4:8, 4:8] <= np.float32(0.45))
+
+
+def _ring(cells, bed, w, tc, S):
+    return {"cells": np.asarray(cells), "bed": np.asarray(bed, float), "w": np.asarray(w, float),
+            "t_common": np.asarray(tc, float), "stage_all": np.asarray(S, float)}
+
+
+def test_ring_depth_at_and_outside_the_sample_times():
+    S = np.array([[0.5, 1.5, 0.9, 2.0], [1.0, 0.2, 0.4, 3.0]])
+    tc = [0.0, 360.0, 720.0, 1080.0]
+    r = _ring([[0, 0], [1, 1]], [0.1, 0.3], [[1.0, 0.0], [0.0, 1.0]], tc, S)
+    # first sample, and before it (weight clamped to 0): the first column exactly
+    for t in (0.0, -50.0):
+        assert np.array_equal(O.ring_depth(r, t, np.float32), np.float32(S[[0, 1], 0]) - np.float32([0.1, 0.3]))
+    # past the last sample (weight clamped to 1): the last column exactly
+    assert np.array_equal(O.ring_depth(r, 5000.0, np.float32), np.float32(S[[0, 1], 3]) - np.float32([0.1, 0.3]))
+    # midway between two samples: the mean (float64)
+    h = O.ring_depth(r, 540.0, np.float64)
+    assert np.allclose(h, [(1.5 + 0.9) / 2 - 0.1, (0.2 + 0.4) / 2 - 0.3], rtol=1e-12)
+
+
+def test_ring_depth_idw_weights_and_dry_cells():
+    S = np.array([[1.0, 1.0], [1.0, 1.0], [1.0, 1.0]])
+    w = np.array([[0.2, 0.3, 0.5], [0.6, 0.3, 0.1]])
+    r = _ring([[0, 0], [0, 1]], [0.25, 1.5], w, [0.0, 600.0], S)
+    h = O.ring_depth(r, 100.0, np.float64)
+    assert np.isclose(h[0], 0.75, rtol=1e-14) and h[1] == 0.0         # above the stage: dry
+    r["bed"] = np.array([np.nan, 0.0])
+    assert O.ring_depth(r, 100.0, np.float32)[0] == 0.0                # fmax(0, NaN) = 0
+
+
+def test_ring_uses_the_step_start_time_and_sits_between_sponge_and_clamp():
+    nx, ny = 12, 10
+    cells = [[0, 2], [0, 3], [0, 4]]
+    ring = _ring(cells, [0.0, 0.0, 0.0], [[1.0], [1.0], [1.0]], [0.0, 100.0], [[1.0, 11.0]])
+    keep = np.ones((nx, ny))
+    keep[0, 3] = 0.5                       # a band cell of the ring: the ring overwrites it
+    hmax = np.full((nx, ny), np.inf)
+    hmax[0, 4] = 0.8                       # a clamped ring cell: capped after the ring
+    case = _case(active=np.pad(np.ones((nx - 2, ny), bool), ((2, 0), (0, 0))), ring=ring,
+                 sponge=(keep, np.zeros((nx, ny))), clamp=hmax)
+    case.h[:2] = 0.0
+    case.hu[:2] = 0.0
+    o = O.DenseOracle(case, O.Physics(friction=False), np.float32)
+    o.cfl_skip = ~case.active
+    dt1 = o.next_dt()
+    o.step(dt1)
+    assert np.all(o.interior[0, 0, 2:4] == np.float32(1.0)) and o.interior[0, 0, 4] == np.float32(0.8)
+    assert np.all(o.interior[1:, 0, 2:5] == 0)
+    o.step(o.next_dt())
+    want = np.float32(np.float32(1.0) * np.float32(1 - dt1 / 100.0) + np.float32(11.0) * np.float32(dt1 / 100.0))
+    assert o.interior[0, 0, 2] == want != np.float32(1.0)
