@@ -65,8 +65,8 @@ class GaugeMisfit:
     def __init__(self, solver: CompressedSolver, gauges: np.ndarray, dts: np.ndarray,
                  h_star: Optional[np.ndarray] = None, checkpoint_every: int = 100):
         if solver.has_coastal_forcing:
-            raise NotImplementedError("GaugeMisfit: the sponge, the stage clamp and the gauge ring have no reverse "
-                                      "kernels; build the misfit on a solver without set_sponge/set_clamp/set_ring")
+            raise NotImplementedError("GaugeMisfit: the coastal forcings (sponge, gauge ring, clamp, infiltration, "
+                                      "drain) have no reverse kernels; build the misfit on a solver without them")
         self.s = solver
         solver._tables_taken = True
         fm = solver.fm

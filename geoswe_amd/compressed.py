@@ -43,6 +43,7 @@ class CompressedSolver:
         self._flags = L.DeviceArray.from_host(fl) if fl.any() else None
         self._flags_base = fl              # without the coastal-forcing bit (set_sponge / set_clamp)
         self._sponge = self._clamp = self._force_ring = self._force_ring_row = self._ring = None
+        self._infil = self._drain = None
         self._tables_taken = False         # an Ensemble or GaugeMisfit holds this solver's device pointers
         t = np.zeros(L.MAX_CLASSES, self.dtype)
         t[:len(ntab)] = ntab
@@ -208,6 +209,53 @@ class CompressedSolver:
         self._rebuild_force()
         return self
 
+    def set_infil(self, infil):
+        """Constant-rate infiltration per land-cover class (compressed_solver.py:3212, the
+        ``infiltrate_flat`` kernel 143-158): ``{"tab": rate}``, one rate in m/s per Manning
+        class. Each step, after the update and the running max and before the sponge, an
+        active cell with ``h >= h_min`` loses ``rate[class]*dt`` of depth (floored at 0) with
+        its velocity kept. ``None`` removes it."""
+        self._require_unbuilt("set_infil")
+        new = None
+        if infil is not None and infil.get("tab") is not None:
+            tab = np.asarray(infil["tab"], np.float64).ravel()
+            if self._n_field is not None:
+                raise ValueError("set_infil: the solver has a per-cell Manning n plane and no class per cell")
+            if len(tab) > L.MAX_CLASSES:
+                raise ValueError(f"set_infil: {len(tab)} rates for at most {L.MAX_CLASSES} classes")
+            t = np.zeros(L.MAX_CLASSES, self.dtype)
+            t[:len(tab)] = tab
+            new = L.DeviceArray.from_host(t)
+        self._infil = new
+        return self
+
+    def set_drain(self, drain):
+        """Karst drain cap (compressed_solver.py:3211, 1688-1694; the ``drain_cap`` kernel
+        269-286): ``{"idx": flat ids, "h_tgt": depth}`` or ``{"idx", "h_tgt_arr": per-cell
+        depths}`` (default 0.5 m). Each step, after the clamp, a listed cell deeper than its
+        target is set to it with its velocity kept (momentum dropped below 1e-12 m).
+        ``None`` or an empty ``idx`` removes it."""
+        self._require_unbuilt("set_drain")
+        new = None
+        if drain is not None and np.asarray(drain["idx"]).size > 0:
+            idx = np.asarray(drain["idx"], np.int64).ravel()
+            if (idx < 0).any() or (idx >= self.fm.n_stored).any():
+                raise ValueError(f"set_drain: flat ids must lie in [0, {self.fm.n_stored})")
+            if drain.get("h_tgt_arr") is not None:
+                ht = np.asarray(drain["h_tgt_arr"], np.float64).ravel()
+                if len(ht) != len(idx):
+                    raise ValueError(f"set_drain: {len(ht)} target depths for {len(idx)} cells")
+            else:
+                ht = np.full(len(idx), float(drain.get("h_tgt", 0.5)))
+            hd = np.full(self.fm.n_stored, np.inf, self.dtype)
+            hd[idx] = ht.astype(self.dtype)
+            cells = ~(np.isnan(hd) | (hd == np.inf))
+            if cells.any():
+                new = (L.DeviceArray.from_host(hd), cells)
+        self._drain = new
+        self._rebuild_force()
+        return self
+
     def _flat_ids(self, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
         """Flat id of each padded (row, col), -1 where the cell is not stored."""
         fm = self.fm
@@ -219,7 +267,7 @@ class CompressedSolver:
 
     def _rebuild_force(self) -> None:
         force = np.zeros(self.fm.n_stored, bool)
-        for f in (self._sponge, self._clamp):
+        for f in (self._sponge, self._clamp, self._drain):
             if f is not None:
                 force |= f[1]
         fl = self._flags_base
@@ -239,7 +287,7 @@ class CompressedSolver:
 
     @property
     def has_coastal_forcing(self) -> bool:
-        return self._sponge is not None or self._clamp is not None or self._ring is not None
+        return any(f is not None for f in (self._sponge, self._clamp, self._ring, self._infil, self._drain))
 
     # ------------------------------------------------------------------ structs
     def _phys(self) -> L.GsPhys:
@@ -269,6 +317,8 @@ class CompressedSolver:
         f.tile_e0, f.tile_irc, f.ent = (t.ptr for t in self._tiles)
         f.sponge = self._sponge[0].ptr if self._sponge is not None else None
         f.clamp = self._clamp[0].ptr if self._clamp is not None else None
+        f.infil = self._infil.ptr if self._infil is not None else None
+        f.drain = self._drain[0].ptr if self._drain is not None else None
         f.n_force_ring = self._force_ring.shape[0] if self._force_ring is not None else 0
         f.force_ring = self._force_ring.ptr if self._force_ring is not None else None
         r = self._ring

@@ -57,6 +57,8 @@ struct FlatK {
   const uint64_t* ent;
   const T* sponge;  // nullable [N][2] (keep, amb) of GS_FLAG_FORCE cells
   const T* clamp;   // nullable [N] h_max of GS_FLAG_FORCE cells
+  const T* infil;   // nullable [32] infiltration rate per Manning class
+  const T* drain;   // nullable [N] karst target depth of GS_FLAG_FORCE cells
   int64_t n_fring;  // forced cells of the ring (stored, not active): sponge, clamp, gauge ring
   const int64_t* fring;
   const int32_t* fring_row;  // nullable: gauge-ring row of each entry, -1 for none
@@ -104,6 +106,8 @@ struct FlatCtx {
   const GS_G uint64_t* ent;
   const GS_G T* sponge;
   const GS_G T* clamp;
+  const GS_G T* infil;
+  const GS_G T* drain;
   const GS_G T* frame;   // gridded rain frame of this step (rain_mode 2)
   const T* ntab;         // Manning n per class (LDS copy or the member's table)
   int64_t n, rain_pitch;
@@ -192,6 +196,8 @@ __device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k,
   x.ent = gp(a.ent);
   x.sponge = gp(a.sponge);
   x.clamp = gp(a.clamp);
+  x.infil = gp(a.infil);
+  x.drain = gp(a.drain);
   x.frame = k.frame;
   x.ntab = a.ntab;
   x.n = a.n;
@@ -208,6 +214,16 @@ __device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k,
 // One wavefront pass over flat ids chunk*60-2 .. chunk*60+61: the residual (MODE 0) or the
 // fused update (MODE 1) of the 60 middle ids, folding their CFL speed bits into lam.
 // Every lane of the wavefront must call it (cross-lane shuffles).
+// Element at byte offset `off` of an array segment that starts at `row` (uniform).
+template <typename P>
+__device__ __forceinline__ const GS_G P& gat(const GS_G P* row, uint32_t off) {
+  return *(const GS_G P*)((const GS_G char*)row + off);
+}
+template <typename P>
+__device__ __forceinline__ GS_G P& gat(GS_G P* row, uint32_t off) {
+  return *(GS_G P*)((GS_G char*)row + off);
+}
+
 // Sponge band then stage clamp on a GS_FLAG_FORCE cell, after the update, the stage and the
 // running max (sponge_band, compressed_solver.py:1320-1326: h*keep then + amb, two roundings;
 // stage_clamp_flat, 424-440). keep = 1, amb = 0, h_max = +inf leave a cell unchanged.
@@ -227,9 +243,34 @@ __device__ __forceinline__ void clamp_cell(T hx, T& h, T& hu, T& hv) {
   }
 }
 template <typename T>
-__device__ __forceinline__ void coastal_force(T kp, T am, T hx, bool sp, bool cl, T& h, T& hu, T& hv) {
-  if (sp) sponge_cell<T>(kp, am, h, hu, hv);
-  if (cl) clamp_cell<T>(hx, h, hu, hv);
+__device__ __forceinline__ void drain_cell(T ht, T& h, T& hu, T& hv) {
+  if (h > ht) {
+    const T r = h > T(1e-12) ? ht / h : T(0);
+    h = ht;
+    hu = hu * r;
+    hv = hv * r;
+  }
+}
+// Constant-rate infiltration of one active cell (infiltrate_flat): nothing below h_min.
+template <typename T>
+__device__ __forceinline__ void infil_cell(T rate, T dt, T hmin, T& h, T& hu, T& hv) {
+  if (!(h < hmin)) {
+    const T dh = rate * dt;
+    T hn = h - dh;
+    hn = hn < T(0) ? T(0) : hn;
+    const T r = hn / h;
+    h = hn;
+    hu = hu * r;
+    hv = hv * r;
+  }
+}
+// The forcings of a GS_FLAG_FORCE cell in order: sponge, (gauge ring: the ring pass), clamp,
+// karst drain.
+template <typename T>
+__device__ __forceinline__ void coastal_force(const FlatCtx<T>& x, int64_t base, uint32_t off, T& h, T& hu, T& hv) {
+  if (x.sponge) sponge_cell<T>(gat(x.sponge + 2 * base, 2 * off), gat(x.sponge + 2 * base + 1, 2 * off), h, hu, hv);
+  if (x.clamp) clamp_cell<T>(gat(x.clamp + base, off), h, hu, hv);
+  if (x.drain) drain_cell<T>(gat(x.drain + base, off), h, hu, hv);
 }
 
 template <typename T, int MODE>
@@ -333,11 +374,8 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
     hv = T(0);
   }
   if (a.max_h && h > a.max_h[k]) a.max_h[k] = h;
-  if (fg & GS_FLAG_FORCE) {
-    const bool sp = x.sponge != nullptr, cl = x.clamp != nullptr;
-    coastal_force<T>(sp ? x.sponge[2 * k] : T(1), sp ? x.sponge[2 * k + 1] : T(0), cl ? x.clamp[k] : T(0), sp, cl,
-                     h, hu, hv);
-  }
+  if (x.infil) infil_cell<T>(x.infil[fg & GS_FLAG_CLASS_MASK], x.dt, P.h_min, h, hu, hv);
+  if (fg & GS_FLAG_FORCE) coastal_force<T>(x, k, 0u, h, hu, hv);
   x.qo0[k] = h;
   x.qo1[k] = hu;
   x.qo2[k] = hv;
@@ -359,16 +397,6 @@ __device__ __forceinline__ uint32_t sweep_lane(const GS_G uint64_t* e, int lane,
       __builtin_amdgcn_mbcnt_hi((unsigned)(sm >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)sm, 0u));
   st = ((sm >> lane) & 1ull) != 0ull;
   return below * (uint32_t)sizeof(T);
-}
-
-// Element at byte offset `off` of an array segment that starts at `row` (uniform).
-template <typename P>
-__device__ __forceinline__ const GS_G P& gat(const GS_G P* row, uint32_t off) {
-  return *(const GS_G P*)((const GS_G char*)row + off);
-}
-template <typename P>
-__device__ __forceinline__ GS_G P& gat(GS_G P* row, uint32_t off) {
-  return *(GS_G P*)((GS_G char*)row + off);
 }
 
 // State and bed of a lane's cell; lanes on unstored cells read nothing and hold zeros
@@ -506,11 +534,8 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
         hv = T(0);
       }
       if (x.max_h && h > gat(x.max_h + gC, oC)) gat(x.max_h + gC, oC) = h;
-      if (fg & GS_FLAG_FORCE) {
-        const bool sp = x.sponge != nullptr, cl = x.clamp != nullptr;
-        coastal_force<T>(sp ? gat(x.sponge + 2 * gC, 2 * oC) : T(1), sp ? gat(x.sponge + 2 * gC + 1, 2 * oC) : T(0),
-                         cl ? gat(x.clamp + gC, oC) : T(0), sp, cl, h, hu, hv);
-      }
+      if (x.infil) infil_cell<T>(x.infil[fg & GS_FLAG_CLASS_MASK], x.dt, P.h_min, h, hu, hv);
+      if (fg & GS_FLAG_FORCE) coastal_force<T>(x, gC, oC, h, hu, hv);
       gat(x.qo0 + gC, oC) = h;
       gat(x.qo1 + gC, oC) = hu;
       gat(x.qo2 + gC, oC) = hv;
@@ -632,6 +657,7 @@ __global__ __launch_bounds__(kFBlock) void k_flat_ring_force(FlatK<T> a) {
       hv = T(0);
     }
     if (fc && cl) clamp_cell<T>(x.clamp[k], h, hu, hv);
+    if (fc && x.drain) drain_cell<T>(x.drain[k], h, hu, hv);
     x.qo0[k] = h;
     x.qo1[k] = hu;
     x.qo2[k] = hv;
@@ -670,8 +696,8 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   if (f->n_tiles > 0 && (!f->tile_e0 || !f->tile_irc || !f->ent)) return fail(GS_E_ARG, "tile tables are NULL");
   if (f->n_force_ring < 0 || f->n_force_ring > f->n_stored) return fail(GS_E_RANGE, "n_force_ring out of range");
   if (f->n_force_ring > 0 && !f->force_ring) return fail(GS_E_ARG, "force_ring is NULL");
-  if (f->n_force_ring > 0 && !f->sponge && !f->clamp && !f->force_ring_row)
-    return fail(GS_E_ARG, "force_ring lists cells but no sponge, clamp or gauge ring is set");
+  if (f->n_force_ring > 0 && !f->sponge && !f->clamp && !f->drain && !f->force_ring_row)
+    return fail(GS_E_ARG, "force_ring lists cells but no sponge, clamp, drain or gauge ring is set");
   if (f->force_ring_row && (f->ring_ng < 1 || f->ring_nt < 2 || !f->ring_bed || !f->ring_w || !f->ring_t ||
                             !f->ring_stage))
     return fail(GS_E_ARG, "gauge ring: needs >= 1 gauge, >= 2 time samples and its bed, weight, time and stage "
@@ -712,6 +738,8 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   a.ent = f->ent;
   a.sponge = (const T*)f->sponge;
   a.clamp = (const T*)f->clamp;
+  a.infil = (const T*)f->infil;
+  a.drain = (const T*)f->drain;
   a.n_fring = f->n_force_ring;
   a.fring = f->force_ring;
   a.fring_row = f->force_ring_row;

@@ -180,6 +180,13 @@ typedef struct GsFlat {
    * r = h_max/h, h <- h_max, hu, hv <- r*hu, r*hv. */
   const void* sponge;        /* nullable [N][2] (keep, amb), state dtype */
   const void* clamp;         /* nullable [N] h_max, state dtype (+inf: no clamp) */
+  /* Constant-rate infiltration (infiltrate_flat, compressed_solver.py:143-158) on every active
+   * cell, after the running max and before the sponge: where h >= h_min,
+   * h' = max(h - rate[class]*dt, 0), hu, hv scaled by h'/h. Karst drain (drain_cap, 269-286)
+   * on GS_FLAG_FORCE cells, last (after the clamp): where h > h_tgt,
+   * r = h > 1e-12 ? h_tgt/h : 0, h = h_tgt, hu, hv scaled by r. */
+  const void* infil;         /* nullable [32] rate per Manning class (m/s), state dtype */
+  const void* drain;         /* nullable [N] h_tgt, state dtype (+inf: no drain) */
   /* GS_FLAG_FORCE cells of the ring (stored, not active): the step writes
    * force(state) for them, as the reference's gathered kernels touch every stored cell
    * of the band; the residual and the CFL never see them. */

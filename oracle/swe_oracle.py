@@ -23,6 +23,7 @@ Sources followed (paths relative to the reference checkout):
 * rain lookup / stage boundary ........ forcing.py:170-216, 315-377
 * open-boundary sponge, stage clamp .... compressed_solver.py:1320-1326, 424-440
 * IDW gauge ring ....................... compressed_solver.py:287-307, 2338-2348
+* infiltration, karst drain ............ compressed_solver.py:143-158, 269-286
 
 Documented departures (shared with the HIP kernels, which implement the same rule):
 
@@ -519,8 +520,16 @@ def ring_depth(ring: dict, t: float, dtype) -> np.ndarray:
     return np.fmax(dt.type(0), eta - np.asarray(ring["bed"]).astype(dt))
 
 
-def coastal_force(q_int: np.ndarray, sponge=None, clamp=None, ring=None, t: float = 0.0) -> None:
-    """Open-boundary sponge, gauge ring, stage clamp, in place on a (3, nx, ny) state.
+def coastal_force(q_int: np.ndarray, sponge=None, clamp=None, ring=None, t: float = 0.0, drain=None,
+                  infil=None) -> None:
+    """Infiltration, open-boundary sponge, gauge ring, stage clamp, karst drain, in place on a
+    (3, nx, ny) state.
+
+    Infiltration (infiltrate_flat, compressed_solver.py:143-158), ``infil = (cls, rate, dt,
+    h_min, active)``: on active cells with ``h >= h_min``, ``h' = max(h - rate[cls]*dt, 0)``
+    and ``hu, hv`` scaled by ``h'/h`` (``dt`` and ``h_min`` in the state dtype). Karst drain
+    (drain_cap, 269-286), ``drain`` an ``h_tgt`` field (+inf where none), last: where
+    ``h > h_tgt``, ``r = h_tgt/h`` (0 when ``h <= 1e-12``), ``h = h_tgt``, momentum scaled by r.
 
     Sponge (compressed_solver.py:1320-1326): ``h <- h*keep + amb`` as a multiply then an
     add (two roundings), ``hu, hv <- hu*keep, hv*keep``, on the band (``keep != 1`` or
@@ -529,9 +538,24 @@ def coastal_force(q_int: np.ndarray, sponge=None, clamp=None, ring=None, t: floa
     ``h_max`` that is neither +inf nor NaN. A cell named by either list gets both (the
     identity sponge ``h*1 + 0`` included), as the fused step applies them. ``ring``
     (``{"cells": (n, 2) ij, "bed", "w": (n, NG), "t_common", "stage_all": (NG, nt)}``) sets
-    its cells to still water at :func:`ring_depth` of time ``t`` between the two.
+    its cells to still water at :func:`ring_depth` of time ``t`` between the two. The cells
+    named by the sponge band, the clamp or the drain are the forced cells: each gets the
+    sponge (if any), the clamp (if any) and the drain (if any).
     """
     dt = q_int.dtype
+    h, hu, hv = q_int[0], q_int[1], q_int[2]
+    if infil is not None:
+        cls, rate, idt, hmin, act = infil
+        rate = np.asarray(rate).astype(dt)[np.asarray(cls)]
+        on = ~(h < dt.type(hmin))
+        if act is not None:
+            on &= np.asarray(act, bool)
+        hn = h[on] - rate[on] * dt.type(idt)
+        hn = np.where(hn < 0, dt.type(0), hn)
+        r = hn / h[on]
+        h[on] = hn
+        hu[on] = hu[on] * r
+        hv[on] = hv[on] * r
     cells = np.zeros(q_int.shape[1:], bool)
     if sponge is not None:
         kp = np.asarray(sponge[0]).astype(dt)
@@ -540,7 +564,9 @@ def coastal_force(q_int: np.ndarray, sponge=None, clamp=None, ring=None, t: floa
     if clamp is not None:
         hx = np.asarray(clamp).astype(dt)
         cells |= ~(np.isnan(hx) | (hx == np.inf))
-    h, hu, hv = q_int[0], q_int[1], q_int[2]
+    if drain is not None:
+        ht = np.asarray(drain).astype(dt)
+        cells |= ~(np.isnan(ht) | (ht == np.inf))
     if sponge is not None:
         h[cells] = h[cells] * kp[cells] + am[cells]
         hu[cells] = hu[cells] * kp[cells]
@@ -554,6 +580,12 @@ def coastal_force(q_int: np.ndarray, sponge=None, clamp=None, ring=None, t: floa
         over = cells & (h > hx)
         r = hx[over] / h[over]
         h[over] = hx[over]
+        hu[over] = hu[over] * r
+        hv[over] = hv[over] * r
+    if drain is not None:
+        over = cells & (h > ht)
+        r = np.where(h[over] > dt.type(1e-12), ht[over] / h[over], dt.type(0))
+        h[over] = ht[over]
         hu[over] = hu[over] * r
         hv[over] = hv[over] * r
 
@@ -577,12 +609,14 @@ class DenseCase:
     sponge: Optional[tuple] = None     # (keep, amb), (nx, ny) each
     clamp: Optional[np.ndarray] = None  # (nx, ny) h_max, +inf where none
     ring: Optional[dict] = None         # gauge ring: see coastal_force
+    infil: Optional[tuple] = None       # (class (nx, ny) ints, rate per class m/s)
+    drain: Optional[np.ndarray] = None  # (nx, ny) karst target depth, +inf where none
 
 
 class DenseOracle:
     """Dense time stepping in the reference order: BC fill -> residual -> update -> t += dt
-    -> stage -> sponge -> gauge ring (at the step's start time) -> clamp -> (next) CFL.
-    Works in float32 or float64."""
+    -> stage -> infiltration -> sponge -> gauge ring (at the step's start time) -> clamp ->
+    karst drain -> (next) CFL. Works in float32 or float64."""
 
     def __init__(self, case: DenseCase, ph: Physics, dtype=np.float32):
         self.case, self.ph = case, ph
@@ -650,8 +684,11 @@ class DenseOracle:
             self.q[0, ii + g, jj + g] = np.maximum(eta - bed, self.dtype.type(0))
             self.q[1, ii + g, jj + g] = 0
             self.q[2, ii + g, jj + g] = 0
-        if c.sponge is not None or c.clamp is not None or c.ring is not None:
-            coastal_force(self.q[I], c.sponge, c.clamp, c.ring, t0)
+        if any(f is not None for f in (c.sponge, c.clamp, c.ring, c.drain, c.infil)):
+            inf = None
+            if c.infil is not None:
+                inf = (c.infil[0], c.infil[1], dt, ph.h_min, act)
+            coastal_force(self.q[I], c.sponge, c.clamp, c.ring, t0, c.drain, inf)
         self.nsteps += 1
 
     def next_dt(self, t_end=None, dt_max=None) -> float:

@@ -248,3 +248,109 @@ def test_gauge_ring_matches_oracle(gpu, dtype, trav):
     assert np.array_equal(qf[:, inner], o.interior[:, ij[inner, 0], ij[inner, 1]])
     k = c._flat_ids(cells[:, 0] + g, cells[:, 1] + g)
     assert (k >= 0).all() and (qf[0, k] > 0).any() and np.all(qf[1:, k] == 0)   # wet still-water ring
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("trav", ["tiles", "chunks"])
+def test_full_coastal_tail_matches_oracle(gpu, dtype, trav, monkeypatch):
+    """Every coastal forcing at once (Manning classes with per-class infiltration, sponge,
+    gauge ring, clamp, karst drain on active and ring cells): state of every stored interior
+    cell, running maximum, steps and time equal the oracle's bit for bit."""
+    from geoswe_amd import CompressedSolver
+    q0, bed = rough_state(NX, NY, seed=37, depth=0.7, rough=1.0, vel=0.3)
+    m = irregular_mask(NX, NY, seed=9)
+    q0 = q0 * m
+    rng = np.random.default_rng(8)
+    cls = rng.integers(0, 6, (NX, NY)).astype(np.uint8)
+    table = np.linspace(0.02, 0.07, 6).astype(np.float32)
+    rates = np.array([0.0, 2e-3, 5e-3, 0.0, 1e-2, 3e-3])
+    keep, amb, rows, cols, hmax, grid = _forcings(m)
+    s = dense(q0, bed, friction="manning", rainfall=30.0 / 3.6e6, dtype=dtype)
+    g = s.mesh.ngh
+    s.set_manning_table(np.pad(cls, g), table)
+    s.set_inside_mask(m)
+    s.set_cfl_ghost_mask(~m)
+    c = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=True)
+    c.set_sponge({"keep": _padded(keep, g, 1.0), "amb": _padded(amb, g, 0.0)})
+    c.set_clamp({"rows": rows + g, "cols": cols + g, "hmax": hmax})
+    near = np.zeros_like(m)
+    for ax in (0, 1):
+        for sh in (1, -1):
+            near |= np.roll(m, sh, ax)
+    cells = np.argwhere(~m & near)
+    w = rng.random((len(cells), 2))
+    w /= w.sum(axis=1, keepdims=True)
+    tc = np.array([-1.0, 0.5, 1.2, 3.0])
+    S = 0.45 + 0.5 * rng.random((2, len(tc)))
+    ring = {"cells": cells, "bed": bed[cells[:, 0], cells[:, 1]], "w": w, "t_common": tc, "stage_all": S}
+    c.set_ring({"i": cells[:, 0] + g, "j": cells[:, 1] + g, "bed": ring["bed"], "wg": w, "NG": 2, "n": len(cells),
+                "t_common": tc, "stage_all": S})
+    c.set_infil({"tab": rates})
+    ij = c.fm.ij - g
+    act = c.fm.active
+    cand = np.nonzero(act & (ij[:, 0] > 30) & (ij[:, 0] < 40))[0][::3]
+    ring_ids = np.nonzero(~act & (ij[:, 0] >= 0) & (ij[:, 0] < NX) & (ij[:, 1] >= 0) & (ij[:, 1] < NY))[0][::7]
+    didx = np.concatenate([cand, ring_ids])
+    ht = 0.3 + 0.2 * rng.random(len(didx))
+    c.set_drain({"idx": didx, "h_tgt_arr": ht})
+    c.traversal = trav
+    n = c.run(T_END)
+
+    dgrid = np.full((NX, NY), np.inf)
+    dgrid[ij[didx, 0], ij[didx, 1]] = ht
+    mx = np.zeros((NX, NY), np.dtype(dtype))
+    orig = O.coastal_force
+
+    def recording(q, *args):
+        np.maximum(mx, q[0], out=mx)
+        orig(q, *args)
+
+    monkeypatch.setattr(O, "coastal_force", recording)
+    case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=bed, dx=2.0, dy=2.0, bc_x="fall", bc_y="fall",
+                       manning=table[cls], rain=30.0 / 3.6e6, active=m, sponge=(keep, amb), clamp=grid,
+                       ring=ring, infil=(cls, rates), drain=dgrid)
+    o = O.DenseOracle(case, _physics(c.cfg), np.dtype(dtype))
+    o.cfl_skip = ~m
+    o.run(T_END)
+    print(dtype, trav, n, flush=True)
+    assert o.nsteps == c.nsteps == n > 5 and o.t == c.t
+    inner = (ij[:, 0] >= 0) & (ij[:, 0] < NX) & (ij[:, 1] >= 0) & (ij[:, 1] < NY)
+    qf = c.q_flat()
+    assert np.array_equal(qf[:, inner], o.interior[:, ij[inner, 0], ij[inner, 1]])
+    ia = ij[act]
+    assert np.array_equal(c.max_depth()[ia[:, 0], ia[:, 1]], mx[ia[:, 0], ia[:, 1]])
+    assert np.all(qf[0, didx] <= ht.astype(dtype))
+
+
+def test_ensemble_carries_infiltration_and_active_drain(gpu):
+    from geoswe_amd import CompressedSolver
+    from geoswe_amd.ensemble import Ensemble
+    q0, bed = rough_state(NX, NY, seed=38, depth=0.7, rough=1.0, vel=0.3)
+    m = np.zeros((NX, NY), bool)
+    m[2:NX - 2, 2:NY - 2] = True
+    q0 = q0 * m
+    cls = np.random.default_rng(9).integers(0, 4, (NX, NY)).astype(np.uint8)
+
+    def mk():
+        s = dense(q0, bed, friction="manning")
+        g = s.mesh.ngh
+        s.set_manning_table(np.pad(cls, g), np.array([0.02, 0.03, 0.05, 0.04], np.float32))
+        s.set_inside_mask(m)
+        s.set_cfl_ghost_mask(~m)
+        c = CompressedSolver.from_dense(s, cfl_linf=True)
+        c.set_infil({"tab": [0.0, 4e-3, 1e-2, 2e-3]})
+        ids = np.nonzero(c.fm.active)[0][::11]
+        c.set_drain({"idx": ids, "h_tgt": 0.35})
+        assert c._force_ring is None
+        return c
+
+    plain = [mk(), mk()]
+    for p in plain:
+        p.advance(20)
+    mem = [mk(), mk()]
+    ens = Ensemble(mem, graph_batch=8)
+    ens.advance(20)
+    for p, e in zip(plain, mem):
+        assert p.nsteps == e.nsteps == 20 and p.t == e.t
+        assert np.array_equal(p.q_flat(), e.q_flat())
+    ens.close()

@@ -193,3 +193,53 @@ def test_ring_uses_the_step_start_time_and_sits_between_sponge_and_clamp():
     o.step(o.next_dt())
     want = np.float32(np.float32(1.0) * np.float32(1 - dt1 / 100.0) + np.float32(11.0) * np.float32(dt1 / 100.0))
     assert o.interior[0, 0, 2] == want != np.float32(1.0)
+
+
+def _one(h, hu, hv=0.0, dtype=np.float32):
+    return np.array([[[h]], [[hu]], [[hv]]], dtype)
+
+
+def test_infiltration_closed_forms():
+    cls = np.zeros((1, 1), int)
+    # partial loss: h = 1, rate*dt = 0.25 -> 0.75 exactly, momentum scaled by 0.75
+    q = _one(1.0, 0.5)
+    O.coastal_force(q, infil=(cls, [0.125], 2.0, 1e-6, None))
+    assert q[0, 0, 0] == np.float32(0.75) and q[1, 0, 0] == np.float32(0.375)
+    # more than the depth: dry, no momentum
+    q = _one(0.1, 0.02, -0.01)
+    O.coastal_force(q, infil=(cls, [1.0], 1.0, 1e-6, None))
+    assert q[0, 0, 0] == 0 and q[1, 0, 0] == 0 and q[2, 0, 0] == 0
+    # below h_min: untouched; zero rate: bitwise identity; inactive: untouched
+    for h, rate, act in ((5e-7, 1.0, None), (0.3, 0.0, None), (0.3, 1.0, np.zeros((1, 1), bool))):
+        q = _one(h, 1e-3)
+        q0 = q.copy()
+        O.coastal_force(q, infil=(cls, [rate], 0.1, 1e-6, act))
+        assert np.array_equal(q, q0)
+
+
+def test_infiltration_by_class_comes_before_the_sponge():
+    cls = np.array([[0, 1]])
+    q = np.array([[[1.0, 1.0]], [[0.0, 0.0]], [[0.0, 0.0]]], np.float32)
+    O.coastal_force(q, sponge=(np.full((1, 2), 0.5), np.full((1, 2), 0.25)), infil=(cls, [0.0, 0.5], 1.0, 1e-6, None))
+    assert q[0, 0, 0] == np.float32(0.75)                 # class 0: no loss, then sponge
+    assert q[0, 0, 1] == np.float32(0.5)                  # (1 - 0.5)*0.5 + 0.25; the other order gives 0.25
+
+
+def test_drain_caps_with_its_tiny_depth_guard_after_the_clamp():
+    q = _one(2.0, 1.0)
+    O.coastal_force(q, drain=np.array([[0.5]]))
+    assert q[0, 0, 0] == np.float32(0.5) and q[1, 0, 0] == np.float32(0.25)
+    q = _one(5e-13, 1e-13)                                 # above the target but under 1e-12: r = 0
+    O.coastal_force(q, drain=np.array([[1e-13]]))
+    assert q[0, 0, 0] == np.float32(1e-13) and q[1, 0, 0] == 0
+    q = _one(0.3, 0.1)
+    q0 = q.copy()
+    O.coastal_force(q, drain=np.array([[0.5]]))            # under the target: untouched
+    assert np.array_equal(q, q0)
+    # clamp (0.5) then drain (0.8): the momentum is scaled once, by 0.5/h
+    h, hu = np.float32(1.9748254), np.float32(1.905881)
+    q = _one(h, hu)
+    O.coastal_force(q, clamp=np.array([[0.5]]), drain=np.array([[0.8]]))
+    once = np.float32(hu * np.float32(np.float32(0.5) / h))
+    twice = np.float32(np.float32(hu * np.float32(np.float32(0.8) / h)) * np.float32(np.float32(0.5) / np.float32(0.8)))
+    assert once != twice and q[1, 0, 0] == once and q[0, 0, 0] == np.float32(0.5)
