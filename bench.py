@@ -222,9 +222,32 @@ def coastal(args):
         "dense_ms_per_step": ms_d,
         "dense_equals_flat": ok,
     }
+    if args.forcings:
+        out["forcing_tail"] = coastal_forcing_tail(case, args, ms_f)
     print(json.dumps(out), flush=True)
     if not ok:
         sys.exit(1)
+
+
+def coastal_forcing_tail(case, args, plain_ms):
+    """The coastal step with a driver's whole forcing tail attached (infiltration, sponge,
+    IDW gauge ring, clamp, karst drain): a fresh flat solver, one warm-up step, then the same
+    number of timed steps as the plain run."""
+    from geoswe_amd import _lib as L
+    from geoswe_amd.cases import attach_coastal_forcings
+    t0 = time.perf_counter()
+    f = case.flat(track_max_depth=False, sweep_rows=args.sweep_rows)
+    cells = attach_coastal_forcings(case, f)["cells"]
+    log(f"forcing tail attached in {time.perf_counter() - t0:.0f} s: {cells}")
+    timed_steps(f, 1)
+    _, ms = timed_steps(f, args.steps - 1)
+    if f._clockv().status & L.ST_NONFINITE:
+        raise FloatingPointError("non-finite state in the forced coastal run")
+    ms /= args.steps - 1
+    log(f"with the forcing tail: {ms:.3f} ms/step against {plain_ms:.3f} ms/step plain "
+        f"({(ms / plain_ms - 1) * 100:+.1f} %)")
+    return {"ms_per_step": ms, "plain_ms_per_step": plain_ms, "overhead": ms / plain_ms - 1.0, "cells": cells,
+            "kernels": ["gs::k_flat_ring_force<float>", "gs::k_flat_sweep<float, false>"]}
 
 
 def ensemble(args):
@@ -556,6 +579,8 @@ def main():
     ap.add_argument("--checkpoint", type=int, default=100)
     ap.add_argument("--no-fd", action="store_true", help="adjoint: timings only (no finite differences)")
     ap.add_argument("--n", type=int, default=16384, help="coastal box side")
+    ap.add_argument("--forcings", action="store_true",
+                    help="coastal: also time the step with a driver's whole forcing tail attached")
     args = ap.parse_args()
     if args.copy_bytes:
         copy_calibration(args.copy_bytes, args.copy_reps)

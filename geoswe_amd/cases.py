@@ -127,6 +127,76 @@ class CoastalCase:
                                            sweep_rows=sweep_rows)
 
 
+def attach_coastal_forcings(case: CoastalCase, f, band: int = 32, n_clamp: int = 10000, n_drain: int = 5000,
+                            seed: int = 20261207, fields: bool = False) -> dict:
+    """A coastal driver's forcing tail on the flat solver `f` of `case` (before its first
+    step). Sponge: a `band`-cell strip along the box edges relaxing toward still water at
+    eta = 0 (keep = 1 - 0.05 r, amb = 0.05 r max(-b, 0), r rising from 0 to 1 toward the
+    edge). IDW gauge ring: every stored inactive cell beside an active one gets still water
+    at the inverse-square-distance mean of four corner tide gauges (M2 tide, phase lags of
+    0-45 min, sampled every 360 s). Clamp: `n_clamp` seeded active cells capped at 1.5 m.
+    Infiltration: 0-10 mm/h by Manning class. Karst drain: `n_drain` seeded active cells
+    capped at 0.5 m. Returns the number of cells each one touches (``"cells"``) and, with
+    `fields`, the same forcings on the unpadded grid (``"fields"``: keep, amb, ring cells,
+    bed, weights, times and stages, clamp and drain depth fields, +inf where none)."""
+    fm = f.fm
+    g, n = case.mesh.ngh, case.mesh.nx
+    rng = np.random.default_rng(seed)
+    ij = fm.ij.astype(np.int64)
+    ii, jj = ij[:, 0] - g, ij[:, 1] - g
+    inner = (ii >= 0) & (ii < n) & (jj >= 0) & (jj < n)
+    dist = np.minimum(np.minimum(ii, jj), np.minimum(n - 1 - ii, n - 1 - jj))
+    in_band = inner & (dist < band)
+    r = np.where(in_band, (band - dist) / band, 0.0)
+    bed_f = np.zeros(fm.n_stored, np.float32)
+    bed_f[inner] = case.bed[ii[inner], jj[inner]]
+    keep = np.ones((fm.nxp, fm.nyp), np.float32)
+    amb = np.zeros((fm.nxp, fm.nyp), np.float32)
+    keep[ij[in_band, 0], ij[in_band, 1]] = 1.0 - 0.05 * r[in_band]
+    amb[ij[in_band, 0], ij[in_band, 1]] = 0.05 * r[in_band] * np.maximum(-bed_f[in_band], 0.0)
+    f.set_sponge({"keep": keep, "amb": amb})
+    del keep, amb
+    # ring: stored inactive cells with an active 4-neighbour, inside the box
+    act_pad = np.zeros((fm.nxp, fm.nyp), bool)
+    act_pad[ij[fm.active, 0], ij[fm.active, 1]] = True
+    cand = np.nonzero(~fm.active & inner)[0]
+    ci, cj = ij[cand, 0], ij[cand, 1]
+    near = act_pad[ci + 1, cj] | act_pad[ci - 1, cj] | act_pad[ci, cj + 1] | act_pad[ci, cj - 1]
+    ring = cand[near]
+    del act_pad
+    gauges = np.array([[0.0, 0.0], [0.0, n - 1.0], [n - 1.0, 0.0], [n - 1.0, n - 1.0]])
+    d2 = (ii[ring, None] - gauges[None, :, 0]) ** 2 + (jj[ring, None] - gauges[None, :, 1]) ** 2 + 1.0
+    w = 1.0 / d2
+    w /= w.sum(axis=1, keepdims=True)
+    tc = np.arange(-3600.0, 2 * TIDE_PERIOD_S + 3600.0, 360.0)
+    lag = np.array([0.0, 900.0, 1800.0, 2700.0])
+    S = np.sin(2.0 * np.pi * (tc[None, :] - lag[:, None]) / TIDE_PERIOD_S).astype(np.float32)
+    f.set_ring({"i": ij[ring, 0], "j": ij[ring, 1], "bed": bed_f[ring], "wg": w.astype(np.float32), "NG": 4,
+                "n": len(ring), "t_common": tc, "stage_all": S})
+    act_ids = np.nonzero(fm.active)[0]
+    cl = rng.choice(act_ids, size=min(n_clamp, len(act_ids)), replace=False)
+    f.set_clamp({"rows": ij[cl, 0], "cols": ij[cl, 1], "hmax": np.full(len(cl), 1.5)})
+    f.set_infil({"tab": np.linspace(0.0, 10.0, 8) / 3.6e6})
+    dr = rng.choice(act_ids, size=min(n_drain, len(act_ids)), replace=False)
+    f.set_drain({"idx": dr, "h_tgt": 0.5})
+    out = {"cells": {"sponge_band": int(in_band.sum()), "gauge_ring": int(len(ring)), "clamp": int(len(cl)),
+                     "drain": int(len(dr)), "infiltration": int(fm.n_active)}}
+    if fields:
+        kg = np.ones((n, n))
+        ag = np.zeros((n, n))
+        kg[ii[in_band], jj[in_band]] = 1.0 - 0.05 * r[in_band]
+        ag[ii[in_band], jj[in_band]] = 0.05 * r[in_band] * np.maximum(-bed_f[in_band], 0.0)
+        cg = np.full((n, n), np.inf)
+        cg[ii[cl], jj[cl]] = 1.5
+        dg = np.full((n, n), np.inf)
+        dg[ii[dr], jj[dr]] = 0.5
+        out["fields"] = {"keep": kg.astype(np.float32), "amb": ag.astype(np.float32), "clamp": cg, "drain": dg,
+                         "infil": np.linspace(0.0, 10.0, 8) / 3.6e6,
+                         "ring": {"cells": np.stack([ii[ring], jj[ring]], axis=1), "bed": bed_f[ring],
+                                  "w": w.astype(np.float32), "t_common": tc, "stage_all": S}}
+    return out
+
+
 def _taxicab_dilate(mask: np.ndarray, r: int) -> np.ndarray:
     """Cross (4-neighbour) dilation by r iterations = taxicab distance <= r."""
     from scipy.ndimage import distance_transform_cdt

@@ -354,3 +354,33 @@ def test_ensemble_carries_infiltration_and_active_drain(gpu):
         assert p.nsteps == e.nsteps == 20 and p.t == e.t
         assert np.array_equal(p.q_flat(), e.q_flat())
     ens.close()
+
+
+def test_coastal_case_with_its_forcing_tail_matches_oracle(gpu):
+    """The forcing tail the coastal benchmark attaches (infiltration by class, edge sponge,
+    four-gauge IDW tide ring, clamp, drain) on the coastal case at test size, with its
+    gridded rain and tidal stage: 25 flat steps bitwise the oracle's."""
+    from geoswe_amd.cases import attach_coastal_forcings, coastal_c3
+    case = coastal_c3(n=192, band=8, n_sites=24, rain_frames=2)
+    f = case.flat()
+    spec = attach_coastal_forcings(case, f, band=12, n_clamp=300, n_drain=150, fields=True)
+    cells = spec["cells"]
+    assert all(v > 0 for v in cells.values()), cells
+    F = spec["fields"]
+    f.advance(25)
+    sb, rf = case.cfg.stage_boundary, case.cfg.rainfall_forcing
+    oc = O.DenseCase(h=case.q0[0], hu=case.q0[1], hv=case.q0[2], bed=case.bed, dx=3.0, dy=3.0, bc_x="fall",
+                     bc_y="fall", manning=case.ntab.astype(np.float32)[case.cls], rain=(rf.time_s, rf.rate_mm_h),
+                     active=case.active,
+                     stage={"cells": np.argwhere(case.sea_edge), "time_s": sb.time_s, "stage_m": sb.stage_m},
+                     sponge=(F["keep"], F["amb"]), ring=F["ring"], clamp=F["clamp"], drain=F["drain"],
+                     infil=(case.cls, F["infil"]))
+    o = O.DenseOracle(oc, _physics(f.cfg), np.float32)
+    o.cfl_skip = ~case.active
+    o.run_steps(25)
+    assert f.nsteps == o.nsteps == 25 and f.t == o.t
+    g = case.mesh.ngh
+    ij = f.fm.ij - g
+    n = case.mesh.nx
+    inner = (ij[:, 0] >= 0) & (ij[:, 0] < n) & (ij[:, 1] >= 0) & (ij[:, 1] < n)
+    assert np.array_equal(f.q_flat()[:, inner], o.interior[:, ij[inner, 0], ij[inner, 1]])
