@@ -43,7 +43,7 @@ class CompressedSolver:
         self._flags = L.DeviceArray.from_host(fl) if fl.any() else None
         self._flags_base = fl              # without the coastal-forcing bit (set_sponge / set_clamp)
         self._sponge = self._clamp = self._force_ring = self._force_ring_row = self._ring = None
-        self._infil = self._drain = None
+        self._infil = self._drain = self._ga = None
         self._tables_taken = False         # an Ensemble or GaugeMisfit holds this solver's device pointers
         t = np.zeros(L.MAX_CLASSES, self.dtype)
         t[:len(ntab)] = ntab
@@ -256,6 +256,53 @@ class CompressedSolver:
         self._rebuild_force()
         return self
 
+    def set_ga_drain(self, b):
+        """Green-Ampt infiltration and linear-reservoir drain (compressed_solver.py:3214-3215,
+        3282-3291; kernels 320-421): ``{"cls_pad", "Ks_t", "psi_t", "dth_t", "F_pad",
+        "Fmax_pad"?, "inv_tau_pad"?, "mode"}`` with padded-grid fields (soil class, initial
+        cumulative infiltration F, its cap, the drain rate 1/tau) and per-class tables.
+        ``mode``: ``"fused"`` (default: both), ``"ga"`` (infiltration only) or ``"drain"``
+        (drain only). Each step, after the sponge and the gauge ring and before the clamp,
+        on every active cell with h > 0. F is state: checkpoints carry it. The drain's decay
+        uses an exponential built from IEEE operations (bitwise the oracle's) where the
+        reference uses the fast ``__expf``. ``None`` removes it."""
+        self._require_unbuilt("set_ga_drain")
+        new = None
+        if b is not None:
+            fm, dt = self.fm, self.dtype
+            mode = b.get("mode", "fused")
+            if mode not in ("fused", "ga", "drain"):
+                raise ValueError(f"set_ga_drain: mode must be 'fused', 'ga' or 'drain', got {mode!r}")
+            new = {"mode": mode}
+            if mode in ("fused", "ga"):
+                cls = fm.pack(np.asarray(b["cls_pad"]))
+                nc = int(cls.max()) + 1 if cls.size else 1
+                tabs = [np.asarray(b[k], np.float64).ravel() for k in ("Ks_t", "psi_t", "dth_t")]
+                if any(len(t) < nc for t in tabs):
+                    raise ValueError(f"set_ga_drain: the soil classes reach {nc - 1} but Ks_t, psi_t, dth_t have "
+                                     f"{[len(t) for t in tabs]} entries")
+                new["cls"] = L.DeviceArray.from_host(np.ascontiguousarray(cls.astype(np.uint8)))
+                new["ks"], new["psi"], new["dth"] = (L.DeviceArray.from_host(np.ascontiguousarray(t.astype(dt)))
+                                                     for t in tabs)
+                F0 = b.get("F_pad")
+                F = fm.pack(np.asarray(F0)).astype(dt) if F0 is not None else np.zeros(fm.n_stored, dt)
+                new["F"] = L.DeviceArray.from_host(np.ascontiguousarray(F))
+                if b.get("Fmax_pad") is not None:
+                    new["fmax"] = L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(np.asarray(b["Fmax_pad"]))
+                                                                               .astype(dt)))
+            if mode in ("fused", "drain") and b.get("inv_tau_pad") is not None:
+                new["inv_tau"] = L.DeviceArray.from_host(
+                    np.ascontiguousarray(fm.pack(np.asarray(b["inv_tau_pad"])).astype(dt)))
+        self._ga = new
+        return self
+
+    def ga_F(self) -> Optional[np.ndarray]:
+        """The Green-Ampt cumulative infiltration per stored cell (None without Green-Ampt)."""
+        if self._ga is None or "F" not in self._ga:
+            return None
+        L.sync(self.stream)
+        return self._ga["F"].download()
+
     def _flat_ids(self, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
         """Flat id of each padded (row, col), -1 where the cell is not stored."""
         fm = self.fm
@@ -287,7 +334,7 @@ class CompressedSolver:
 
     @property
     def has_coastal_forcing(self) -> bool:
-        return any(f is not None for f in (self._sponge, self._clamp, self._ring, self._infil, self._drain))
+        return any(f is not None for f in (self._sponge, self._clamp, self._ring, self._infil, self._drain, self._ga))
 
     # ------------------------------------------------------------------ structs
     def _phys(self) -> L.GsPhys:
@@ -319,6 +366,9 @@ class CompressedSolver:
         f.clamp = self._clamp[0].ptr if self._clamp is not None else None
         f.infil = self._infil.ptr if self._infil is not None else None
         f.drain = self._drain[0].ptr if self._drain is not None else None
+        ga = self._ga or {}
+        f.ga_cls, f.ga_ks, f.ga_psi, f.ga_dth, f.ga_F, f.ga_fmax, f.ga_inv_tau = (
+            ga[k].ptr if k in ga else None for k in ("cls", "ks", "psi", "dth", "F", "fmax", "inv_tau"))
         f.n_force_ring = self._force_ring.shape[0] if self._force_ring is not None else 0
         f.force_ring = self._force_ring.ptr if self._force_ring is not None else None
         r = self._ring
@@ -404,8 +454,9 @@ class CompressedSolver:
     def save_checkpoint(self, ckpt_dir: str, fidx: int = 0, next_frame: Optional[float] = None) -> None:
         """Write the current state to ``ckpt_dir`` in the reference's single-rank layout
         (compressed_solver.py:1789-1880): ``ckpt_r00.npz`` holding ``t``, ``steps``, ``q0``,
-        ``q1``, ``q2`` (flat arrays in the mesh's stored order) and ``max_h`` when the
-        running maximum is tracked, published by an atomic rename, then ``ckpt_meta.json``
+        ``q1``, ``q2`` (flat arrays in the mesh's stored order), ``max_h`` when the running
+        maximum is tracked and ``F`` with Green-Ampt, published by an atomic rename, then
+        ``ckpt_meta.json``
         with ``t``, ``steps``, ``fidx``, ``next_frame`` and ``nranks``."""
         os.makedirs(ckpt_dir, exist_ok=True)
         c = self._clockv()
@@ -413,6 +464,8 @@ class CompressedSolver:
         arrays = {"t": np.float64(c.t), "steps": np.int64(c.nsteps), "q0": q[0], "q1": q[1], "q2": q[2]}
         if self._max_h is not None:
             arrays["max_h"] = self._max_h.download()
+        if self._ga is not None and "F" in self._ga:
+            arrays["F"] = self._ga["F"].download()
         tmp = os.path.join(ckpt_dir, "ckpt_r00_tmp")
         np.savez(tmp, **arrays)
         os.replace(tmp + ".npz", os.path.join(ckpt_dir, "ckpt_r00.npz"))
@@ -453,6 +506,8 @@ class CompressedSolver:
             a = np.ascontiguousarray(z[f"q{k}"], dtype=self.dtype)
             for par in (0, 1):
                 self._bufs[par][k].upload(a, self.stream)
+        if self._ga is not None and "F" in self._ga and "F" in z.files:
+            self._ga["F"].upload(np.ascontiguousarray(z["F"], dtype=self.dtype), self.stream)
         if self._max_h is not None:
             if "max_h" in z.files:
                 self._max_h.upload(np.ascontiguousarray(z["max_h"], dtype=self.dtype), self.stream)

@@ -59,6 +59,13 @@ struct FlatK {
   const T* clamp;   // nullable [N] h_max of GS_FLAG_FORCE cells
   const T* infil;   // nullable [32] infiltration rate per Manning class
   const T* drain;   // nullable [N] karst target depth of GS_FLAG_FORCE cells
+  const uint8_t* ga_cls;  // Green-Ampt / drain-tau (nullable tables, see the ABI header)
+  const T* ga_ks;
+  const T* ga_psi;
+  const T* ga_dth;
+  T* ga_F;
+  const T* ga_fmax;
+  const T* ga_inv_tau;
   int64_t n_fring;  // forced cells of the ring (stored, not active): sponge, clamp, gauge ring
   const int64_t* fring;
   const int32_t* fring_row;  // nullable: gauge-ring row of each entry, -1 for none
@@ -108,6 +115,13 @@ struct FlatCtx {
   const GS_G T* clamp;
   const GS_G T* infil;
   const GS_G T* drain;
+  const GS_G uint8_t* ga_cls;
+  const GS_G T* ga_ks;
+  const GS_G T* ga_psi;
+  const GS_G T* ga_dth;
+  GS_G T* ga_F;
+  const GS_G T* ga_fmax;
+  const GS_G T* ga_inv_tau;
   const GS_G T* frame;   // gridded rain frame of this step (rain_mode 2)
   const T* ntab;         // Manning n per class (LDS copy or the member's table)
   int64_t n, rain_pitch;
@@ -198,6 +212,13 @@ __device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k,
   x.clamp = gp(a.clamp);
   x.infil = gp(a.infil);
   x.drain = gp(a.drain);
+  x.ga_cls = gp(a.ga_cls);
+  x.ga_ks = gp(a.ga_ks);
+  x.ga_psi = gp(a.ga_psi);
+  x.ga_dth = gp(a.ga_dth);
+  x.ga_F = gp(a.ga_F);
+  x.ga_fmax = gp(a.ga_fmax);
+  x.ga_inv_tau = gp(a.ga_inv_tau);
   x.frame = k.frame;
   x.ntab = a.ntab;
   x.n = a.n;
@@ -264,13 +285,91 @@ __device__ __forceinline__ void infil_cell(T rate, T dt, T hmin, T& h, T& hu, T&
     hv = hv * r;
   }
 }
-// The forcings of a GS_FLAG_FORCE cell in order: sponge, (gauge ring: the ring pass), clamp,
-// karst drain.
+// exp(x) in float64 from IEEE +, -, *, rint and ldexp only, so that it is the oracle's
+// exp_ieee bit for bit: Cody-Waite reduction x = k ln2 + r (ln2 split in a 32-bit head, so
+// k*LN2_HI is exact, and a tail), exp(r) on |r| <= ln2/2 by the degree-13 Taylor polynomial in
+// Horner form, scaled by 2^k. It replaces the fast exponential of the reference's drain decay.
+__device__ __forceinline__ double exp_ieee(double x) {
+  if (!(x == x)) return x;
+  if (x > 709.8) return __builtin_inf();
+  if (x < -745.2) return 0.0;
+  const double kf = rint(x * 1.44269504088896338700e+00);
+  const double r = (x - kf * 6.93147180369123816490e-01) - kf * 1.90821492927058770002e-10;
+  double p = 1.6059043836821613e-10;
+  p = p * r + 2.08767569878681e-09;
+  p = p * r + 2.505210838544172e-08;
+  p = p * r + 2.755731922398589e-07;
+  p = p * r + 2.7557319223985893e-06;
+  p = p * r + 2.48015873015873e-05;
+  p = p * r + 0.0001984126984126984;
+  p = p * r + 0.001388888888888889;
+  p = p * r + 0.008333333333333333;
+  p = p * r + 0.041666666666666664;
+  p = p * r + 0.16666666666666666;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  return ldexp(p, (int)kf);
+}
+
+// Green-Ampt infiltration then drain-tau of one active cell (ga_drain_flat); `ob` is the
+// cell's offset in a one-byte plane.
 template <typename T>
-__device__ __forceinline__ void coastal_force(const FlatCtx<T>& x, int64_t base, uint32_t off, T& h, T& hu, T& hv) {
-  if (x.sponge) sponge_cell<T>(gat(x.sponge + 2 * base, 2 * off), gat(x.sponge + 2 * base + 1, 2 * off), h, hu, hv);
-  if (x.clamp) clamp_cell<T>(gat(x.clamp + base, off), h, hu, hv);
-  if (x.drain) drain_cell<T>(gat(x.drain + base, off), h, hu, hv);
+__device__ __forceinline__ void ga_cell(const FlatCtx<T>& x, int64_t base, uint32_t off, uint32_t ob, T& h, T& hu,
+                                        T& hv) {
+  if (h <= T(0)) return;
+  T scale = T(1);
+  if (x.ga_ks) {
+    const int c = gat(x.ga_cls + base, ob);
+    const T K = x.ga_ks[c];
+    if (K > T(0)) {
+      const T KsDt = K * x.dt;
+      const T head = x.ga_psi[c] + h;
+      GS_G T& Fr = gat(x.ga_F + base, off);
+      const T F0 = Fr;
+      const T a = F0 + KsDt;
+      const T disc = a * a + ((T(4) * KsDt) * head) * x.ga_dth[c];
+      const T F1 = T(0.5) * (a + sqrt(disc > T(0) ? disc : T(0)));   // fmax(disc, 0): NaN gives 0
+      T dF = F1 - F0;
+      dF = dF > T(0) ? dF : T(0);
+      if (dF > h) dF = h;
+      if (x.ga_fmax) {
+        const T room = gat(x.ga_fmax + base, off) - F0;
+        if (dF > room) dF = room > T(0) ? room : T(0);
+      }
+      const T hn = h - dF;
+      scale = scale * (hn / h);
+      h = hn;
+      Fr = F0 + dF;
+    }
+  }
+  if (x.ga_inv_tau) {
+    const T it = gat(x.ga_inv_tau + base, off);
+    if (it > T(0) && h > T(0)) {
+      const T decay = (T)exp_ieee((double)((-x.dt) * it));
+      scale = scale * decay;
+      h = h * decay;
+    }
+  }
+  hu = hu * scale;
+  hv = hv * scale;
+}
+
+// The forcings after the running max, in the reference's order: infiltration (every active
+// cell, by Manning class), the sponge of a GS_FLAG_FORCE cell, (the gauge ring: ring cells
+// only, in the ring pass), Green-Ampt and drain-tau (every active cell), then the clamp and
+// the karst drain of a GS_FLAG_FORCE cell. GA = false leaves Green-Ampt out of the code (the
+// plain sweep's instantiation, which has no register to spare for it).
+template <typename T, bool GA>
+__device__ __forceinline__ void active_force(const FlatCtx<T>& x, uint8_t fg, int64_t base, uint32_t off,
+                                             uint32_t ob, T& h, T& hu, T& hv) {
+  if (x.infil) infil_cell<T>(x.infil[fg & GS_FLAG_CLASS_MASK], x.dt, x.P.h_min, h, hu, hv);
+  const bool fc = (fg & GS_FLAG_FORCE) != 0;
+  if (fc && x.sponge)
+    sponge_cell<T>(gat(x.sponge + 2 * base, 2 * off), gat(x.sponge + 2 * base + 1, 2 * off), h, hu, hv);
+  if (GA && (x.ga_ks || x.ga_inv_tau)) ga_cell<T>(x, base, off, ob, h, hu, hv);
+  if (fc && x.clamp) clamp_cell<T>(gat(x.clamp + base, off), h, hu, hv);
+  if (fc && x.drain) drain_cell<T>(gat(x.drain + base, off), h, hu, hv);
 }
 
 template <typename T, int MODE>
@@ -374,8 +473,7 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
     hv = T(0);
   }
   if (a.max_h && h > a.max_h[k]) a.max_h[k] = h;
-  if (x.infil) infil_cell<T>(x.infil[fg & GS_FLAG_CLASS_MASK], x.dt, P.h_min, h, hu, hv);
-  if (fg & GS_FLAG_FORCE) coastal_force<T>(x, k, 0u, h, hu, hv);
+  active_force<T, true>(x, fg, k, 0u, 0u, h, hu, hv);
   x.qo0[k] = h;
   x.qo1[k] = hu;
   x.qo2[k] = hv;
@@ -428,7 +526,7 @@ __device__ __forceinline__ T sweep_b(const FlatCtx<T>& x, int64_t base, uint32_t
 // flat_chunk), so all three traversals give the same bits. Every lane of the wavefront
 // must call it. PRE: also write the pre-friction state (the adjoint's recomputation); a
 // separate instantiation keeps those pointers out of the plain step's scalar registers.
-template <typename T, bool PRE>
+template <typename T, bool PRE, bool GA>
 __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int lane, uint64_t& lam) {
   const Phys<T>& P = x.P;
   const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
@@ -534,8 +632,7 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
         hv = T(0);
       }
       if (x.max_h && h > gat(x.max_h + gC, oC)) gat(x.max_h + gC, oC) = h;
-      if (x.infil) infil_cell<T>(x.infil[fg & GS_FLAG_CLASS_MASK], x.dt, P.h_min, h, hu, hv);
-      if (fg & GS_FLAG_FORCE) coastal_force<T>(x, gC, oC, h, hu, hv);
+      active_force<T, GA>(x, fg, gC, oC, ob, h, hu, hv);
       gat(x.qo0 + gC, oC) = h;
       gat(x.qo1 + gC, oC) = hu;
       gat(x.qo2 + gC, oC) = hv;
@@ -600,7 +697,28 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(sizeof(
   const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
   uint64_t lam = 0ull;
   for (int64_t t = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv; t < a.n_tiles; t += nwaves)
-    flat_tile<T, PRE>(x, t, lane, lam);
+    flat_tile<T, PRE, false>(x, t, lane, lam);
+  grid_reduce_and_plan(lam, a.clock, a.plan, true);
+}
+
+// The sweep with Green-Ampt infiltration and drain-tau compiled in (launched only when
+// their tables are set).
+template <typename T>
+__global__ __launch_bounds__(kFBlock) void k_flat_sweep_ga(FlatK<T> a) {
+  const FlatClk<T> ck = flat_clock<T, 1>(a);
+  if (!ck.live) return;
+  FlatCtx<T> x;
+  flat_ctx<T, 1>(a, ck, x);
+  __shared__ T s_ntab[32];
+  if (threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
+  __syncthreads();
+  x.ntab = s_ntab;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
+  uint64_t lam = 0ull;
+  for (int64_t t = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv; t < a.n_tiles; t += nwaves)
+    flat_tile<T, false, true>(x, t, lane, lam);
   grid_reduce_and_plan(lam, a.clock, a.plan, true);
 }
 
@@ -695,6 +813,8 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   if (f->n_tiles < 0) return fail(GS_E_ARG, "n_tiles < 0");
   if (f->n_tiles > 0 && (!f->tile_e0 || !f->tile_irc || !f->ent)) return fail(GS_E_ARG, "tile tables are NULL");
   if (f->n_force_ring < 0 || f->n_force_ring > f->n_stored) return fail(GS_E_RANGE, "n_force_ring out of range");
+  if (f->ga_ks && (!f->ga_cls || !f->ga_psi || !f->ga_dth || !f->ga_F))
+    return fail(GS_E_ARG, "Green-Ampt needs the soil class, psi, dth and F arrays");
   if (f->n_force_ring > 0 && !f->force_ring) return fail(GS_E_ARG, "force_ring is NULL");
   if (f->n_force_ring > 0 && !f->sponge && !f->clamp && !f->drain && !f->force_ring_row)
     return fail(GS_E_ARG, "force_ring lists cells but no sponge, clamp, drain or gauge ring is set");
@@ -740,6 +860,13 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   a.clamp = (const T*)f->clamp;
   a.infil = (const T*)f->infil;
   a.drain = (const T*)f->drain;
+  a.ga_cls = f->ga_cls;
+  a.ga_ks = (const T*)f->ga_ks;
+  a.ga_psi = (const T*)f->ga_psi;
+  a.ga_dth = (const T*)f->ga_dth;
+  a.ga_F = (T*)f->ga_F;
+  a.ga_fmax = (const T*)f->ga_fmax;
+  a.ga_inv_tau = (const T*)f->ga_inv_tau;
   a.n_fring = f->n_force_ring;
   a.fring = f->force_ring;
   a.fring_row = f->force_ring_row;
@@ -789,6 +916,12 @@ static unsigned sweep_grid(int64_t ntiles) {
   return resident_grid(k_flat_sweep<T, PRE>, cache, ntiles);
 }
 
+template <typename T>
+static unsigned sweep_ga_grid(int64_t ntiles) {
+  static std::atomic<int> cache[64];
+  return resident_grid(k_flat_sweep_ga<T>, cache, ntiles);
+}
+
 // The fused step: row-sweep tiles when the descriptor has them, else chunks of the run table.
 template <typename T>
 static int flat_step_launch(const FlatK<T>& a, hipStream_t s) {
@@ -800,6 +933,9 @@ static int flat_step_launch(const FlatK<T>& a, hipStream_t s) {
   if (a.n_tiles > 0 && a.pre[0]) {
     hipLaunchKernelGGL((k_flat_sweep<T, true>), dim3(sweep_grid<T, true>(a.n_tiles)), dim3(kFBlock), 0, s, a);
     GS_HIP(hipGetLastError(), "k_flat_sweep(pre) launch");
+  } else if (a.n_tiles > 0 && (a.ga_ks || a.ga_inv_tau)) {
+    hipLaunchKernelGGL((k_flat_sweep_ga<T>), dim3(sweep_ga_grid<T>(a.n_tiles)), dim3(kFBlock), 0, s, a);
+    GS_HIP(hipGetLastError(), "k_flat_sweep_ga launch");
   } else if (a.n_tiles > 0) {
     hipLaunchKernelGGL((k_flat_sweep<T, false>), dim3(sweep_grid<T, false>(a.n_tiles)), dim3(kFBlock), 0, s, a);
     GS_HIP(hipGetLastError(), "k_flat_sweep launch");
@@ -925,7 +1061,7 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>*
       // it stays a register temporary
       FlatCtx<T> x;
       flat_ctx<T, 1>(mem[m], ck, x);
-      flat_tile<T, false>(x, gc - c0, lane, lam);
+      flat_tile<T, false, true>(x, gc - c0, lane, lam);
     }
   }
   ens_flush(e, m, lam);

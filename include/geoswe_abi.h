@@ -187,6 +187,21 @@ typedef struct GsFlat {
    * r = h > 1e-12 ? h_tgt/h : 0, h = h_tgt, hu, hv scaled by r. */
   const void* infil;         /* nullable [32] rate per Manning class (m/s), state dtype */
   const void* drain;         /* nullable [N] h_tgt, state dtype (+inf: no drain) */
+  /* Green-Ampt infiltration and drain-tau (ga_drain_flat, compressed_solver.py:320-367; its ga
+   * and drain modes, 369-421, are this with one part NULL) on every active cell with h > 0,
+   * after the sponge and before the clamp: with K = ks[c] > 0, a = F + K dt,
+   * disc = a*a + ((4 K dt)(psi[c] + h)) dth[c], F1 = (a + sqrt(max(disc, 0)))/2,
+   * dF = min(max(F1 - F, 0), h), capped by the room fmax - F (0 when none), h -= dF, F += dF,
+   * scale = h'/h; then with it = inv_tau > 0 and h > 0, decay = exp(-dt it) from IEEE
+   * operations only (in float64, rounded to the state dtype), scale *= decay, h *= decay;
+   * hu, hv *= scale. */
+  const uint8_t* ga_cls;     /* [N] soil class, needed with ga_ks */
+  const void* ga_ks;         /* nullable [classes] Ks (m/s), state dtype: no Green-Ampt when NULL */
+  const void* ga_psi;        /* [classes] suction head (m) */
+  const void* ga_dth;        /* [classes] moisture deficit */
+  void* ga_F;                /* [N] cumulative infiltration (m), state dtype, updated in place */
+  const void* ga_fmax;       /* nullable [N] storage cap of F */
+  const void* ga_inv_tau;    /* nullable [N] drain rate 1/tau (1/s): no drain when NULL */
   /* GS_FLAG_FORCE cells of the ring (stored, not active): the step writes
    * force(state) for them, as the reference's gathered kernels touch every stored cell
    * of the band; the residual and the CFL never see them. */

@@ -24,6 +24,7 @@ Sources followed (paths relative to the reference checkout):
 * open-boundary sponge, stage clamp .... compressed_solver.py:1320-1326, 424-440
 * IDW gauge ring ....................... compressed_solver.py:287-307, 2338-2348
 * infiltration, karst drain ............ compressed_solver.py:143-158, 269-286
+* Green-Ampt infiltration, drain-tau ... compressed_solver.py:320-421
 
 Documented departures (shared with the HIP kernels, which implement the same rule):
 
@@ -496,6 +497,97 @@ def stage_at_time(time_s, stage_m, t: float) -> float:
 # a dense run (Solver2D.run/step, solver.py:2695-3003)
 # ---------------------------------------------------------------------------
 
+# exp(r) on |r| <= ln2/2 by Horner's rule on the Taylor coefficients 1/n!, n = 13 .. 0
+# (the truncation term r^14/14! is below 4e-18 relative)
+_EXP_C = (1.6059043836821613e-10, 2.08767569878681e-09, 2.505210838544172e-08, 2.755731922398589e-07,
+          2.7557319223985893e-06, 2.48015873015873e-05, 0.0001984126984126984, 0.001388888888888889,
+          0.008333333333333333, 0.041666666666666664, 0.16666666666666666, 0.5, 1.0, 1.0)
+_LN2_HI, _LN2_LO, _INV_LN2 = 6.93147180369123816490e-01, 1.90821492927058770002e-10, 1.44269504088896338700e+00
+
+
+def exp_ieee(x) -> np.ndarray:
+    """``exp(x)`` in float64 from IEEE +, -, *, round-to-nearest-integer and ``ldexp`` only, so
+    CPU and GPU agree bit for bit: Cody-Waite reduction ``x = k ln2 + r`` with ln2 split in a
+    32-bit head and a tail (k ln2_hi is exact), ``exp(r)`` by a degree-13 Taylor polynomial in
+    Horner form, scaled by ``2^k``. 0 below -745.2, +inf above 709.8, NaN kept. Stands in for
+    the reference's fast ``__expf`` in the drain-tau decay (compressed_solver.py:355-356,
+    417), which differs by a few ulp. Pinned against ``numpy.exp`` (<= 2 ulp).
+    """
+    x = np.asarray(x, np.float64)
+    out = np.empty_like(x)
+    big, small, nan = x > 709.8, x < -745.2, x != x
+    mid = ~(big | small | nan)
+    xm = x[mid]
+    kf = np.rint(xm * _INV_LN2)
+    r = (xm - kf * _LN2_HI) - kf * _LN2_LO
+    p = np.full_like(r, _EXP_C[0])
+    for c in _EXP_C[1:]:
+        p = p * r + c
+    out[mid] = np.ldexp(p, kf.astype(np.int64))
+    out[big] = np.inf
+    out[small] = 0.0
+    out[nan] = x[nan]
+    return out
+
+
+def green_ampt_drain(q_int: np.ndarray, F: np.ndarray, ga: dict, dt: float, active=None) -> None:
+    """Fused Green-Ampt infiltration and linear-reservoir drain (ga_drain_flat,
+    compressed_solver.py:320-367), in place on a (3, nx, ny) state and the cumulative
+    infiltration ``F``. ``ga``: ``cls`` (nx, ny) soil class, tables ``Ks``, ``psi``, ``dth``
+    (None: no Green-Ampt), ``Fmax`` (nx, ny) or None (no cap), ``inv_tau`` (nx, ny) or None (no
+    drain). On active cells with ``h > 0``: with ``K = Ks[c] > 0``, ``a = F0 + K dt``,
+    ``disc = a*a + ((4 K dt) (psi[c] + h)) dth[c]``, ``F1 = (a + sqrt(max(disc, 0)))/2``,
+    ``dF = min(max(F1 - F0, 0), h)`` capped by the room ``Fmax - F0`` (0 when none left),
+    ``h <- h - dF``, ``F <- F0 + dF``, ``scale = h'/h``; then with ``it = inv_tau > 0`` and
+    ``h > 0``: ``decay = exp_ieee(-dt*it)`` (state dtype), ``scale *= decay``,
+    ``h *= decay``; momentum times ``scale``. The ``ga`` and ``drain`` modes of the reference
+    (ga_flat, drain_tau_flat, 369-421) are this with one part left out, bit for bit."""
+    dtp = q_int.dtype
+    T = dtp.type
+    h, hu, hv = q_int[0], q_int[1], q_int[2]
+    on = ~(h <= 0)
+    if active is not None:
+        on &= np.asarray(active, bool)
+    dtv = T(dt)
+    hh = h[on]
+    scale = np.ones_like(hh)
+    if ga.get("Ks") is not None:
+        c = np.asarray(ga["cls"])[on]
+        K = np.asarray(ga["Ks"]).astype(dtp)[c]
+        psi = np.asarray(ga["psi"]).astype(dtp)[c]
+        dth = np.asarray(ga["dth"]).astype(dtp)[c]
+        F0 = F[on]
+        g = K > 0
+        KsDt = K * dtv
+        head = psi + hh
+        a = F0 + KsDt
+        disc = a * a + ((T(4.0) * KsDt) * head) * dth
+        F1 = T(0.5) * (a + np.sqrt(np.fmax(disc, T(0))))
+        dF = F1 - F0
+        dF = np.where(dF > 0, dF, T(0))
+        dF = np.where(dF > hh, hh, dF)
+        Fmax = ga.get("Fmax")
+        if Fmax is not None:
+            room = np.asarray(Fmax).astype(dtp)[on] - F0
+            dF = np.where(dF > room, np.where(room > 0, room, T(0)), dF)
+        h_new = hh - dF
+        with np.errstate(invalid="ignore", divide="ignore"):
+            scale = np.where(g, scale * (h_new / hh), scale)
+        hh = np.where(g, h_new, hh)
+        Fn = F0.copy()
+        Fn[g] = F0[g] + dF[g]
+        F[on] = Fn
+    if ga.get("inv_tau") is not None:
+        it = np.asarray(ga["inv_tau"]).astype(dtp)[on]
+        d = (it > 0) & (hh > 0)
+        decay = exp_ieee(((-dtv) * it[d]).astype(np.float64)).astype(dtp)
+        scale[d] = scale[d] * decay
+        hh[d] = hh[d] * decay
+    h[on] = hh
+    hu[on] = hu[on] * scale
+    hv[on] = hv[on] * scale
+
+
 def ring_depth(ring: dict, t: float, dtype) -> np.ndarray:
     """Still-water depth of each gauge-ring cell at time ``t`` (the step's start time).
 
@@ -521,7 +613,7 @@ def ring_depth(ring: dict, t: float, dtype) -> np.ndarray:
 
 
 def coastal_force(q_int: np.ndarray, sponge=None, clamp=None, ring=None, t: float = 0.0, drain=None,
-                  infil=None) -> None:
+                  infil=None, ga=None) -> None:
     """Infiltration, open-boundary sponge, gauge ring, stage clamp, karst drain, in place on a
     (3, nx, ny) state.
 
@@ -538,7 +630,8 @@ def coastal_force(q_int: np.ndarray, sponge=None, clamp=None, ring=None, t: floa
     ``h_max`` that is neither +inf nor NaN. A cell named by either list gets both (the
     identity sponge ``h*1 + 0`` included), as the fused step applies them. ``ring``
     (``{"cells": (n, 2) ij, "bed", "w": (n, NG), "t_common", "stage_all": (NG, nt)}``) sets
-    its cells to still water at :func:`ring_depth` of time ``t`` between the two. The cells
+    its cells to still water at :func:`ring_depth` of time ``t`` between the two; ``ga`` =
+    ``(spec, F, dt, active)`` runs :func:`green_ampt_drain` after the ring. The cells
     named by the sponge band, the clamp or the drain are the forced cells: each gets the
     sponge (if any), the clamp (if any) and the drain (if any).
     """
@@ -576,6 +669,8 @@ def coastal_force(q_int: np.ndarray, sponge=None, clamp=None, ring=None, t: floa
         h[ii, jj] = ring_depth(ring, t, dt)
         hu[ii, jj] = 0
         hv[ii, jj] = 0
+    if ga is not None:
+        green_ampt_drain(q_int, ga[1], ga[0], ga[2], ga[3])
     if clamp is not None:
         over = cells & (h > hx)
         r = hx[over] / h[over]
@@ -611,12 +706,13 @@ class DenseCase:
     ring: Optional[dict] = None         # gauge ring: see coastal_force
     infil: Optional[tuple] = None       # (class (nx, ny) ints, rate per class m/s)
     drain: Optional[np.ndarray] = None  # (nx, ny) karst target depth, +inf where none
+    ga: Optional[dict] = None           # Green-Ampt / drain-tau: see green_ampt_drain (+ "F" initial)
 
 
 class DenseOracle:
     """Dense time stepping in the reference order: BC fill -> residual -> update -> t += dt
-    -> stage -> infiltration -> sponge -> gauge ring (at the step's start time) -> clamp ->
-    karst drain -> (next) CFL. Works in float32 or float64."""
+    -> stage -> infiltration -> sponge -> gauge ring (at the step's start time) -> Green-Ampt
+    and drain-tau -> clamp -> karst drain -> (next) CFL. Works in float32 or float64."""
 
     def __init__(self, case: DenseCase, ph: Physics, dtype=np.float32):
         self.case, self.ph = case, ph
@@ -635,6 +731,10 @@ class DenseOracle:
         self.t = 0.0
         self.nsteps = 0
         self.dmin = min(case.dx, case.dy)
+        self.ga_F = None
+        if case.ga is not None:
+            F0 = case.ga.get("F")
+            self.ga_F = (np.zeros((nx, ny), self.dtype) if F0 is None else np.asarray(F0).astype(self.dtype))
         self.dts: list = []
 
     @property
@@ -684,11 +784,12 @@ class DenseOracle:
             self.q[0, ii + g, jj + g] = np.maximum(eta - bed, self.dtype.type(0))
             self.q[1, ii + g, jj + g] = 0
             self.q[2, ii + g, jj + g] = 0
-        if any(f is not None for f in (c.sponge, c.clamp, c.ring, c.drain, c.infil)):
+        if any(f is not None for f in (c.sponge, c.clamp, c.ring, c.drain, c.infil, c.ga)):
             inf = None
             if c.infil is not None:
                 inf = (c.infil[0], c.infil[1], dt, ph.h_min, act)
-            coastal_force(self.q[I], c.sponge, c.clamp, c.ring, t0, c.drain, inf)
+            ga = None if c.ga is None else (c.ga, self.ga_F, dt, act)
+            coastal_force(self.q[I], c.sponge, c.clamp, c.ring, t0, c.drain, inf, ga)
         self.nsteps += 1
 
     def next_dt(self, t_end=None, dt_max=None) -> float:

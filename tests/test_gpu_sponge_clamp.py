@@ -384,3 +384,92 @@ def test_coastal_case_with_its_forcing_tail_matches_oracle(gpu):
     n = case.mesh.nx
     inner = (ij[:, 0] >= 0) & (ij[:, 0] < n) & (ij[:, 1] >= 0) & (ij[:, 1] < n)
     assert np.array_equal(f.q_flat()[:, inner], o.interior[:, ij[inner, 0], ij[inner, 1]])
+
+
+def _ga_setup(dtype, mode, seed=41):
+    from geoswe_amd import CompressedSolver
+    q0, bed = rough_state(NX, NY, seed=seed, depth=0.7, rough=1.0, vel=0.3)
+    m = irregular_mask(NX, NY, seed=9)
+    q0 = q0 * m
+    rng = np.random.default_rng(seed)
+    scls = rng.integers(0, 4, (NX, NY)).astype(np.uint8)
+    Ks, psi, dth = np.array([0.0, 2e-5, 8e-5, 3e-4]), np.array([0.05, 0.11, 0.2, 0.3]), np.array([0.2, 0.3, 0.35, 0.4])
+    F0 = (rng.random((NX, NY)) * 0.01) * m
+    Fmax = F0 + np.where(rng.random((NX, NY)) < 0.3, 2e-4, 1.0)
+    inv_tau = np.where(rng.random((NX, NY)) < 0.5, 1.0 / 40.0, 0.0)
+    s = dense(q0, bed, friction="manning", manning_n=0.03, rainfall=30.0 / 3.6e6, dtype=dtype)
+    s.set_inside_mask(m)
+    s.set_cfl_ghost_mask(~m)
+    c = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=True)
+    g = s.mesh.ngh
+    c.set_ga_drain({"cls_pad": np.pad(scls, g), "Ks_t": Ks, "psi_t": psi, "dth_t": dth, "F_pad": np.pad(F0, g),
+                    "Fmax_pad": np.pad(Fmax, g), "inv_tau_pad": np.pad(inv_tau, g), "mode": mode})
+    ga = {"cls": scls, "Ks": None if mode == "drain" else Ks, "psi": psi, "dth": dth, "F": F0,
+          "Fmax": None if mode == "drain" else Fmax, "inv_tau": None if mode == "ga" else inv_tau}
+    case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=bed, dx=2.0, dy=2.0, bc_x="fall", bc_y="fall",
+                       manning=0.03, rain=30.0 / 3.6e6, active=m, ga=ga)
+    return c, g, m, case
+
+
+@pytest.mark.parametrize("dtype,trav,mode", [("float32", "tiles", "fused"), ("float64", "tiles", "fused"),
+                                             ("float32", "chunks", "fused"), ("float64", "chunks", "fused"),
+                                             ("float32", "tiles", "ga"), ("float32", "tiles", "drain")])
+def test_green_ampt_drain_matches_oracle(gpu, dtype, trav, mode, monkeypatch):
+    """Green-Ampt infiltration with a storage cap and drain-tau on an irregular domain: flat
+    state of every stored interior cell, the cumulative infiltration F, the running maximum,
+    steps and time equal the oracle's bit for bit (the drain's exponential is the IEEE-only
+    one on both sides)."""
+    c, g, m, case = _ga_setup(dtype, mode)
+    c.traversal = trav
+    n = c.run(T_END)
+    mx = np.zeros((NX, NY), np.dtype(dtype))
+    orig = O.coastal_force
+
+    def recording(q, *args):
+        np.maximum(mx, q[0], out=mx)
+        orig(q, *args)
+
+    monkeypatch.setattr(O, "coastal_force", recording)
+    o = O.DenseOracle(case, _physics(c.cfg), np.dtype(dtype))
+    o.cfl_skip = ~m
+    o.run(T_END)
+    print(dtype, trav, mode, n, flush=True)
+    assert o.nsteps == c.nsteps == n > 5 and o.t == c.t
+    ij = c.fm.ij - g
+    inner = (ij[:, 0] >= 0) & (ij[:, 0] < NX) & (ij[:, 1] >= 0) & (ij[:, 1] < NY)
+    qf = c.q_flat()
+    assert np.array_equal(qf[:, inner], o.interior[:, ij[inner, 0], ij[inner, 1]])
+    act = c.fm.active
+    ia = ij[act]
+    assert np.array_equal(c.max_depth()[ia[:, 0], ia[:, 1]], mx[ia[:, 0], ia[:, 1]])
+    if mode != "drain":
+        F = c.ga_F()
+        assert np.array_equal(F[act], o.ga_F[ia[:, 0], ia[:, 1]])
+        assert (F[act] > np.asarray(case.ga["F"])[ia[:, 0], ia[:, 1]]).any()       # water went into the soil
+
+
+def test_green_ampt_resume_and_ensemble(gpu, tmp_path):
+    """F is state: a run stopped at an odd step and resumed from its checkpoint ends bitwise
+    where the uninterrupted run ends (state and F); ensemble members with Green-Ampt equal
+    their plain runs."""
+    from geoswe_amd.ensemble import Ensemble
+    mk = lambda: _ga_setup("float32", "fused", seed=42)[0]
+    a = mk()
+    a.run(6.0, batch=8)
+    d = str(tmp_path / "ckpt")
+    b = mk()
+    b.run(6.0, max_steps=13, batch=4, ckpt_dir=d, checkpoint_every_s=1.0)
+    assert b.nsteps == 13 and a.nsteps > 13
+    c = mk()
+    c.run(6.0, batch=8, ckpt_dir=d, resume=True)
+    assert c.nsteps == a.nsteps and np.array_equal(c.q_flat(), a.q_flat()) and np.array_equal(c.ga_F(), a.ga_F())
+    plain = [mk(), mk()]
+    for p in plain:
+        p.advance(15)
+    mem = [mk(), mk()]
+    ens = Ensemble(mem, graph_batch=8)
+    ens.advance(15)
+    for p, e in zip(plain, mem):
+        assert p.nsteps == e.nsteps == 15
+        assert np.array_equal(p.q_flat(), e.q_flat()) and np.array_equal(p.ga_F(), e.ga_F())
+    ens.close()

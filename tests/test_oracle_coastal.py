@@ -243,3 +243,77 @@ def test_drain_caps_with_its_tiny_depth_guard_after_the_clamp():
     once = np.float32(hu * np.float32(np.float32(0.5) / h))
     twice = np.float32(np.float32(hu * np.float32(np.float32(0.8) / h)) * np.float32(np.float32(0.5) / np.float32(0.8)))
     assert once != twice and q[1, 0, 0] == once and q[0, 0, 0] == np.float32(0.5)
+
+
+def test_exp_ieee_within_two_ulp_of_numpy_exp():
+    rng = np.random.default_rng(11)
+    x = np.concatenate([np.linspace(-745.0, 709.0, 200001), rng.uniform(-50.0, 0.0, 100000),
+                        rng.uniform(-1e-3, 1e-3, 10000), [0.0, -0.0, 1.0, -1.0, np.log(2.0)]])
+    got, want = O.exp_ieee(x), np.exp(x)
+    ok = want > 0
+    ulps = np.abs(got[ok] - want[ok]) / np.spacing(want[ok])
+    assert ulps.max() <= 2.0, ulps.max()
+    assert O.exp_ieee(np.array([0.0]))[0] == 1.0
+    sp = O.exp_ieee(np.array([-np.inf, -800.0, np.inf, 800.0, np.nan]))
+    assert sp[0] == 0 and sp[1] == 0 and sp[2] == np.inf and sp[3] == np.inf and np.isnan(sp[4])
+
+
+def _ga_state(dtype=np.float64):
+    rng = np.random.default_rng(21)
+    nx, ny = 6, 5
+    h = (rng.random((nx, ny)) * 0.3 + 0.01).astype(dtype)
+    hu = ((rng.random((nx, ny)) - 0.5) * h).astype(dtype)
+    hv = ((rng.random((nx, ny)) - 0.5) * h).astype(dtype)
+    return np.stack([h, hu, hv]), rng.integers(0, 3, (nx, ny)), (rng.random((nx, ny)) * 0.02).astype(dtype)
+
+
+def test_green_ampt_step_solves_its_quadratic_and_conserves_water():
+    q, cls, F = _ga_state()
+    q0, F0 = q.copy(), F.copy()
+    Ks, psi, dth, dt = np.array([1e-5, 3e-6, 0.0]), np.array([0.11, 0.05, 0.2]), np.array([0.3, 0.25, 0.4]), 2.0
+    O.green_ampt_drain(q, F, {"cls": cls, "Ks": Ks, "psi": psi, "dth": dth}, dt)
+    K, head = Ks[cls], psi[cls] + q0[0]
+    a = F0 + K * dt
+    F1 = F                                                      # no cap, dF < h here
+    g = K > 0
+    assert np.allclose((F1 * (F1 - a))[g], (K * dt * head * dth[cls])[g], rtol=1e-10)
+    assert np.allclose(q[0] + F, q0[0] + F0, rtol=1e-14, atol=0)          # water moves into the soil
+    assert np.array_equal(q[:, ~g], q0[:, ~g]) and np.array_equal(F[~g], F0[~g])   # K = 0: untouched
+    for m in (1, 2):
+        assert np.allclose(q[m] / q[0], q0[m] / q0[0], rtol=1e-14)
+
+
+def test_green_ampt_caps_and_drain_tau_decay():
+    q, cls, F = _ga_state()
+    q0, F0 = q.copy(), F.copy()
+    Fmax = F0 + 1e-7
+    Fmax[0, 0] = F0[0, 0] - 1.0                                # no room left: nothing infiltrates
+    inv_tau = np.full(F.shape, 1.0 / 3600.0)
+    inv_tau[1, :] = 0.0
+    dt = 30.0
+    O.green_ampt_drain(q, F, {"cls": cls, "Ks": np.array([1e-4, 1e-4, 1e-4]), "psi": np.full(3, 0.1),
+                              "dth": np.full(3, 0.3), "Fmax": Fmax, "inv_tau": inv_tau}, dt)
+    assert np.all((F <= Fmax + 1e-18) | (F == F0)) and F[0, 0] == F0[0, 0]
+    decay = np.exp(-dt / 3600.0)
+    dF = F - F0
+    assert np.allclose(q[0][2:], ((q0[0] - dF) * decay)[2:], rtol=1e-14)
+    assert np.allclose(q[0][1], (q0[0] - dF)[1], rtol=1e-15)          # inv_tau = 0: no decay
+    # drain alone: h * exp(-dt/tau) exactly as exp_ieee gives it, momentum by the same factor
+    q2, _, F2 = _ga_state()
+    q20 = q2.copy()
+    O.green_ampt_drain(q2, F2, {"cls": cls, "Ks": None, "inv_tau": inv_tau}, dt)
+    d = O.exp_ieee(np.array([-dt * (1.0 / 3600.0)]))[0]
+    assert np.array_equal(q2[:, 2:], q20[:, 2:] * d)
+
+
+def test_green_ampt_sits_between_ring_and_clamp():
+    """A clamp above the Green-Ampt result leaves it alone; clamping first would cap and then
+    infiltrate, ending below the cap."""
+    q = np.array([[[0.5]], [[0.1]], [[0.0]]], np.float32)
+    F = np.zeros((1, 1), np.float32)
+    ga = {"cls": np.zeros((1, 1), int), "Ks": [1e-3], "psi": [0.1], "dth": [0.3]}
+    qa = q.copy()
+    O.green_ampt_drain(qa, F.copy(), ga, 10.0)
+    hmax = float(qa[0, 0, 0]) + 0.01
+    O.coastal_force(q, clamp=np.array([[hmax]]), ga=(ga, F, 10.0, None))
+    assert q[0, 0, 0] == qa[0, 0, 0] and np.float32(hmax) < np.float32(0.5)
