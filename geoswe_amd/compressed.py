@@ -451,6 +451,18 @@ class CompressedSolver:
             self._raise_if_bad(self._clockv())
 
     # ------------------------------------------------------------------ checkpoints
+    def save_cache(self, cache_dir: str, x0: float = 0.0, y0: float = 0.0, crs_wkt: str = "") -> None:
+        """Write this solver in the reference's cache layout (compressed_solver.py:2504-2569);
+        :func:`geoswe_amd.cache.run_cached` replays it."""
+        from .cache import save_cache
+        save_cache(self, cache_dir, x0=x0, y0=y0, crs_wkt=crs_wkt)
+
+    @classmethod
+    def from_cache(cls, cache_dir: str, **kw) -> "CompressedSolver":
+        """A solver over a cache directory (see :func:`geoswe_amd.cache.from_cache`)."""
+        from .cache import from_cache
+        return from_cache(cache_dir, **kw)
+
     def save_checkpoint(self, ckpt_dir: str, fidx: int = 0, next_frame: Optional[float] = None) -> None:
         """Write the current state to ``ckpt_dir`` in the reference's single-rank layout
         (compressed_solver.py:1789-1880): ``ckpt_r00.npz`` holding ``t``, ``steps``, ``q0``,

@@ -116,6 +116,11 @@ class _Forcing:
             self.s.n_stage = len(sb.time_s)
             self.s.stage_t, self.s.stage_v = st.ptr, sv.ptr
 
+    def drop_rain_cap(self) -> None:
+        """No rain-film step bound (the reference's cached replay applies none by default)."""
+        if self.rain_host is not None:
+            self.keep[2].upload(np.full(self.s.n_rain, np.inf))
+
     def rain_rate_at(self, t: float):
         """(rate, rmax) at time t on the host, for the host-side conveniences."""
         if self.rain_host is None:

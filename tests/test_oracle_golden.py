@@ -117,3 +117,21 @@ def test_flat_mesh_order_and_neighbours():
             assert nb[k, 2] == k + 1 and nb[k, 3] == k - 1
             assert nb[nb[k, 0], 0] == k + fm.run_d[r, 2] and nb[nb[k, 1], 1] == k + fm.run_d[r, 3]
     assert sum(int(x) for x in fm.run_len) == act.size == int(z["inside"].sum())
+
+
+def test_cache_neighbour_table_is_the_reference_one():
+    """The cache's nbr.npy (int16 deltas, -32768 for none) over every stored cell equals the
+    reference's CompressedMesh2D neighbour table of the golden mask, delta-encoded as
+    nbr_to_int16_delta does (compressed_rhs.py:194-235)."""
+    from geoswe_amd.cache import _nbr_int16
+    from geoswe_amd.flatmesh import build_flat_mesh
+    z = np.load(G / "flatmesh_small.npz")
+    g = int(z["ngh"])
+    inside = np.zeros((z["inside"].shape[0] + 2 * g, z["inside"].shape[1] + 2 * g), bool)
+    inside[g:-g, g:-g] = z["inside"]
+    fm = build_flat_mesh(inside, g, ring=2)
+    nb = z["neighbors"].astype(np.int64)
+    k = np.arange(fm.n_stored, dtype=np.int64)[:, None]
+    want = np.where(nb >= 0, nb - k, -32768).astype(np.int16)
+    got = _nbr_int16(fm)
+    assert got.dtype == np.int16 and np.array_equal(got, want)
