@@ -6,7 +6,7 @@ scheme: ``Config``, ``Mesh2D``, ``Solver2D``, ``CompressedSolver``, ``RainfallFo
 ``include/geoswe_abi.h``; there is no CPU fallback.
 """
 from .config import Config, Mesh2D, G, H_MIN
-from .forcing import RainfallForcing, StageBoundary
+from .forcing import NativeRain, RainfallForcing, StageBoundary
 
 __version__ = "0.1.0"
 

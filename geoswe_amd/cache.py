@@ -9,8 +9,8 @@ rain and karst-drain bundles applied, Green-Ampt, the stage clamp and constant-r
 infiltration not stored (the reference warns about the same three).
 
 Rain bundles hold a native table ``(nt, npix)`` in m/s and a per-cell index into it; a
-uniform rate is written as a one-pixel table. On loading, the table is spread onto the
-solver's grid (one frame per time), which is exact but holds ``nt`` full frames.
+uniform rate is written as a one-pixel table, (nx, ny) frames as a table of nx*ny columns.
+On loading, the step reads the table through the per-cell index (a :class:`NativeRain`).
 """
 from __future__ import annotations
 
@@ -24,7 +24,7 @@ import numpy as np
 from . import _lib as L
 from .config import Config, Mesh2D
 from .flatmesh import build_flat_mesh
-from .forcing import RainfallForcing, rain_tables
+from .forcing import NativeRain, rain_tables
 
 
 def _nbr_int16(fm) -> np.ndarray:
@@ -107,7 +107,10 @@ def save_cache(solver, cache_dir: str, x0: float = 0.0, y0: float = 0.0, crs_wkt
         sv("sponge_amb_f", ka[:, 1].astype(np.float32))
     if rt is not None:
         t, rate, _, frames = rt
-        if frames is None:
+        if isinstance(s.cfg.rainfall_forcing, NativeRain):
+            native = frames.astype(np.float32)
+            lookup = s._rain_lookup.download().astype(np.int32)
+        elif frames is None:
             native = rate.astype(np.float32)[:, None]
             lookup = np.zeros(N, np.int32)
         else:
@@ -165,24 +168,22 @@ def from_cache(cache_dir: str, cfl: float = 0.5, h_min: float = 1e-6, g: float =
     rain = None
     if meta.get("has_rain"):
         native = ld("rain_native_rate_dev").astype(np.float32)
-        lookup = ld("rain_lookup_flat").astype(np.int64)
-        t_s = ld("rain_t_s").astype(np.float64)
-        if native.shape[1] == 1:
-            rain = RainfallForcing(time_s=t_s, rate_mm_h=native[:, 0].astype(np.float64) * 3.6e6)
-        else:
-            frames = np.zeros((native.shape[0], nx, ny), np.float64)
-            ii, jj = ij[act, 0] - ngh, ij[act, 1] - ngh
-            inb = (ii >= 0) & (ii < nx) & (jj >= 0) & (jj < ny)
-            frames[:, ii[inb], jj[inb]] = native[:, lookup[act][inb]].astype(np.float64) * 3.6e6
-            rain = RainfallForcing(time_s=t_s, rate_mm_h=frames)
-    cfg = Config(bc_x="fall", bc_y="fall", cfl=cfl, g=g, h_min=h_min, dtype="float32", friction="manning",
-                 rainfall_forcing=rain)
+        lookup_flat = ld("rain_lookup_flat").astype(np.int64)
+        # the (nx, ny) column map the solver takes, from the per-cell one
+        lk = np.zeros((nx, ny), np.int64)
+        ii, jj = ij[act, 0] - ngh, ij[act, 1] - ngh
+        inb = (ii >= 0) & (ii < nx) & (jj >= 0) & (jj < ny)
+        lk[ii[inb], jj[inb]] = lookup_flat[act][inb]
+        rain = NativeRain(time_s=ld("rain_t_s").astype(np.float64), table=native, lookup=lk)
+    cfg = Config(bc_x="fall", bc_y="fall", cfl=cfl, g=g, h_min=h_min, dtype="float32", friction="manning")
     q = np.stack([ld(f"q{k}").astype(np.float32) for k in range(3)])
     m_tab = ld("m_tab").astype(np.float64)
     s = CompressedSolver(mesh, cfg, fm, fm.unpack(q), fm.unpack(ld("bed_f").astype(np.float32)),
                          fm.unpack(ld("mcls_f").astype(np.uint8)), m_tab, None, cfl_linf, track_max_depth)
-    if rain is not None and not rain_cap:
-        s._fo.drop_rain_cap()
+    if rain is not None:
+        s.set_rain(rain)
+        if not rain_cap:
+            s._fo.drop_rain_cap()
     if meta.get("has_sponge"):
         s.set_sponge({"keep": fm.unpack(ld("sponge_keep_f"), fill=1.0), "amb": fm.unpack(ld("sponge_amb_f"))})
     if meta.get("has_ring"):

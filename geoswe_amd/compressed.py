@@ -20,7 +20,7 @@ import numpy as np
 from . import _lib as L
 from .config import Config, check_numeric_inputs
 from .flatmesh import build_flat_mesh
-from .forcing import RainfallForcing
+from .forcing import NativeRain, RainfallForcing
 from .solver import Solver2D, _Forcing, host_bc_fill
 
 
@@ -44,6 +44,7 @@ class CompressedSolver:
         self._flags_base = fl              # without the coastal-forcing bit (set_sponge / set_clamp)
         self._sponge = self._clamp = self._force_ring = self._force_ring_row = self._ring = None
         self._infil = self._drain = self._ga = None
+        self._rain_lookup = None
         self._tables_taken = False         # an Ensemble or GaugeMisfit holds this solver's device pointers
         t = np.zeros(L.MAX_CLASSES, self.dtype)
         t[:len(ntab)] = ntab
@@ -102,12 +103,33 @@ class CompressedSolver:
     def n_stored(self) -> int:
         return self.fm.n_stored
 
-    def set_rain(self, forcing: RainfallForcing):
-        """Replace the rain forcing (compressed_solver.py:3157-3183)."""
+    def set_rain(self, forcing):
+        """Replace the rain forcing (compressed_solver.py:3157-3183): a :class:`RainfallForcing`
+        (uniform, or one (nx, ny) frame per time), a :class:`NativeRain`, or the run driver's
+        bundle ``{"native_rate_dev", "lookup_dev", "t_s"}`` (a rate table on its native grid in
+        m/s, one row per time, and the (nx, ny) table column of each cell), which the step
+        reads through a per-cell lookup. Before the first step, as there."""
+        self._require_unbuilt("set_rain")
+        if isinstance(forcing, dict):
+            _h = lambda a: a.get() if hasattr(a, "get") else np.asarray(a)
+            forcing = NativeRain(time_s=_h(forcing["t_s"]), table=_h(forcing["native_rate_dev"]),
+                                 lookup=_h(forcing["lookup_dev"]))
+        lookup = None
+        if isinstance(forcing, NativeRain):
+            fm, nx, ny = self.fm, self.mesh.nx, self.mesh.ny
+            if forcing.lookup.shape != (nx, ny):
+                raise ValueError(f"set_rain: lookup has shape {forcing.lookup.shape}, not ({nx}, {ny})")
+            ii = fm.ij[:, 0].astype(np.int64) - fm.ngh
+            jj = fm.ij[:, 1].astype(np.int64) - fm.ngh
+            inb = fm.active & (ii >= 0) & (ii < nx) & (jj >= 0) & (jj < ny)
+            lk = np.where(inb, forcing.lookup[np.clip(ii, 0, nx - 1), np.clip(jj, 0, ny - 1)], 0)
+            lookup = L.DeviceArray.from_host(np.ascontiguousarray(lk.astype(np.int32)))
         self.cfg.rainfall_forcing = forcing
         self._fo = _Forcing(self.cfg, min(self.mesh.dx, self.mesh.dy), self.dtype, (self.mesh.nx, self.mesh.ny))
+        self._rain_lookup = lookup
         self._lam_valid = False
         self._target = None
+        return self
 
     # ------------------------------------------------------------------ coastal forcings
     def _require_unbuilt(self, who: str) -> None:
@@ -367,6 +389,7 @@ class CompressedSolver:
         f.infil = self._infil.ptr if self._infil is not None else None
         f.drain = self._drain[0].ptr if self._drain is not None else None
         ga = self._ga or {}
+        f.rain_lookup = self._rain_lookup.ptr if self._rain_lookup is not None else None
         f.ga_cls, f.ga_ks, f.ga_psi, f.ga_dth, f.ga_F, f.ga_fmax, f.ga_inv_tau = (
             ga[k].ptr if k in ga else None for k in ("cls", "ks", "psi", "dth", "F", "fmax", "inv_tau"))
         f.n_force_ring = self._force_ring.shape[0] if self._force_ring is not None else 0

@@ -66,6 +66,7 @@ struct FlatK {
   T* ga_F;
   const T* ga_fmax;
   const T* ga_inv_tau;
+  const int32_t* rain_lookup;  // rain_mode 3: native-table column of each cell
   int64_t n_fring;  // forced cells of the ring (stored, not active): sponge, clamp, gauge ring
   const int64_t* fring;
   const int32_t* fring_row;  // nullable: gauge-ring row of each entry, -1 for none
@@ -122,6 +123,7 @@ struct FlatCtx {
   GS_G T* ga_F;
   const GS_G T* ga_fmax;
   const GS_G T* ga_inv_tau;
+  const GS_G int32_t* rain_lookup;
   const GS_G T* frame;   // gridded rain frame of this step (rain_mode 2)
   const T* ntab;         // Manning n per class (LDS copy or the member's table)
   int64_t n, rain_pitch;
@@ -173,7 +175,7 @@ __device__ __forceinline__ FlatClk<T> flat_clock(const FlatK<T>& a) {
       k.rain_u = a.rain_mode == 1 ? (T)gp(a.plan.rain_rate)[ridx] : T(0);
       k.eta = (T)c->eta64;
     }
-    if (a.rain_mode == 2) k.frame = gp(a.rain_frames) + (int64_t)ridx * a.frame_stride;
+    if (a.rain_mode >= 2) k.frame = gp(a.rain_frames) + (int64_t)ridx * a.frame_stride;
   }
   return k;
 }
@@ -219,6 +221,7 @@ __device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k,
   x.ga_F = gp(a.ga_F);
   x.ga_fmax = gp(a.ga_fmax);
   x.ga_inv_tau = gp(a.ga_inv_tau);
+  x.rain_lookup = gp(a.rain_lookup);
   x.frame = k.frame;
   x.ntab = a.ntab;
   x.n = a.n;
@@ -456,6 +459,8 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
     const int64_t ii = (int64_t)a.run_i[r] - a.ngh;
     const int64_t jj = (int64_t)a.run_j0[r] + (k - a.run_k0[r]) - a.ngh;
     rr0 = rr0 + x.frame[ii * a.rain_pitch + jj];
+  } else if (a.rain_mode == 3) {
+    rr0 = rr0 + x.frame[x.rain_lookup[k]];
   }
   if (a.pre0) {   // the first three statements of update(), kept for its reverse
     a.pre0[k] = hC + x.dt * rr0;
@@ -526,7 +531,7 @@ __device__ __forceinline__ T sweep_b(const FlatCtx<T>& x, int64_t base, uint32_t
 // flat_chunk), so all three traversals give the same bits. Every lane of the wavefront
 // must call it. PRE: also write the pre-friction state (the adjoint's recomputation); a
 // separate instantiation keeps those pointers out of the plain step's scalar registers.
-template <typename T, bool PRE, bool GA>
+template <typename T, bool PRE, bool EXT>
 __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int lane, uint64_t& lam) {
   const Phys<T>& P = x.P;
   const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
@@ -615,6 +620,8 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
         rr0 = rr0 + x.rain_u;
       } else if (x.rain_mode == 2) {
         rr0 = rr0 + x.frame[(int64_t)(i0 + r - x.ngh) * x.rain_pitch + (c0 + lane - x.ngh)];
+      } else if (EXT && x.rain_mode == 3) {
+        rr0 = rr0 + x.frame[gat(x.rain_lookup + gC, (oC / (uint32_t)sizeof(T)) * 4u)];
       }
       if (PRE) {   // the first three statements of update(), kept for its reverse
         gat(x.pre0 + gC, oC) = hC + x.dt * rr0;
@@ -632,7 +639,7 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
         hv = T(0);
       }
       if (x.max_h && h > gat(x.max_h + gC, oC)) gat(x.max_h + gC, oC) = h;
-      active_force<T, GA>(x, fg, gC, oC, ob, h, hu, hv);
+      active_force<T, EXT>(x, fg, gC, oC, ob, h, hu, hv);
       gat(x.qo0 + gC, oC) = h;
       gat(x.qo1 + gC, oC) = hu;
       gat(x.qo2 + gC, oC) = hv;
@@ -701,10 +708,11 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(sizeof(
   grid_reduce_and_plan(lam, a.clock, a.plan, true);
 }
 
-// The sweep with Green-Ampt infiltration and drain-tau compiled in (launched only when
-// their tables are set).
+// The sweep with the extended forcings compiled in: Green-Ampt infiltration, drain-tau and
+// rain read through a per-cell lookup into a native-grid table (launched only when one of
+// them is set; the plain sweep has no register to spare for them).
 template <typename T>
-__global__ __launch_bounds__(kFBlock) void k_flat_sweep_ga(FlatK<T> a) {
+__global__ __launch_bounds__(kFBlock) void k_flat_sweep_ext(FlatK<T> a) {
   const FlatClk<T> ck = flat_clock<T, 1>(a);
   if (!ck.live) return;
   FlatCtx<T> x;
@@ -877,7 +885,8 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   a.ring_t = f->ring_t;
   a.ring_stage = (const T*)f->ring_stage;
   a.P = make_phys<T>(*p, f->dx, f->dy);
-  a.rain_mode = fo.n_rain > 0 ? (fo.rain_gridded ? 2 : 1) : 0;
+  a.rain_mode = fo.n_rain > 0 ? (fo.rain_gridded ? (f->rain_lookup ? 3 : 2) : 1) : 0;
+  a.rain_lookup = f->rain_lookup;
   a.rain_frames = (const T*)fo.rain_frames;
   a.frame_stride = fo.rain_frame_stride;
   a.rain_pitch = fo.rain_row_pitch;
@@ -917,9 +926,9 @@ static unsigned sweep_grid(int64_t ntiles) {
 }
 
 template <typename T>
-static unsigned sweep_ga_grid(int64_t ntiles) {
+static unsigned sweep_ext_grid(int64_t ntiles) {
   static std::atomic<int> cache[64];
-  return resident_grid(k_flat_sweep_ga<T>, cache, ntiles);
+  return resident_grid(k_flat_sweep_ext<T>, cache, ntiles);
 }
 
 // The fused step: row-sweep tiles when the descriptor has them, else chunks of the run table.
@@ -933,9 +942,9 @@ static int flat_step_launch(const FlatK<T>& a, hipStream_t s) {
   if (a.n_tiles > 0 && a.pre[0]) {
     hipLaunchKernelGGL((k_flat_sweep<T, true>), dim3(sweep_grid<T, true>(a.n_tiles)), dim3(kFBlock), 0, s, a);
     GS_HIP(hipGetLastError(), "k_flat_sweep(pre) launch");
-  } else if (a.n_tiles > 0 && (a.ga_ks || a.ga_inv_tau)) {
-    hipLaunchKernelGGL((k_flat_sweep_ga<T>), dim3(sweep_ga_grid<T>(a.n_tiles)), dim3(kFBlock), 0, s, a);
-    GS_HIP(hipGetLastError(), "k_flat_sweep_ga launch");
+  } else if (a.n_tiles > 0 && (a.ga_ks || a.ga_inv_tau || a.rain_mode == 3)) {
+    hipLaunchKernelGGL((k_flat_sweep_ext<T>), dim3(sweep_ext_grid<T>(a.n_tiles)), dim3(kFBlock), 0, s, a);
+    GS_HIP(hipGetLastError(), "k_flat_sweep_ext launch");
   } else if (a.n_tiles > 0) {
     hipLaunchKernelGGL((k_flat_sweep<T, false>), dim3(sweep_grid<T, false>(a.n_tiles)), dim3(kFBlock), 0, s, a);
     GS_HIP(hipGetLastError(), "k_flat_sweep launch");

@@ -66,6 +66,28 @@ class RainfallForcing:
 
 
 @dataclass
+class NativeRain:
+    """Rain on a native grid, the run driver's bundle (compressed_solver.py:3157-3183):
+    ``table`` (nt, npix) rates in m/s, one row per time of ``time_s`` (piecewise constant, as
+    :class:`RainfallForcing`), and ``lookup`` (nx, ny) the table column each cell reads.
+    A flat-solver forcing (``CompressedSolver.set_rain``)."""
+    time_s: np.ndarray
+    table: np.ndarray
+    lookup: np.ndarray
+
+    def __post_init__(self):
+        self.time_s = np.asarray(self.time_s, dtype=np.float64)
+        self.table = np.asarray(self.table)
+        self.lookup = np.asarray(self.lookup)
+        if self.table.ndim != 2 or self.table.shape[0] != len(self.time_s):
+            raise ValueError(f"NativeRain: table has shape {self.table.shape}; it must be (nt, npix) with nt = "
+                             f"{len(self.time_s)}")
+        if self.lookup.size and (self.lookup.min() < 0 or self.lookup.max() >= self.table.shape[1]):
+            raise ValueError(f"NativeRain: lookup columns must lie in [0, {self.table.shape[1]})")
+        _require_time_order("NativeRain", self.time_s)
+
+
+@dataclass
 class StageBoundary:
     """Water level eta(t) imposed on ``cells`` (M, 2) padded (i, j) indices after each step."""
     cells: np.ndarray
@@ -121,6 +143,12 @@ def rain_tables(forcing, rainfall: float, cfl: float, g: float, dmin: float, dty
             return np.inf
         return (cfl * dmin) ** (2.0 / 3.0) / (g * rmax) ** (1.0 / 3.0)
 
+    if isinstance(forcing, NativeRain):
+        # one bound from the largest rate of the whole table (compressed_solver.py:3429-3436)
+        t = forcing.time_s
+        frames = forcing.table.astype(dtype)
+        cap = cap_of(float(frames.max())) if frames.size else np.inf
+        return t, np.zeros(len(t)), np.full(len(t), cap), frames
     if forcing is not None:
         t = np.asarray(forcing.time_s, dtype=np.float64)
         r = np.asarray(forcing.rate_mm_h)

@@ -22,7 +22,7 @@ import numpy as np
 
 from . import _lib as L
 from .config import Config, Mesh2D
-from .forcing import rain_tables
+from .forcing import NativeRain, rain_tables
 
 _PITCH_ALIGN = 64
 
@@ -89,7 +89,14 @@ class _Forcing:
             self.keep += [dt_, dr, dc]
             self.s.n_rain = len(t)
             self.s.rain_t, self.s.rain_rate, self.s.rain_cap = dt_.ptr, dr.ptr, dc.ptr
-            if frames is not None:
+            if frames is not None and frames.ndim == 2:       # a native table read through a lookup
+                df = L.DeviceArray.from_host(np.ascontiguousarray(frames))
+                self.keep.append(df)
+                self.s.rain_gridded = 1
+                self.s.rain_frames = df.ptr
+                self.s.rain_frame_stride = frames.shape[1]
+                self.s.rain_row_pitch = frames.shape[1]
+            elif frames is not None:
                 if tuple(frames.shape[1:]) != tuple(frame_shape):
                     raise ValueError(f"gridded rain must be (nt, nx, ny) = (nt, {frame_shape[0]}, "
                                      f"{frame_shape[1]}); got {tuple(frames.shape)}")
@@ -166,6 +173,9 @@ class Solver2D:
         if mesh.nx < 4 or mesh.ny < 4:
             raise ValueError("the grid needs at least 4 x 4 interior cells")
         rf = cfg.rainfall_forcing
+        if isinstance(rf, NativeRain):
+            raise NotImplementedError("rain on a native grid is read through a per-cell lookup by the flat solver "
+                                      "only (CompressedSolver.set_rain)")
         if rf is not None and np.asarray(rf.rate_mm_h).ndim == 3 and tuple(rf.rate_mm_h.shape[1:]) != (mesh.nx, mesh.ny):
             raise ValueError(f"gridded rain must be (nt, nx, ny) = (nt, {mesh.nx}, {mesh.ny}); got "
                              f"{tuple(rf.rate_mm_h.shape)}")

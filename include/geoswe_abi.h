@@ -202,6 +202,11 @@ typedef struct GsFlat {
   void* ga_F;                /* [N] cumulative infiltration (m), state dtype, updated in place */
   const void* ga_fmax;       /* nullable [N] storage cap of F */
   const void* ga_inv_tau;    /* nullable [N] drain rate 1/tau (1/s): no drain when NULL */
+  /* Rain on a native grid (the run driver's bundle, compressed_solver.py:3157-3183, 2329):
+   * with gridded rain, a cell reads column rain_lookup[k] of the step's row of the
+   * (nt, npix) rate table (GsForcing.rain_frame_stride = npix) instead of its own cell of an
+   * (nx, ny) frame. */
+  const int32_t* rain_lookup; /* nullable [N] */
   /* GS_FLAG_FORCE cells of the ring (stored, not active): the step writes
    * force(state) for them, as the reference's gathered kernels touch every stored cell
    * of the band; the residual and the CFL never see them. */

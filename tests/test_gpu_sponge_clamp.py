@@ -473,3 +473,79 @@ def test_green_ampt_resume_and_ensemble(gpu, tmp_path):
         assert p.nsteps == e.nsteps == 15
         assert np.array_equal(p.q_flat(), e.q_flat()) and np.array_equal(p.ga_F(), e.ga_F())
     ens.close()
+
+
+def _native_rain(seed=61):
+    rng = np.random.default_rng(seed)
+    bi, bj = 7, 9
+    pny = -(-NY // bj)
+    npix = -(-NX // bi) * pny
+    lk = (np.arange(NX)[:, None] // bi) * pny + (np.arange(NY)[None, :] // bj)
+    t_s = np.array([0.0, 0.6, 1.3])
+    mmh = rng.random((3, npix)) * 40.0
+    return t_s, mmh, lk
+
+
+@pytest.mark.parametrize("dtype,trav", [("float32", "tiles"), ("float32", "chunks"), ("float64", "tiles")])
+def test_native_grid_rain_matches_oracle(gpu, dtype, trav):
+    """The run driver's rain bundle (a rate table on a coarser native grid and each cell's
+    column) read through the per-cell lookup: bitwise the oracle given the same rates spread
+    onto the grid."""
+    from geoswe_amd import CompressedSolver
+    q0, bed = rough_state(NX, NY, seed=62, depth=0.7, rough=1.0, vel=0.3)
+    m = irregular_mask(NX, NY, seed=9)
+    q0 = q0 * m
+    s = dense(q0, bed, friction="manning", manning_n=0.03, dtype=dtype)
+    s.set_inside_mask(m)
+    s.set_cfl_ghost_mask(~m)
+    c = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=True)
+    t_s, mmh, lk = _native_rain()
+    c.set_rain({"native_rate_dev": mmh / 3.6e6, "lookup_dev": lk, "t_s": t_s})
+    c.traversal = trav
+    n = c.run(T_END)
+    case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=bed, dx=2.0, dy=2.0, bc_x="fall", bc_y="fall",
+                       manning=0.03, rain=(t_s, mmh[:, lk]), active=m)
+    o = O.DenseOracle(case, _physics(c.cfg), np.dtype(dtype))
+    o.cfl_skip = ~m
+    o.run(T_END)
+    print(dtype, trav, n, flush=True)
+    assert o.nsteps == c.nsteps == n > 5 and o.t == c.t
+    g = c.fm.ngh
+    ij = c.fm.ij - g
+    inner = (ij[:, 0] >= 0) & (ij[:, 0] < NX) & (ij[:, 1] >= 0) & (ij[:, 1] < NY)
+    assert np.array_equal(c.q_flat()[:, inner], o.interior[:, ij[inner, 0], ij[inner, 1]])
+
+
+def test_native_grid_rain_in_the_ensemble_and_the_cache(gpu, tmp_path):
+    from geoswe_amd import CompressedSolver
+    from geoswe_amd.cache import run_cached
+    from geoswe_amd.ensemble import Ensemble
+    q0, bed = rough_state(NX, NY, seed=63, depth=0.7, rough=1.0, vel=0.3)
+    m = irregular_mask(NX, NY, seed=9)
+    q0 = q0 * m
+    t_s, mmh, lk = _native_rain(64)
+
+    def mk(linf=True):
+        s = dense(q0, bed, friction="manning", manning_n=0.03)
+        s.set_inside_mask(m)
+        s.set_cfl_ghost_mask(~m)
+        c = CompressedSolver.from_dense(s, cfl_linf=linf, track_max_depth=True)
+        c.set_rain({"native_rate_dev": mmh / 3.6e6, "lookup_dev": lk, "t_s": t_s})
+        return c
+
+    plain = [mk(), mk()]
+    for p in plain:
+        p.advance(15)
+    mem = [mk(), mk()]
+    ens = Ensemble(mem, graph_batch=8)
+    ens.advance(15)
+    for p, e in zip(plain, mem):
+        assert p.nsteps == e.nsteps == 15 and np.array_equal(p.q_flat(), e.q_flat())
+    ens.close()
+    a = mk(linf=False)
+    d = str(tmp_path / "cache")
+    a.save_cache(d)
+    a.run(2.5)
+    b = run_cached(d, t_end=2.5, say=None)
+    assert b.nsteps == a.nsteps and b.t == a.t and np.array_equal(b.q_flat(), a.q_flat())
+    assert np.load(tmp_path / "cache" / "rain_native_rate_dev.npy").shape == mmh.shape
