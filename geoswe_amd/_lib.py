@@ -66,7 +66,8 @@ class GsFlat(C.Structure):
                 ("tile_irc", C.c_void_p), ("ent", C.c_void_p), ("sponge", C.c_void_p), ("clamp", C.c_void_p), ("infil", C.c_void_p), ("drain", C.c_void_p),
                 ("ga_cls", C.c_void_p), ("ga_ks", C.c_void_p), ("ga_psi", C.c_void_p), ("ga_dth", C.c_void_p),
                 ("ga_F", C.c_void_p), ("ga_fmax", C.c_void_p), ("ga_inv_tau", C.c_void_p),
-                ("rain_lookup", C.c_void_p),
+                ("rain_lookup", C.c_void_p), ("n_ring_mode", C.c_int64), ("ring_mode_ids", C.c_void_p),
+                ("ring_mode_nb", C.c_void_p), ("ring_mode_kind", C.c_void_p), ("ring_mode_eta", C.c_double),
                 ("n_force_ring", C.c_int64), ("force_ring", C.c_void_p), ("force_ring_row", C.c_void_p),
                 ("ring_ng", C.c_int32), ("ring_nt", C.c_int32), ("ring_bed", C.c_void_p), ("ring_w", C.c_void_p),
                 ("ring_t", C.c_void_p), ("ring_stage", C.c_void_p)]

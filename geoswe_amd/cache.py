@@ -4,8 +4,9 @@
 forcing bundles as ``.npy`` files plus ``meta.json`` under the names and dtypes of
 ``geoswe.compressed_solver.save_cache`` (compressed_solver.py:2504-2569); ``from_cache``
 and ``run_cached`` (2616-2859) load such a directory into a solver and run it, as the
-reference's cached replay does: no rain step cap unless ``dt_max="rain"``, the ring, sponge,
-rain and karst-drain bundles applied, Green-Ampt, the stage clamp and constant-rate
+reference's cached replay does: no rain step cap unless ``dt_max="rain"``, the ring mode the
+reference's environment variables select, the ring, sponge, rain and karst-drain bundles
+applied, Green-Ampt, the stage clamp and constant-rate
 infiltration not stored (the reference warns about the same three).
 
 Rain bundles hold a native table ``(nt, npix)`` in m/s and a per-cell index into it; a
@@ -140,11 +141,8 @@ def from_cache(cache_dir: str, cfl: float = 0.5, h_min: float = 1e-6, g: float =
     if not meta.get("no_sigma", False) and os.path.exists(os.path.join(cache_dir, "sig_f.npy")):
         if np.load(os.path.join(cache_dir, "sig_f.npy")).any():
             raise NotImplementedError("from_cache: sub-grid storage (sig_f) is not part of this solver")
-    for var in ("GEOSWE_RING_BC", "GEOSWE_RING_ETA", "SWE_GHOST_ETA"):
-        v = os.environ.get(var)
-        if v is not None and not (var == "GEOSWE_RING_BC" and v.lower() in ("auto", "off", "none")):
-            raise NotImplementedError(f"from_cache: {var}={v!r} selects a ring mode of the reference's replay, "
-                                      f"which this solver does not implement")
+    if os.environ.get("SWE_GHOST_ETA_RAMP_MMHR"):
+        raise NotImplementedError("SWE_GHOST_ETA_RAMP_MMHR is set but the ring stage is constant; unset it")
 
     def ld(name):
         return np.load(os.path.join(cache_dir, name + ".npy"))
@@ -184,6 +182,14 @@ def from_cache(cache_dir: str, cfl: float = 0.5, h_min: float = 1e-6, g: float =
         s.set_rain(rain)
         if not rain_cap:
             s._fo.drop_rain_cap()
+    # the ring mode of the replay, from the reference's environment variables (2730-2735)
+    eta = os.environ.get("GEOSWE_RING_ETA", os.environ.get("SWE_GHOST_ETA"))
+    mode = os.environ.get("GEOSWE_RING_BC", "auto").lower()
+    if os.environ.get("SWE_GHOST_UV") in ("1", "true", "True") and mode in ("auto", "stage"):
+        mode = "stage_uv"
+    cls_env = os.environ.get("GEOSWE_RING_CLASSIFY")
+    s.set_ring_mode(mode, eta=None if eta is None else float(eta),
+                    classify=None if cls_env is None else float(cls_env))
     if meta.get("has_sponge"):
         s.set_sponge({"keep": fm.unpack(ld("sponge_keep_f"), fill=1.0), "amb": fm.unpack(ld("sponge_amb_f"))})
     if meta.get("has_ring"):

@@ -45,6 +45,7 @@ class CompressedSolver:
         self._sponge = self._clamp = self._force_ring = self._force_ring_row = self._ring = None
         self._infil = self._drain = self._ga = None
         self._rain_lookup = None
+        self._rmode = None
         self._tables_taken = False         # an Ensemble or GaugeMisfit holds this solver's device pointers
         t = np.zeros(L.MAX_CLASSES, self.dtype)
         t[:len(ntab)] = ntab
@@ -325,6 +326,61 @@ class CompressedSolver:
         L.sync(self.stream)
         return self._ga["F"].download()
 
+    def set_ring_mode(self, mode: str = "auto", eta: Optional[float] = None, classify: Optional[float] = None):
+        """The outer boundary of the ring in the reference's cached replay (_build_ring_bc,
+        compressed_solver.py:486-575, applied at the top of every step, 2134-2144; chosen
+        there by GEOSWE_RING_BC, GEOSWE_RING_ETA and GEOSWE_RING_CLASSIFY). ``"auto"`` or
+        ``"stage"``: still water at `eta` with zero momentum on every ring cell;
+        ``"extrapolate"``: each ring cell beside the active set takes the state of its first
+        active neighbour (i+1, i-1, j+1, j-1) and, once, its bed; ``"hybrid"``: those ring
+        cells whose bed lies below `classify` (default `eta`) get the stage, the others
+        extrapolate; ``"stage_uv"``: the stage, then the velocity of the neighbour
+        (ghost_uv, 450-465); ``"off"``, or any stage mode without `eta`, leaves the ring
+        dry. The ring cells' stored values between steps are the previous step's."""
+        from .adjoint import flat_neighbours
+        self._require_unbuilt("set_ring_mode")
+        mode = (mode or "auto").lower()
+        if mode not in ("auto", "stage", "extrapolate", "hybrid", "stage_uv", "off", "none"):
+            raise ValueError(f"set_ring_mode: mode {mode!r}: expected auto|stage|extrapolate|hybrid|stage_uv|off")
+        self._rmode = None
+        if mode in ("off", "none") or (mode != "extrapolate" and eta is None):
+            return self
+        fm = self.fm
+        act = fm.active
+        nb = flat_neighbours(fm)
+        best = np.full(fm.n_stored, -1, np.int64)
+        for d in range(4):                    # the first active neighbour of each ring cell
+            cand = nb[:, d].astype(np.int64)
+            ok = (cand >= 0) & ~act & (best < 0)
+            ok[ok] = act[cand[ok]]
+            best = np.where(ok, cand, best)
+        gi = np.nonzero(best >= 0)[0]
+        gn = best[gi]
+        bed = self._bed.download()
+        if mode == "extrapolate":
+            bed[gi] = bed[gn]
+            ids, nbs, kind = gi, gn, np.zeros(len(gi), np.uint8)
+        elif mode == "hybrid":
+            thr = self.dtype.type(classify if classify is not None else eta)
+            water = bed[gi] < thr
+            mi, mn = gi[~water], gn[~water]
+            bed[mi] = bed[mn]
+            ids = np.concatenate([mi, gi[water]])
+            nbs = np.concatenate([mn, np.full(int(water.sum()), -1)])
+            kind = np.concatenate([np.zeros(len(mi), np.uint8), np.ones(int(water.sum()), np.uint8)])
+        else:
+            ids = np.nonzero(~act)[0]
+            nbs = best[ids]
+            kind = np.where((nbs >= 0) & (mode == "stage_uv"), 2, 1).astype(np.uint8)
+        self._bed.upload(bed)
+        if len(ids):
+            self._rmode = dict(ids=L.DeviceArray.from_host(np.ascontiguousarray(ids, np.int64)),
+                               nb=L.DeviceArray.from_host(np.ascontiguousarray(nbs, np.int64)),
+                               kind=L.DeviceArray.from_host(np.ascontiguousarray(kind)), n=len(ids),
+                               eta=float(eta) if eta is not None else 0.0, mode=mode,
+                               host=(ids, nbs, kind))
+        return self
+
     def _flat_ids(self, rows: np.ndarray, cols: np.ndarray) -> np.ndarray:
         """Flat id of each padded (row, col), -1 where the cell is not stored."""
         fm = self.fm
@@ -356,7 +412,8 @@ class CompressedSolver:
 
     @property
     def has_coastal_forcing(self) -> bool:
-        return any(f is not None for f in (self._sponge, self._clamp, self._ring, self._infil, self._drain, self._ga))
+        return any(f is not None for f in (self._sponge, self._clamp, self._ring, self._infil, self._drain, self._ga,
+                                           self._rmode))
 
     # ------------------------------------------------------------------ structs
     def _phys(self) -> L.GsPhys:
@@ -390,6 +447,10 @@ class CompressedSolver:
         f.drain = self._drain[0].ptr if self._drain is not None else None
         ga = self._ga or {}
         f.rain_lookup = self._rain_lookup.ptr if self._rain_lookup is not None else None
+        rm = self._rmode
+        if rm is not None:
+            f.n_ring_mode, f.ring_mode_eta = rm["n"], rm["eta"]
+            f.ring_mode_ids, f.ring_mode_nb, f.ring_mode_kind = rm["ids"].ptr, rm["nb"].ptr, rm["kind"].ptr
         f.ga_cls, f.ga_ks, f.ga_psi, f.ga_dth, f.ga_F, f.ga_fmax, f.ga_inv_tau = (
             ga[k].ptr if k in ga else None for k in ("cls", "ks", "psi", "dth", "F", "fmax", "inv_tau"))
         f.n_force_ring = self._force_ring.shape[0] if self._force_ring is not None else 0

@@ -67,6 +67,11 @@ struct FlatK {
   const T* ga_fmax;
   const T* ga_inv_tau;
   const int32_t* rain_lookup;  // rain_mode 3: native-table column of each cell
+  int64_t n_rmode;             // ring modes of the cached replay (top of the step)
+  const int64_t* rmode_ids;
+  const int64_t* rmode_nb;
+  const uint8_t* rmode_kind;
+  T rmode_eta;
   int64_t n_fring;  // forced cells of the ring (stored, not active): sponge, clamp, gauge ring
   const int64_t* fring;
   const int32_t* fring_row;  // nullable: gauge-ring row of each entry, -1 for none
@@ -790,6 +795,48 @@ __global__ __launch_bounds__(kFBlock) void k_flat_ring_force(FlatK<T> a) {
   }
 }
 
+// The ring modes of the cached replay on the step's input state, before the step reads
+// it: each listed ring cell is overwritten from its active neighbour (extrapolate), with
+// still water (stage), or with still-water depth and the neighbour's velocity. Only ring
+// cells are written and only active cells (which nothing writes here) are read from.
+template <typename T>
+__global__ __launch_bounds__(kFBlock) void k_flat_ring_mode(FlatK<T> a) {
+  const FlatClk<T> ck = flat_clock<T, 1>(a);
+  if (!ck.live) return;
+  FlatCtx<T> x;
+  flat_ctx<T, 1>(a, ck, x);
+  GS_G T* q0 = (GS_G T*)x.qi0;
+  GS_G T* q1 = (GS_G T*)x.qi1;
+  GS_G T* q2 = (GS_G T*)x.qi2;
+  const GS_G int64_t* ids = gp(a.rmode_ids);
+  const GS_G int64_t* nbs = gp(a.rmode_nb);
+  const GS_G uint8_t* kind = gp(a.rmode_kind);
+  for (int64_t i = (int64_t)blockIdx.x * kFBlock + threadIdx.x; i < a.n_rmode; i += (int64_t)gridDim.x * kFBlock) {
+    const int64_t k = ids[i];
+    const int kd = kind[i];
+    if (kd == 0) {
+      const int64_t nb = nbs[i];
+      q0[k] = q0[nb];
+      q1[k] = q1[nb];
+      q2[k] = q2[nb];
+      continue;
+    }
+    const T d = a.rmode_eta - x.bed[k];
+    const T h = d > T(0) ? d : T(0);
+    T hu = T(0), hv = T(0);
+    if (kd == 2) {
+      const int64_t nb = nbs[i];
+      const T hn = q0[nb];
+      const T inv = hn > a.P.h_min ? T(1) / hn : T(0);
+      hu = h * (q1[nb] * inv);
+      hv = h * (q2[nb] * inv);
+    }
+    q0[k] = h;
+    q1[k] = hu;
+    q2[k] = hv;
+  }
+}
+
 // CFL wave speed of the current flat state over active cells, then plan dt.
 template <typename T>
 __global__ __launch_bounds__(kFBlock) void k_flat_cfl(FlatK<T> a) {
@@ -821,6 +868,9 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   if (f->n_tiles < 0) return fail(GS_E_ARG, "n_tiles < 0");
   if (f->n_tiles > 0 && (!f->tile_e0 || !f->tile_irc || !f->ent)) return fail(GS_E_ARG, "tile tables are NULL");
   if (f->n_force_ring < 0 || f->n_force_ring > f->n_stored) return fail(GS_E_RANGE, "n_force_ring out of range");
+  if (f->n_ring_mode < 0 || f->n_ring_mode > f->n_stored) return fail(GS_E_RANGE, "n_ring_mode out of range");
+  if (f->n_ring_mode > 0 && (!f->ring_mode_ids || !f->ring_mode_nb || !f->ring_mode_kind))
+    return fail(GS_E_ARG, "ring mode arrays are NULL");
   if (f->ga_ks && (!f->ga_cls || !f->ga_psi || !f->ga_dth || !f->ga_F))
     return fail(GS_E_ARG, "Green-Ampt needs the soil class, psi, dth and F arrays");
   if (f->n_force_ring > 0 && !f->force_ring) return fail(GS_E_ARG, "force_ring is NULL");
@@ -887,6 +937,11 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   a.P = make_phys<T>(*p, f->dx, f->dy);
   a.rain_mode = fo.n_rain > 0 ? (fo.rain_gridded ? (f->rain_lookup ? 3 : 2) : 1) : 0;
   a.rain_lookup = f->rain_lookup;
+  a.n_rmode = f->n_ring_mode;
+  a.rmode_ids = f->ring_mode_ids;
+  a.rmode_nb = f->ring_mode_nb;
+  a.rmode_kind = f->ring_mode_kind;
+  a.rmode_eta = (T)f->ring_mode_eta;
   a.rain_frames = (const T*)fo.rain_frames;
   a.frame_stride = fo.rain_frame_stride;
   a.rain_pitch = fo.rain_row_pitch;
@@ -934,6 +989,11 @@ static unsigned sweep_ext_grid(int64_t ntiles) {
 // The fused step: row-sweep tiles when the descriptor has them, else chunks of the run table.
 template <typename T>
 static int flat_step_launch(const FlatK<T>& a, hipStream_t s) {
+  if (a.n_rmode > 0) {
+    const int64_t nb = (a.n_rmode + kFBlock - 1) / kFBlock;
+    hipLaunchKernelGGL((k_flat_ring_mode<T>), dim3((unsigned)(nb < 1024 ? nb : 1024)), dim3(kFBlock), 0, s, a);
+    GS_HIP(hipGetLastError(), "k_flat_ring_mode launch");
+  }
   if (a.n_fring > 0) {
     const int64_t nb = (a.n_fring + kFBlock - 1) / kFBlock;
     hipLaunchKernelGGL((k_flat_ring_force<T>), dim3((unsigned)(nb < 1024 ? nb : 1024)), dim3(kFBlock), 0, s, a);
@@ -1207,6 +1267,8 @@ static int ens_build_views(GsEns* e, const GsFlat* members, const GsPhys* phys, 
     }
     if (views[m].n_runs > 0 && views[m].n_tiles < 1)
       return fail(GS_E_ARG, "member %d has active cells but no row-sweep tiles", m);
+    if (views[m].n_rmode > 0)
+      return fail(GS_E_ARG, "member %d has ring modes; the ensemble step does not apply them", m);
     if (views[m].n_fring > 0)
       return fail(GS_E_ARG, "member %d forces ring cells (gauge ring, or sponge or clamp outside the active set); the ensemble "
                   "step updates active cells only", m);

@@ -34,6 +34,9 @@ class Ensemble:
                 raise TypeError(f"member {k} is {type(m).__name__}, not CompressedSolver")
             if m.dtype != dt0:
                 raise ValueError(f"member {k} has dtype {m.dtype}, member 0 {dt0}")
+            if m._rmode is not None:
+                raise NotImplementedError(f"member {k} has ring modes (set_ring_mode); the ensemble step does not "
+                                          f"apply them")
             if m._force_ring is not None:
                 raise NotImplementedError(f"member {k} forces ring cells (gauge ring, or sponge or clamp outside its active set); "
                                           f"the ensemble step updates active cells only")

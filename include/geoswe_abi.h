@@ -207,6 +207,18 @@ typedef struct GsFlat {
    * (nt, npix) rate table (GsForcing.rain_frame_stride = npix) instead of its own cell of an
    * (nx, ny) frame. */
   const int32_t* rain_lookup; /* nullable [N] */
+  /* Ring modes of the cached replay (compressed_solver.py:486-575, applied at 2134-2144):
+   * at the top of each step, on the step's input state, before anything reads it, each
+   * listed ring cell k with kind
+   *   0 (extrapolate): h, hu, hv <- those of its active neighbour ring_mode_nb;
+   *   1 (stage):       h <- max(eta - bed[k], 0), hu = hv = 0, with eta = ring_mode_eta;
+   *   2 (stage, then velocity from the neighbour, ghost_uv 450-465): the stage, then
+   *                    r = h_nb > h_min ? 1/h_nb : 0, hu <- h*(hu_nb*r), hv <- h*(hv_nb*r). */
+  int64_t n_ring_mode;
+  const int64_t* ring_mode_ids;  /* [n_ring_mode] */
+  const int64_t* ring_mode_nb;   /* [n_ring_mode] neighbour id (kinds 0 and 2) */
+  const uint8_t* ring_mode_kind; /* [n_ring_mode] */
+  double ring_mode_eta;
   /* GS_FLAG_FORCE cells of the ring (stored, not active): the step writes
    * force(state) for them, as the reference's gathered kernels touch every stored cell
    * of the band; the residual and the CFL never see them. */

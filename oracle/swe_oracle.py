@@ -25,6 +25,7 @@ Sources followed (paths relative to the reference checkout):
 * IDW gauge ring ....................... compressed_solver.py:287-307, 2338-2348
 * infiltration, karst drain ............ compressed_solver.py:143-158, 269-286
 * Green-Ampt infiltration, drain-tau ... compressed_solver.py:320-421
+* ring modes of the cached replay ...... compressed_solver.py:450-465, 486-575, 2134-2144
 
 Documented departures (shared with the HIP kernels, which implement the same rule):
 
@@ -588,6 +589,35 @@ def green_ampt_drain(q_int: np.ndarray, F: np.ndarray, ga: dict, dt: float, acti
     hv[on] = hv[on] * scale
 
 
+def ring_mode_apply(q_int: np.ndarray, b_int: np.ndarray, rm: dict, h_min: float) -> None:
+    """The ring modes of the cached replay at the top of a step, in place on a (3, nx, ny)
+    state (compressed_solver.py:2134-2144): kind 0 copies the neighbour's h, hu, hv; kind 1
+    sets still water ``max(eta - b, 0)`` (state dtype) with zero momentum; kind 2 does the same
+    and then takes the neighbour's velocity, ``hu = h*(hu_nb*r)`` with ``r = 1/h_nb`` where
+    ``h_nb > h_min``, else 0 (ghost_uv, 450-465)."""
+    dt = q_int.dtype
+    T = dt.type
+    c = np.asarray(rm["cells"])
+    nbc = np.asarray(rm["nb"])
+    kind = np.asarray(rm["kind"])
+    k0 = kind == 0
+    q_int[:, c[k0, 0], c[k0, 1]] = q_int[:, nbc[k0, 0], nbc[k0, 1]]
+    ks = kind > 0
+    d = T(rm["eta"]) - np.asarray(b_int).astype(dt)[c[ks, 0], c[ks, 1]]
+    h = np.where(d > 0, d, T(0))
+    q_int[0, c[ks, 0], c[ks, 1]] = h
+    q_int[1, c[ks, 0], c[ks, 1]] = 0
+    q_int[2, c[ks, 0], c[ks, 1]] = 0
+    k2 = kind == 2
+    if k2.any():
+        hn = q_int[0, nbc[k2, 0], nbc[k2, 1]]
+        with np.errstate(divide="ignore"):
+            inv = np.where(hn > T(h_min), T(1) / hn, T(0))
+        hg = q_int[0, c[k2, 0], c[k2, 1]]
+        q_int[1, c[k2, 0], c[k2, 1]] = hg * (q_int[1, nbc[k2, 0], nbc[k2, 1]] * inv)
+        q_int[2, c[k2, 0], c[k2, 1]] = hg * (q_int[2, nbc[k2, 0], nbc[k2, 1]] * inv)
+
+
 def ring_depth(ring: dict, t: float, dtype) -> np.ndarray:
     """Still-water depth of each gauge-ring cell at time ``t`` (the step's start time).
 
@@ -707,6 +737,7 @@ class DenseCase:
     infil: Optional[tuple] = None       # (class (nx, ny) ints, rate per class m/s)
     drain: Optional[np.ndarray] = None  # (nx, ny) karst target depth, +inf where none
     ga: Optional[dict] = None           # Green-Ampt / drain-tau: see green_ampt_drain (+ "F" initial)
+    ring_mode: Optional[dict] = None    # {"cells", "nb" (n, 2) ij (-1: none), "kind", "eta"}: see ring_mode_apply
 
 
 class DenseOracle:
@@ -764,6 +795,9 @@ class DenseOracle:
 
     def step(self, dt: float) -> None:
         c, ph = self.case, self.ph
+        if c.ring_mode is not None:
+            g = self.ngh
+            ring_mode_apply(self.q[:, g:g + self.nx, g:g + self.ny], np.asarray(c.bed), c.ring_mode, ph.h_min)
         bc_fill(self.q, self.ngh, c.bc_x, c.bc_y)
         r = residual(self.q, self.b, c.dx, c.dy, ph.g, ph.h_min, active=self.act_pad)
         g = self.ngh

@@ -111,9 +111,29 @@ def test_cache_refusals_and_warnings(gpu, tmp_path):
     s = dense(q0, bed, dtype="float64")
     with pytest.raises(ValueError, match="float32"):
         CompressedSolver.from_dense(s).save_cache(str(tmp_path / "b"))
-    os.environ["GEOSWE_RING_BC"] = "hybrid"
+    os.environ["SWE_GHOST_ETA_RAMP_MMHR"] = "5"
     try:
-        with pytest.raises(NotImplementedError, match="ring mode"):
+        with pytest.raises(NotImplementedError, match="constant"):
             from_cache(str(tmp_path / "a"))
     finally:
-        del os.environ["GEOSWE_RING_BC"]
+        del os.environ["SWE_GHOST_ETA_RAMP_MMHR"]
+
+
+@pytest.mark.parametrize("mode", ["hybrid", "extrapolate"])
+def test_cache_replay_takes_the_ring_mode_from_the_environment(gpu, tmp_path, monkeypatch, mode):
+    """GEOSWE_RING_BC / GEOSWE_RING_ETA select the replay's ring mode as in the reference; the
+    replay equals a solver given the same mode by set_ring_mode."""
+    from geoswe_amd.cache import run_cached
+    d = str(tmp_path / "cache")
+    _solver(forcings=False).save_cache(d)
+    monkeypatch.setenv("GEOSWE_RING_BC", mode)
+    monkeypatch.setenv("GEOSWE_RING_ETA", "0.55")
+    b = run_cached(d, t_end=2.0, say=None)
+    assert b._rmode is not None and b._rmode["mode"] == mode
+    a = _solver(forcings=False)
+    a._fo.drop_rain_cap()
+    a.set_ring_mode(mode, eta=0.55)
+    a.run(2.0)
+    assert a.nsteps == b.nsteps and a.t == b.t
+    act = a.fm.active
+    assert np.array_equal(a.q_flat()[:, act], b.q_flat()[:, act])

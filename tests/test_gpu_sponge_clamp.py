@@ -549,3 +549,44 @@ def test_native_grid_rain_in_the_ensemble_and_the_cache(gpu, tmp_path):
     b = run_cached(d, t_end=2.5, say=None)
     assert b.nsteps == a.nsteps and b.t == a.t and np.array_equal(b.q_flat(), a.q_flat())
     assert np.load(tmp_path / "cache" / "rain_native_rate_dev.npy").shape == mmh.shape
+
+
+@pytest.mark.parametrize("mode,trav", [("auto", "tiles"), ("extrapolate", "tiles"), ("hybrid", "tiles"),
+                                       ("stage_uv", "tiles"), ("stage_uv", "chunks")])
+def test_ring_modes_match_oracle(gpu, mode, trav):
+    """The cached replay's ring modes at the top of every step: active cells, steps and time
+    bitwise the oracle's (which applies the same mode on its inactive ring cells, with the
+    extrapolated cells' bed replicated)."""
+    from geoswe_amd import CompressedSolver
+    q0, bed = rough_state(NX, NY, seed=71, depth=0.7, rough=1.0, vel=0.3)
+    m = irregular_mask(NX, NY, seed=9)
+    q0 = q0 * m
+    s = dense(q0, bed, friction="manning", manning_n=0.03, rainfall=20.0 / 3.6e6)
+    s.set_inside_mask(m)
+    s.set_cfl_ghost_mask(~m)
+    c = CompressedSolver.from_dense(s, cfl_linf=True)
+    eta = 0.45
+    c.set_ring_mode(mode, eta=eta, classify=0.5 if mode == "hybrid" else None)
+    ids, nbs, kind = c._rmode["host"]
+    g = c.fm.ngh
+    ij = c.fm.ij.astype(np.int64) - g
+    bed2 = np.array(bed, np.float64)
+    bf = c._bed.download()
+    ring_inner = (ij[:, 0] >= 0) & (ij[:, 0] < NX) & (ij[:, 1] >= 0) & (ij[:, 1] < NY)
+    bed2[ij[ring_inner, 0], ij[ring_inner, 1]] = bf[ring_inner]          # the replicated bed
+    assert ring_inner[ids].all()
+    rm = {"cells": ij[ids], "nb": np.where(nbs[:, None] >= 0, ij[np.maximum(nbs, 0)], -1), "kind": kind, "eta": eta}
+    c.traversal = trav
+    n = c.run(T_END)
+    case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=bed2.astype(np.float32), dx=2.0, dy=2.0, bc_x="fall",
+                       bc_y="fall", manning=0.03, rain=20.0 / 3.6e6, active=m, ring_mode=rm)
+    o = O.DenseOracle(case, _physics(c.cfg), np.float32)
+    o.cfl_skip = ~m
+    o.run(T_END)
+    print(mode, trav, n, len(ids), flush=True)
+    assert o.nsteps == c.nsteps == n > 5 and o.t == c.t
+    act = c.fm.active
+    ia = ij[act]
+    assert np.array_equal(c.q_flat()[:, act], o.interior[:, ia[:, 0], ia[:, 1]])
+    if mode != "extrapolate":
+        assert (kind > 0).any()

@@ -317,3 +317,23 @@ def test_green_ampt_sits_between_ring_and_clamp():
     hmax = float(qa[0, 0, 0]) + 0.01
     O.coastal_force(q, clamp=np.array([[hmax]]), ga=(ga, F, 10.0, None))
     assert q[0, 0, 0] == qa[0, 0, 0] and np.float32(hmax) < np.float32(0.5)
+
+
+def test_ring_modes_closed_forms():
+    """Kind 0 copies the neighbour exactly; kind 1 is still water at eta over the cell's bed;
+    kind 2 is that depth with the neighbour's velocity (none from a dry neighbour)."""
+    q = np.zeros((3, 4, 4), np.float32)
+    q[:, 1, 1] = [0.8, 0.16, -0.08]
+    q[:, 2, 2] = [1e-7, 1e-8, 0.0]                       # a dry neighbour (below h_min)
+    b = np.zeros((4, 4))
+    b[0, 1] = 0.2
+    b[3, 3] = 0.9
+    rm = {"cells": np.array([[1, 0], [0, 1], [3, 3], [1, 2], [2, 3]]),
+          "nb": np.array([[1, 1], [1, 1], [-1, -1], [1, 1], [2, 2]]), "kind": np.array([0, 2, 1, 2, 2]), "eta": 0.5}
+    O.ring_mode_apply(q, b, rm, 1e-6)
+    assert np.array_equal(q[:, 1, 0], q[:, 1, 1])
+    assert q[0, 0, 1] == np.float32(0.5) - np.float32(0.2)
+    assert np.isclose(q[1, 0, 1] / q[0, 0, 1], 0.16 / 0.8, rtol=1e-6)
+    assert np.isclose(q[2, 0, 1] / q[0, 0, 1], -0.08 / 0.8, rtol=1e-6)
+    assert q[0, 3, 3] == 0 and q[1, 3, 3] == 0                          # bed above eta: dry
+    assert q[0, 1, 2] == np.float32(0.5) and q[1, 2, 3] == 0 and q[0, 2, 3] == np.float32(0.5)
