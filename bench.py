@@ -231,8 +231,8 @@ def coastal(args):
 
 def coastal_forcing_tail(case, args, plain_ms):
     """The coastal step with a driver's whole forcing tail attached (infiltration, sponge,
-    IDW gauge ring, clamp, karst drain): a fresh flat solver, one warm-up step, then the same
-    number of timed steps as the plain run."""
+    IDW gauge ring, Green-Ampt with drain-tau, clamp, karst drain): a fresh flat solver, one
+    warm-up step, then the same number of timed steps as the plain run."""
     from geoswe_amd import _lib as L
     from geoswe_amd.cases import attach_coastal_forcings
     t0 = time.perf_counter()
@@ -247,7 +247,7 @@ def coastal_forcing_tail(case, args, plain_ms):
     log(f"with the forcing tail: {ms:.3f} ms/step against {plain_ms:.3f} ms/step plain "
         f"({(ms / plain_ms - 1) * 100:+.1f} %)")
     return {"ms_per_step": ms, "plain_ms_per_step": plain_ms, "overhead": ms / plain_ms - 1.0, "cells": cells,
-            "kernels": ["gs::k_flat_ring_force<float>", "gs::k_flat_sweep<float, false>"]}
+            "kernels": ["gs::k_flat_ring_force<float>", "gs::k_flat_sweep_ext<float>"]}
 
 
 def ensemble(args):

@@ -136,7 +136,9 @@ def attach_coastal_forcings(case: CoastalCase, f, band: int = 32, n_clamp: int =
     at the inverse-square-distance mean of four corner tide gauges (M2 tide, phase lags of
     0-45 min, sampled every 360 s). Clamp: `n_clamp` seeded active cells capped at 1.5 m.
     Infiltration: 0-10 mm/h by Manning class. Karst drain: `n_drain` seeded active cells
-    capped at 0.5 m. Returns the number of cells each one touches (``"cells"``) and, with
+    capped at 0.5 m. Green-Ampt by Manning class (Ks 0-20 mm/h, suction 5-30 cm, deficit
+    0.2-0.4, storage capped at 0.3 m) with a 2-hour drain-tau on every third active cell.
+    Returns the number of cells each one touches (``"cells"``) and, with
     `fields`, the same forcings on the unpadded grid (``"fields"``: keep, amb, ring cells,
     bed, weights, times and stages, clamp and drain depth fields, +inf where none)."""
     fm = f.fm
@@ -179,8 +181,19 @@ def attach_coastal_forcings(case: CoastalCase, f, band: int = 32, n_clamp: int =
     f.set_infil({"tab": np.linspace(0.0, 10.0, 8) / 3.6e6})
     dr = rng.choice(act_ids, size=min(n_drain, len(act_ids)), replace=False)
     f.set_drain({"idx": dr, "h_tgt": 0.5})
+    ks = np.linspace(0.0, 20.0, 8) / 3.6e6
+    psi = np.linspace(0.05, 0.3, 8)
+    dth = np.linspace(0.2, 0.4, 8)
+    cls_pad = np.zeros((fm.nxp, fm.nyp), np.uint8)
+    cls_pad[g:g + n, g:g + n] = case.cls
+    tau = np.zeros((fm.nxp, fm.nyp), np.float32)
+    tau[ij[act_ids[::3], 0], ij[act_ids[::3], 1]] = 1.0 / 7200.0
+    f.set_ga_drain({"cls_pad": cls_pad, "Ks_t": ks, "psi_t": psi, "dth_t": dth, "F_pad": None,
+                    "Fmax_pad": np.full((fm.nxp, fm.nyp), 0.3, np.float32), "inv_tau_pad": tau, "mode": "fused"})
+    del cls_pad
     out = {"cells": {"sponge_band": int(in_band.sum()), "gauge_ring": int(len(ring)), "clamp": int(len(cl)),
-                     "drain": int(len(dr)), "infiltration": int(fm.n_active)}}
+                     "drain": int(len(dr)), "infiltration": int(fm.n_active), "green_ampt": int(fm.n_active),
+                     "drain_tau": int(len(act_ids[::3]))}}
     if fields:
         kg = np.ones((n, n))
         ag = np.zeros((n, n))
@@ -190,7 +203,11 @@ def attach_coastal_forcings(case: CoastalCase, f, band: int = 32, n_clamp: int =
         cg[ii[cl], jj[cl]] = 1.5
         dg = np.full((n, n), np.inf)
         dg[ii[dr], jj[dr]] = 0.5
+        taug = np.zeros((n, n))
+        taug[ii[act_ids[::3]], jj[act_ids[::3]]] = np.float32(1.0 / 7200.0)
         out["fields"] = {"keep": kg.astype(np.float32), "amb": ag.astype(np.float32), "clamp": cg, "drain": dg,
+                         "ga": {"cls": case.cls, "Ks": ks, "psi": psi, "dth": dth, "F": None,
+                                "Fmax": np.full((n, n), 0.3), "inv_tau": taug},
                          "infil": np.linspace(0.0, 10.0, 8) / 3.6e6,
                          "ring": {"cells": np.stack([ii[ring], jj[ring]], axis=1), "bed": bed_f[ring],
                                   "w": w.astype(np.float32), "t_common": tc, "stage_all": S}}

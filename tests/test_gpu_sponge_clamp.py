@@ -358,8 +358,9 @@ def test_ensemble_carries_infiltration_and_active_drain(gpu):
 
 def test_coastal_case_with_its_forcing_tail_matches_oracle(gpu):
     """The forcing tail the coastal benchmark attaches (infiltration by class, edge sponge,
-    four-gauge IDW tide ring, clamp, drain) on the coastal case at test size, with its
-    gridded rain and tidal stage: 25 flat steps bitwise the oracle's."""
+    four-gauge IDW tide ring, Green-Ampt with drain-tau, clamp, drain) on the coastal case at
+    test size, with its gridded rain and tidal stage: 25 flat steps bitwise the oracle's,
+    the cumulative infiltration F included."""
     from geoswe_amd.cases import attach_coastal_forcings, coastal_c3
     case = coastal_c3(n=192, band=8, n_sites=24, rain_frames=2)
     f = case.flat()
@@ -374,7 +375,7 @@ def test_coastal_case_with_its_forcing_tail_matches_oracle(gpu):
                      active=case.active,
                      stage={"cells": np.argwhere(case.sea_edge), "time_s": sb.time_s, "stage_m": sb.stage_m},
                      sponge=(F["keep"], F["amb"]), ring=F["ring"], clamp=F["clamp"], drain=F["drain"],
-                     infil=(case.cls, F["infil"]))
+                     infil=(case.cls, F["infil"]), ga=F["ga"])
     o = O.DenseOracle(oc, _physics(f.cfg), np.float32)
     o.cfl_skip = ~case.active
     o.run_steps(25)
@@ -384,6 +385,8 @@ def test_coastal_case_with_its_forcing_tail_matches_oracle(gpu):
     n = case.mesh.nx
     inner = (ij[:, 0] >= 0) & (ij[:, 0] < n) & (ij[:, 1] >= 0) & (ij[:, 1] < n)
     assert np.array_equal(f.q_flat()[:, inner], o.interior[:, ij[inner, 0], ij[inner, 1]])
+    act = f.fm.active
+    assert np.array_equal(f.ga_F()[act], o.ga_F[ij[act, 0], ij[act, 1]]) and (f.ga_F()[act] > 0).any()
 
 
 def _ga_setup(dtype, mode, seed=41):
