@@ -240,10 +240,11 @@ def coastal_forcing_tail(case, args, plain_ms):
     cells = attach_coastal_forcings(case, f)["cells"]
     log(f"forcing tail attached in {time.perf_counter() - t0:.0f} s: {cells}")
     timed_steps(f, 1)
-    _, ms = timed_steps(f, args.steps - 1)
+    k = max(args.steps - 1, 1)
+    _, ms = timed_steps(f, k)
     if f._clockv().status & L.ST_NONFINITE:
         raise FloatingPointError("non-finite state in the forced coastal run")
-    ms /= args.steps - 1
+    ms /= k
     log(f"with the forcing tail: {ms:.3f} ms/step against {plain_ms:.3f} ms/step plain "
         f"({(ms / plain_ms - 1) * 100:+.1f} %)")
     return {"ms_per_step": ms, "plain_ms_per_step": plain_ms, "overhead": ms / plain_ms - 1.0, "cells": cells,

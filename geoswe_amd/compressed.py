@@ -343,6 +343,8 @@ class CompressedSolver:
         if mode not in ("auto", "stage", "extrapolate", "hybrid", "stage_uv", "off", "none"):
             raise ValueError(f"set_ring_mode: mode {mode!r}: expected auto|stage|extrapolate|hybrid|stage_uv|off")
         self._rmode = None
+        if getattr(self, "_bed0", None) is not None:    # a second call starts from the packed bed
+            self._bed.upload(self._bed0)
         if mode in ("off", "none") or (mode != "extrapolate" and eta is None):
             return self
         fm = self.fm
@@ -356,7 +358,9 @@ class CompressedSolver:
             best = np.where(ok, cand, best)
         gi = np.nonzero(best >= 0)[0]
         gn = best[gi]
-        bed = self._bed.download()
+        if getattr(self, "_bed0", None) is None:      # the packed bed, before any replication
+            self._bed0 = self._bed.download()
+        bed = self._bed0.copy()
         if mode == "extrapolate":
             bed[gi] = bed[gn]
             ids, nbs, kind = gi, gn, np.zeros(len(gi), np.uint8)
