@@ -296,7 +296,8 @@ __device__ __forceinline__ void infil_cell(T rate, T dt, T hmin, T& h, T& hu, T&
 // exp(x) in float64 from IEEE +, -, *, rint and ldexp only, so that it is the oracle's
 // exp_ieee bit for bit: Cody-Waite reduction x = k ln2 + r (ln2 split in a 32-bit head, so
 // k*LN2_HI is exact, and a tail), exp(r) on |r| <= ln2/2 by the degree-13 Taylor polynomial in
-// Horner form, scaled by 2^k. It replaces the fast exponential of the reference's drain decay.
+// Horner form, scaled by 2^k. It replaces the fast exponential of the reference's drain decay
+// (float64 state; float32 state uses exp_ieee_f below).
 __device__ __forceinline__ double exp_ieee(double x) {
   if (!(x == x)) return x;
   if (x > 709.8) return __builtin_inf();
@@ -318,6 +319,25 @@ __device__ __forceinline__ double exp_ieee(double x) {
   p = p * r + 1.0;
   p = p * r + 1.0;
   return ldexp(p, (int)kf);
+}
+
+// The float32 form: ln2 split in a 15-bit head (k*LN2_HI exact for |k| < 512) and a tail, a
+// degree-7 Taylor polynomial (truncation below 5e-9 relative), every operation in float32.
+__device__ __forceinline__ float exp_ieee_f(float x) {
+  if (!(x == x)) return x;
+  if (x > 88.73f) return __builtin_inff();
+  if (x < -103.98f) return 0.0f;
+  const float kf = rintf(x * 1.4426950216293335f);
+  const float r = (x - kf * 0.693145751953125f) - kf * 1.428606765330187e-06f;
+  float p = 0.00019841270113829523f;
+  p = p * r + 0.0013888889225199819f;
+  p = p * r + 0.008333333767950535f;
+  p = p * r + 0.0416666679084301f;
+  p = p * r + 0.1666666716337204f;
+  p = p * r + 0.5f;
+  p = p * r + 1.0f;
+  p = p * r + 1.0f;
+  return ldexpf(p, (int)kf);
 }
 
 // Green-Ampt infiltration then drain-tau of one active cell (ga_drain_flat); `ob` is the
@@ -354,7 +374,10 @@ __device__ __forceinline__ void ga_cell(const FlatCtx<T>& x, int64_t base, uint3
   if (x.ga_inv_tau) {
     const T it = gat(x.ga_inv_tau + base, off);
     if (it > T(0) && h > T(0)) {
-      const T decay = (T)exp_ieee((double)((-x.dt) * it));
+      const T arg = (-x.dt) * it;
+      T decay;
+      if constexpr (sizeof(T) == 4) decay = exp_ieee_f(arg);
+      else decay = exp_ieee(arg);
       scale = scale * decay;
       h = h * decay;
     }

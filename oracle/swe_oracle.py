@@ -531,6 +531,35 @@ def exp_ieee(x) -> np.ndarray:
     return out
 
 
+_EXPF_C = tuple(np.float32(c) for c in (0.00019841270113829523, 0.0013888889225199819, 0.008333333767950535,
+                                          0.0416666679084301, 0.1666666716337204, 0.5, 1.0, 1.0))
+_LN2_HI_F, _LN2_LO_F, _INV_LN2_F = np.float32(0.693145751953125), np.float32(1.428606765330187e-06), \
+    np.float32(1.4426950216293335)
+
+
+def exp_ieee_f32(x) -> np.ndarray:
+    """``exp(x)`` in float32 from IEEE float32 +, -, *, round-to-nearest-integer and ``ldexp``
+    only (the float32 form of :func:`exp_ieee`): Cody-Waite reduction with ln2 split in a
+    15-bit head (k ln2_hi exact for |k| < 512) and a tail, a degree-7 Taylor polynomial in
+    Horner form (truncation below 5e-9 relative), scaled by 2^k. 0 below -103.98, +inf above
+    88.73, NaN kept. Pinned against ``numpy.exp`` (<= 2 ulp)."""
+    x = np.asarray(x, np.float32)
+    out = np.empty_like(x)
+    big, small, nan = x > np.float32(88.73), x < np.float32(-103.98), x != x
+    mid = ~(big | small | nan)
+    xm = x[mid]
+    kf = np.rint(xm * _INV_LN2_F)
+    r = (xm - kf * _LN2_HI_F) - kf * _LN2_LO_F
+    p = np.full_like(r, _EXPF_C[0])
+    for c in _EXPF_C[1:]:
+        p = p * r + c
+    out[mid] = np.ldexp(p, kf.astype(np.int32))
+    out[big] = np.inf
+    out[small] = 0.0
+    out[nan] = x[nan]
+    return out
+
+
 def green_ampt_drain(q_int: np.ndarray, F: np.ndarray, ga: dict, dt: float, active=None) -> None:
     """Fused Green-Ampt infiltration and linear-reservoir drain (ga_drain_flat,
     compressed_solver.py:320-367), in place on a (3, nx, ny) state and the cumulative
@@ -540,7 +569,8 @@ def green_ampt_drain(q_int: np.ndarray, F: np.ndarray, ga: dict, dt: float, acti
     ``disc = a*a + ((4 K dt) (psi[c] + h)) dth[c]``, ``F1 = (a + sqrt(max(disc, 0)))/2``,
     ``dF = min(max(F1 - F0, 0), h)`` capped by the room ``Fmax - F0`` (0 when none left),
     ``h <- h - dF``, ``F <- F0 + dF``, ``scale = h'/h``; then with ``it = inv_tau > 0`` and
-    ``h > 0``: ``decay = exp_ieee(-dt*it)`` (state dtype), ``scale *= decay``,
+    ``h > 0``: ``decay = exp(-dt*it)`` by :func:`exp_ieee_f32` in float32 or :func:`exp_ieee` in
+    float64, ``scale *= decay``,
     ``h *= decay``; momentum times ``scale``. The ``ga`` and ``drain`` modes of the reference
     (ga_flat, drain_tau_flat, 369-421) are this with one part left out, bit for bit."""
     dtp = q_int.dtype
@@ -581,7 +611,8 @@ def green_ampt_drain(q_int: np.ndarray, F: np.ndarray, ga: dict, dt: float, acti
     if ga.get("inv_tau") is not None:
         it = np.asarray(ga["inv_tau"]).astype(dtp)[on]
         d = (it > 0) & (hh > 0)
-        decay = exp_ieee(((-dtv) * it[d]).astype(np.float64)).astype(dtp)
+        arg = (-dtv) * it[d]
+        decay = exp_ieee_f32(arg) if dtp == np.float32 else exp_ieee(arg.astype(np.float64))
         scale[d] = scale[d] * decay
         hh[d] = hh[d] * decay
     h[on] = hh

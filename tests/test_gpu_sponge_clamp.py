@@ -593,3 +593,19 @@ def test_ring_modes_match_oracle(gpu, mode, trav):
     assert np.array_equal(c.q_flat()[:, act], o.interior[:, ia[:, 0], ia[:, 1]])
     if mode != "extrapolate":
         assert (kind > 0).any()
+
+
+def test_ring_mode_second_call_starts_from_the_packed_bed(gpu):
+    from geoswe_amd import CompressedSolver
+    q0, bed = rough_state(NX, NY, seed=72, depth=0.7, rough=1.0, vel=0.3)
+    m = irregular_mask(NX, NY, seed=9)
+    s = dense(q0 * m, bed, friction="manning", manning_n=0.03)
+    s.set_inside_mask(m)
+    c = CompressedSolver.from_dense(s, cfl_linf=True)
+    b0 = c._bed.download()
+    c.set_ring_mode("extrapolate")
+    assert not np.array_equal(c._bed.download(), b0)
+    c.set_ring_mode("auto", eta=0.5)
+    assert np.array_equal(c._bed.download(), b0) and c._rmode["mode"] == "auto"
+    c.set_ring_mode("off")
+    assert c._rmode is None and np.array_equal(c._bed.download(), b0)

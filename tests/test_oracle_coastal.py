@@ -337,3 +337,18 @@ def test_ring_modes_closed_forms():
     assert np.isclose(q[2, 0, 1] / q[0, 0, 1], -0.08 / 0.8, rtol=1e-6)
     assert q[0, 3, 3] == 0 and q[1, 3, 3] == 0                          # bed above eta: dry
     assert q[0, 1, 2] == np.float32(0.5) and q[1, 2, 3] == 0 and q[0, 2, 3] == np.float32(0.5)
+
+
+def test_exp_ieee_f32_within_two_ulp_of_exp():
+    rng = np.random.default_rng(12)
+    x = np.concatenate([np.linspace(-87.0, 88.0, 300001), rng.uniform(-30.0, 0.0, 100000),
+                        rng.uniform(-1e-3, 1e-3, 10000), [0.0, -1.0, 1.0]]).astype(np.float32)
+    got = O.exp_ieee_f32(x).astype(np.float64)
+    want = np.exp(x.astype(np.float64))
+    ulps = np.abs(got - want) / np.spacing(want.astype(np.float32)).astype(np.float64)
+    assert ulps.max() <= 2.0, ulps.max()
+    assert O.exp_ieee_f32(np.float32([0.0]))[0] == 1.0
+    sp = O.exp_ieee_f32(np.float32([-np.inf, -200.0, np.inf, 100.0, np.nan]))
+    assert sp[0] == 0 and sp[1] == 0 and sp[2] == np.inf and sp[3] == np.inf and np.isnan(sp[4])
+    d = O.exp_ieee_f32(np.float32([-95.0, -103.0]))                    # subnormal results, rounded once
+    assert np.all(np.abs(d.astype(np.float64) - np.exp([-95.0, -103.0])) <= 1.5 * 2.0 ** -149)
