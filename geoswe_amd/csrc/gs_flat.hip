@@ -998,15 +998,16 @@ static unsigned flat_grid(int64_t nchunks) {
 }
 
 // The sweeps launch one wavefront per tile (four tiles per workgroup) when the tiles fill
-// the resident grid more than twice: the hardware then keeps every SIMD fed until the last
-// tile, where a resident grid striding over the tiles left a partial last round (and
-// measured 10-14 % slower on the coastal and full 640 M-cell cases). Fewer tiles run on the
-// resident grid, whose smaller launch measured 8 % faster on a 2048^2 domain.
+// the resident grid more than four times: the hardware then keeps every SIMD fed until the
+// last tile, where a resident grid striding over the tiles left a partial last round (and
+// measured 10-14 % slower on the coastal case, 4.9 rounds, and the full 640 M-cell grid,
+// 23). Fewer tiles run on the resident grid, whose smaller launch measured 8 % faster on a
+// 2048^2 domain (1.3 rounds) and kept the mid-size ensemble members' plain runs faster.
 template <typename K>
 static unsigned sweep_grid(K kern, std::atomic<int>* cache, int64_t ntiles) {
   const unsigned resident = resident_grid(kern, cache, ntiles);
   const int64_t need = (ntiles + kFBlock / 64 - 1) / (kFBlock / 64);
-  if (need <= 2 * (int64_t)resident) return resident;
+  if (need <= 4 * (int64_t)resident) return resident;
   return (unsigned)need;
 }
 
@@ -1533,19 +1534,24 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
     return GS_OK;
   };
   const int occ = e->occ;
-  int per_cu[4] = {1, 2, occ > 1 ? occ / 2 : 1, occ};
+  // resident grids of 1, 2, occ/2 and occ workgroups per CU, and (last) one wavefront per
+  // tile, which the launch reaches by capping a larger grid at the tiles' need
+  int per_cu[5] = {1, 2, occ > 1 ? occ / 2 : 1, occ, 0};
+  const int64_t per_tile = (e->ntiles + 3) / 4;
   const int tpps[2] = {1, 2};
   hipEvent_t t0 = nullptr, t1 = nullptr;
   (void)hipEventCreate(&t0);
   (void)hipEventCreate(&t1);
   double best = 1e300;
   int best_b = e->blocks, best_c = e->tpp;
-  for (int a = 0; a < 4 && rc == GS_OK; ++a) {
+  for (int a = 0; a < 5 && rc == GS_OK; ++a) {
     bool dup = false;
     for (int b = 0; b < a; ++b) dup = dup || per_cu[b] == per_cu[a];
-    if (dup || per_cu[a] > occ) continue;
+    if (a < 4 && (dup || per_cu[a] > occ)) continue;
+    if (a == 4 && per_tile <= (int64_t)occ * e->cus) continue;   // already a resident grid
     for (int c = 0; c < 2 && rc == GS_OK; ++c) {
-      e->blocks = per_cu[a] * e->cus;
+      if (a == 4 && c > 0) continue;
+      e->blocks = a < 4 ? per_cu[a] * e->cus : (int)(per_tile < 0x7fffffff ? per_tile : 0x7fffffff);
       e->tpp = tpps[c];
       rc = restore();
       if (!rc) rc = ens_step(e, s);   // warm-up

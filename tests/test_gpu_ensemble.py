@@ -64,8 +64,10 @@ def test_ensemble_launch_shapes_and_autotune(cases):
     ens.advance(10)
     ens.set_launch(blocks=3, tiles_per_pass=5)        # a small, odd grid: every tile still visited
     ens.advance(10)
+    ens.set_launch(blocks=10 ** 6, tiles_per_pass=1)  # capped at one wavefront per tile
+    ens.advance(5)
     ens.set_launch(blocks=0, tiles_per_pass=1)
-    ens.advance(10, graph_batch=4)
+    ens.advance(5, graph_batch=4)
     assert_same(plain, mem)
     ens.close()
 
