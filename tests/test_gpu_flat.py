@@ -264,3 +264,31 @@ def test_frames_in_reference_layout(gpu, tmp_path):
         assert z["h"].dtype == np.float16 and np.array_equal(z["h"], h), name
     man = json.load(open(fdir / "manifest.json"))
     assert man["nranks"] == 1 and man["frame_every_s"] == every and man["ranks"][0]["nx"] == 40
+
+
+def test_per_tile_launch_equals_chunk_traversal(gpu):
+    """A domain with more than four resident rounds of tiles launches one wavefront per tile
+    (2048 x 1040 cells, one-row tiles: about 36 900 tiles); its steps equal the chunk
+    traversal's bit for bit, for the plain and the extended sweep (Green-Ampt attached)."""
+    from geoswe_amd import CompressedSolver
+    nx, ny = 2048, 1040
+    q0, bed = rough_state(nx, ny, seed=81, depth=0.6, rough=0.8, vel=0.2)
+    rng = np.random.default_rng(8)
+    cls = rng.integers(0, 3, (nx, ny)).astype(np.uint8)
+    for ext in (False, True):
+        runs = []
+        for trav in ("tiles", "chunks"):
+            s = dense(q0, bed, friction="manning", manning_n=0.03, rainfall=20.0 / 3.6e6)
+            c = CompressedSolver.from_dense(s, cfl_linf=True, sweep_rows=1)
+            assert len(c.fm.tile_e0) > 4 * 7168
+            if ext:
+                g = s.mesh.ngh
+                c.set_ga_drain({"cls_pad": np.pad(cls, g), "Ks_t": [0.0, 1e-5, 4e-5], "psi_t": [0.1, 0.1, 0.2],
+                                "dth_t": [0.3, 0.3, 0.3], "F_pad": None,
+                                "inv_tau_pad": np.full((nx + 2 * g, ny + 2 * g), 1.0 / 600.0), "mode": "fused"})
+            c.traversal = trav
+            c.advance(6)
+            runs.append((c.q_flat(), c.nsteps, c.t))
+        (qa, na, ta), (qb, nb, tb) = runs
+        print("ext", ext, na, flush=True)
+        assert na == nb == 6 and ta == tb and np.array_equal(qa, qb)
