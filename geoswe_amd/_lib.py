@@ -120,6 +120,8 @@ _SIGS = {
     "gs_flat_step": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, _P, _P]),
     "gs_flat_residual": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), C.c_int, _P, _P, _P, _P]),
     "gs_flat_cfl": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, _P, _P]),
+    "gs_flat_launch_info": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "gs_flat_step_pre": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), GsForcing, _P, _P, _P, _P, _P]),
     "gs_flat_sample": (C.c_int, [C.POINTER(GsFlat), _P, _P, C.c_int, _P, _P]),
     "gs_adj_flat_step": (C.c_int, [C.POINTER(GsAdjFlat), C.POINTER(GsPhys), C.c_double, _P]),
@@ -149,6 +151,11 @@ _SIGS = {
     "gs_flat_mesh_build": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "gs_flat_mesh_free": (C.c_int, [_P]),
     "gs_gen_voronoi": (C.c_int, [_P, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, _P, _P, _P, C.c_int, _P]),
+    "gs_graph_begin": (C.c_int, [_P]),
+    "gs_graph_end": (C.c_int, [_P, C.POINTER(_P)]),
+    "gs_graph_nodes": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "gs_graph_launch": (C.c_int, [_P, _P]),
+    "gs_graph_destroy": (C.c_int, [_P]),
 }
 
 _LIB = None
@@ -244,11 +251,66 @@ def sync(stream=None) -> None:
     call("gs_stream_synchronize", stream)
 
 
-def phys_struct(g, h_min, h_cfl, cfl, dmin, vcap, friction, quadratic, keep_h, cfl_l2, n_classes) -> GsPhys:
+class StepGraph:
+    """Steps recorded into an executable HIP graph (gs_graph_*). ``replay()`` enqueues the
+    recorded launches on the owner's stream; ``close()`` releases the graph."""
+
+    def __init__(self, record, stream=None):
+        """Run ``record(capture_stream)`` under stream capture: every entry point it calls on
+        that stream is recorded instead of run."""
+        cap = C.c_void_p()
+        call("gs_stream_create", C.byref(cap))
+        self._h = None
+        self.stream = stream
+        try:
+            call("gs_graph_begin", cap)
+            err = None
+            try:
+                record(cap)
+            except BaseException as e:       # end the capture in any case, then re-raise
+                err = e
+            h = C.c_void_p()
+            rc = lib().gs_graph_end(cap, C.byref(h))
+            if err is not None:
+                if h.value:
+                    call("gs_graph_destroy", h)
+                raise err
+            if rc != 0:
+                raise GsError(f"gs_graph_end failed ({rc}): {lib().gs_last_error().decode(errors='replace')}")
+            self._h = h
+        finally:
+            call("gs_stream_destroy", cap)
+        n = C.c_int64(0)
+        call("gs_graph_nodes", self._h, C.byref(n))
+        self.nodes = int(n.value)
+
+    def replay(self, times: int = 1) -> None:
+        if self._h is None:
+            raise RuntimeError("StepGraph: closed")
+        for _ in range(int(times)):
+            call("gs_graph_launch", self._h, self.stream)
+
+    def close(self) -> None:
+        if self._h is not None:
+            call("gs_graph_destroy", self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def phys_struct(g, h_min, h_cfl, cfl, dmin, vcap, friction, quadratic, keep_h, cfl_l2, n_classes,
+                floor_depth=None) -> GsPhys:
+    """The step's scalars. ``floor_depth`` is the depth whose gravity-wave speed floors the
+    CFL wave speed: the dense solver's ``h_cfl`` (solver.py:2663-2665, the default), the
+    flat solver's ``h_min`` (compressed_solver.py:713-719, 741-745)."""
     p = GsPhys()
     p.g, p.h_min, p.h_cfl, p.cfl, p.dmin = float(g), float(h_min), float(h_cfl), float(cfl), float(dmin)
-    # the dry-partition floor, with the reference's own host expression (solver.py:2663-2665)
-    p.lam_floor = (float(g) * float(h_cfl)) ** 0.5
+    # the dry-partition floor, with the reference's own host expression
+    p.lam_floor = (float(g) * float(h_cfl if floor_depth is None else floor_depth)) ** 0.5
     p.vcap = float(vcap) if vcap is not None else math.inf
     p.friction, p.quadratic, p.keep_h, p.cfl_l2 = int(friction), int(quadratic), int(keep_h), int(cfl_l2)
     p.n_classes = int(n_classes)

@@ -28,11 +28,22 @@ class CompressedSolver:
     """Flat-storage solver over the active cells plus a 2-cell ring (compressed_solver.py:2882)."""
 
     def __init__(self, mesh, cfg: Config, fm, q_pad: np.ndarray, b_pad: np.ndarray, flags_pad: np.ndarray,
-                 ntab: np.ndarray, n_field_pad: Optional[np.ndarray], cfl_linf: bool, track_max_depth: bool = True):
+                 ntab: np.ndarray, n_field_pad: Optional[np.ndarray], cfl_linf: bool, track_max_depth: bool = True,
+                 h_min_cfl: Optional[float] = None):
         cfg.require_production_scheme("CompressedSolver")
         self.mesh, self.cfg, self.fm = mesh, cfg, fm
         self.dtype = np.dtype(cfg.dtype)
         self.cfl_linf = bool(cfl_linf)
+        # The CFL velocity-divisor floor of the flat layout (CompressedStepper,
+        # compressed_solver.py:599-604): the argument, else SWE_HMIN_CFL, else h_min, floored at
+        # 1e-6 in float32. As there, the dense Config's h_min_cfl does not carry over.
+        if h_min_cfl is None:
+            env = os.environ.get("SWE_HMIN_CFL", "")
+            h_min_cfl = float(env) if env else float(cfg.h_min)
+        h_min_cfl = float(h_min_cfl)
+        if not (h_min_cfl > 0.0 and math.isfinite(h_min_cfl)):
+            raise ValueError(f"CompressedSolver: h_min_cfl={h_min_cfl} must be positive and finite")
+        self.h_min_cfl = max(h_min_cfl, 1.0e-6) if self.dtype == np.float32 else h_min_cfl
         N = fm.n_stored
         qf = fm.pack(q_pad).astype(self.dtype)
         self._bufs = [[L.DeviceArray.from_host(np.ascontiguousarray(qf[k])) for k in range(3)] for _ in range(2)]
@@ -60,7 +71,7 @@ class CompressedSolver:
         # "tiles": the step sweeps 64-column windows down the rows, each face once (default);
         # "chunks": it walks 60-id chunks of the run table (a second traversal, same bits)
         self.traversal = "tiles"
-        self._fo = _Forcing(cfg, min(mesh.dx, mesh.dy), self.dtype, (mesh.nx, mesh.ny))
+        self._fo = self._forcing()
         self.stream = None
         self._clock = L.new_clock(0.0)
         self._lam_valid = False
@@ -71,9 +82,11 @@ class CompressedSolver:
     # ------------------------------------------------------------------ construction
     @classmethod
     def from_dense(cls, s: Solver2D, cfl_linf: bool = False, track_max_depth: bool = True, sweep_rows: int = None,
-                   **_ignored):
+                   h_min_cfl: Optional[float] = None, **_ignored):
         """Pack a dense solver (its current state, bed, Manning, mask, forcings). `sweep_rows`:
-        rows per row-sweep tile of the step (None: chosen from the domain's size)."""
+        rows per row-sweep tile of the step (None: chosen from the domain's size). `h_min_cfl`:
+        the CFL velocity-divisor floor of the flat wave speed (the reference's
+        CompressedStepper argument; None: SWE_HMIN_CFL, else h_min)."""
         mesh, cfg = s.mesh, s.cfg
         g, nx, ny = mesh.ngh, mesh.nx, mesh.ny
         nxp, nyp = nx + 2 * g, ny + 2 * g
@@ -90,11 +103,18 @@ class CompressedSolver:
         if s._n_field is not None:
             nf = np.zeros((nxp, nyp), s.dtype)
             nf[g:g + nx, g:g + ny] = s._n_field.download()[:, :ny]
-        out = cls(mesh, cfg, fm, q_pad, b_pad, flags, ntab, nf, cfl_linf, track_max_depth)
+        out = cls(mesh, cfg, fm, q_pad, b_pad, flags, ntab, nf, cfl_linf, track_max_depth, h_min_cfl)
         c = s._clockv()
         if c.t != 0.0:
             L.call("gs_clock_reset", C.c_void_p(out._clock.ptr), C.c_double(float(c.t)), None)
         return out
+
+    def _forcing(self) -> _Forcing:
+        """Device rain and stage tables; the rain-film step bound from the largest rate of the
+        whole table, as the reference's flat run sets it (compressed_solver.py:3429-3436)."""
+        fo = _Forcing(self.cfg, min(self.mesh.dx, self.mesh.dy), self.dtype, (self.mesh.nx, self.mesh.ny))
+        fo.use_global_rain_cap()
+        return fo
 
     @property
     def n_active(self) -> int:
@@ -126,7 +146,7 @@ class CompressedSolver:
             lk = np.where(inb, forcing.lookup[np.clip(ii, 0, nx - 1), np.clip(jj, 0, ny - 1)], 0)
             lookup = L.DeviceArray.from_host(np.ascontiguousarray(lk.astype(np.int32)))
         self.cfg.rainfall_forcing = forcing
-        self._fo = _Forcing(self.cfg, min(self.mesh.dx, self.mesh.dy), self.dtype, (self.mesh.nx, self.mesh.ny))
+        self._fo = self._forcing()
         self._rain_lookup = lookup
         self._lam_valid = False
         self._target = None
@@ -422,9 +442,13 @@ class CompressedSolver:
     # ------------------------------------------------------------------ structs
     def _phys(self) -> L.GsPhys:
         cfg = self.cfg
-        return L.phys_struct(cfg.g, cfg.h_min, cfg.h_cfl(), cfg.cfl, min(self.mesh.dx, self.mesh.dy),
+        # the flat wave speed skips h < h_min and divides by max(h, h_min_cfl); its floor is
+        # sqrt(g*h_min) (compressed_solver.py:200-240, 713-719); dmin is the reference's dx on its
+        # square cells
+        return L.phys_struct(cfg.g, cfg.h_min, self.h_min_cfl, cfg.cfl, min(self.mesh.dx, self.mesh.dy),
                              cfg.friction_velocity_cap_ms, cfg.friction == "manning_implicit",
-                             cfg.friction_quadratic_alpha, True, not self.cfl_linf, self._n_classes)
+                             cfg.friction_quadratic_alpha, True, not self.cfl_linf, self._n_classes,
+                             floor_depth=cfg.h_min)
 
     def _desc(self) -> L.GsFlat:
         f = L.GsFlat()
@@ -538,6 +562,41 @@ class CompressedSolver:
             L.sync(self.stream)
             self._raise_if_bad(self._clockv())
 
+    def launch_info(self) -> dict:
+        """The step kernel's launch for this solver: ``kernel`` ("chunks", "sweep" or
+        "sweep_ext"), ``blocks`` (its grid) and ``resident`` (that kernel's resident grid on
+        this device; ``blocks > resident`` means one wavefront per tile)."""
+        k, b, r = C.c_int(0), C.c_int64(0), C.c_int64(0)
+        L.call("gs_flat_launch_info", C.byref(self._desc()), C.byref(self._phys()), self._fo.s, C.byref(k),
+               C.byref(b), C.byref(r))
+        return {"kernel": ("chunks", "sweep", "sweep_ext")[k.value], "blocks": int(b.value),
+                "resident": int(r.value)}
+
+    def capture_steps(self, nsteps: int) -> L.StepGraph:
+        """Record ``nsteps`` CFL steps (gs_flat_step, with its ring passes) into a HIP graph;
+        ``replay()`` enqueues them on this solver's stream, bit for bit the direct launches."""
+        if int(nsteps) < 1:
+            raise ValueError("capture_steps: nsteps must be >= 1")
+        self._ensure_lam()
+        self._open_mode()
+        L.sync(self.stream)
+        d, p = self._desc(), self._phys()
+        clk = C.c_void_p(self._clock.ptr)
+        fo = self._fo.s
+
+        def record(cap):
+            for _ in range(int(nsteps)):
+                L.call("gs_flat_step", C.byref(d), C.byref(p), fo, clk, cap)
+
+        g = L.StepGraph(record, self.stream)
+        g._keep = (d, p, self)
+        self._any_step = True
+        return g
+
+    def sync(self) -> None:
+        L.sync(self.stream)
+        self._raise_if_bad(self._clockv())
+
     # ------------------------------------------------------------------ checkpoints
     def save_cache(self, cache_dir: str, x0: float = 0.0, y0: float = 0.0, crs_wkt: str = "") -> None:
         """Write this solver in the reference's cache layout (compressed_solver.py:2504-2569);
@@ -596,6 +655,11 @@ class CompressedSolver:
             a = z[f"q{k}"]
             if a.shape != (N,):
                 raise ValueError(f"checkpoint q{k} has shape {a.shape}, the mesh stores {N} cells")
+        if self._ga is not None and "F" in self._ga and "F" not in z.files:
+            # as the reference (compressed_solver.py:1911-1918): resuming would reset the
+            # cumulative infiltration silently
+            raise RuntimeError("load_checkpoint: Green-Ampt is on but the checkpoint holds no cumulative "
+                               "infiltration F (written without Green-Ampt); resuming would reset it to zero")
         L.call("gs_clock_reset", C.c_void_p(self._clock.ptr), C.c_double(t), self.stream)
         c = self._clockv()
         c.nsteps = steps
@@ -606,7 +670,7 @@ class CompressedSolver:
             a = np.ascontiguousarray(z[f"q{k}"], dtype=self.dtype)
             for par in (0, 1):
                 self._bufs[par][k].upload(a, self.stream)
-        if self._ga is not None and "F" in self._ga and "F" in z.files:
+        if self._ga is not None and "F" in self._ga:
             self._ga["F"].upload(np.ascontiguousarray(z["F"], dtype=self.dtype), self.stream)
         if self._max_h is not None:
             if "max_h" in z.files:

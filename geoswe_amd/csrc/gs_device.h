@@ -208,21 +208,29 @@ GS_HD void update(T q0, T q1, T q2, T rr0, T r1, T r2, T dt, T n,
 }
 
 // CFL wave speed of one cell as float64 bits; 0 for a skipped cell, canonical NaN bits
-// for a NaN speed (solver.py:152-205 expressions; compressed_solver.py:200-240 NaN rule).
-template <typename T>
+// for a NaN speed. The two layouts follow their own reference kernels:
+//   dense (FLAT = false), lammax_fp32_lean (solver.py:152-205): cells with h < h_cfl are
+//     skipped and the velocity divides by h;
+//   flat (FLAT = true), cfl_lammax_flat_blk and the fused step's lambda
+//     (compressed_solver.py:200-240, compressed_rhs.py:1041-1053): cells with h < h_min are
+//     skipped and the velocity divides by max(h, h_cfl) (h_min_cfl there).
+// Both keep a NaN speed (compressed_solver.py:200-240 NaN rule).
+template <typename T, bool FLAT = false>
 __device__ __forceinline__ uint64_t lam_bits(T h, T hu, T hv, const Phys<T>& P) {
-  if (h < P.h_cfl) return 0ull;
+  if (h < (FLAT ? P.h_min : P.h_cfl)) return 0ull;
+  const T hs = FLAT ? (h > P.h_cfl ? h : P.h_cfl) : h;
   T vel;
   if (P.cfl_l2) {
-    const T u = hu / h;
-    const T v = hv / h;
+    const T u = hu / hs;
+    const T v = hv / hs;
     vel = vsqrt(u * u + v * v);
   } else {
-    // max(|hu/h|, |hv/h|) = max(|hu|, |hv|)/h bit for bit: correctly rounded division by
-    // the same positive h is monotonic, so one division serves both components
+    // max(|hu/hs|, |hv/hs|) = max(|hu|, |hv|)/hs bit for bit: correctly rounded division
+    // by the same positive hs is monotonic, so one division serves both components
     const T au = vabs(hu), av = vabs(hv);
-    vel = ((au > av) ? au : av) / h;
+    vel = ((au > av) ? au : av) / hs;
   }
+  // h >= h_min > 0 (or NaN) here, so sqrt(g*max(h, 0)) of the flat kernels is sqrt(g*h)
   const T lam = vel + vsqrt(P.g * h);
   const double l = (double)lam;
   if (l != l) return 0x7FF8000000000000ull;

@@ -511,7 +511,7 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
   x.qo1[k] = hu;
   x.qo2[k] = hv;
   if (!(fg & GS_FLAG_CFL_SKIP)) {
-    const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
+    const uint64_t lb_ = lam_bits<T, true>(h, hu, hv, P);
     lam = lb_ > lam ? lb_ : lam;
   }
 }
@@ -672,7 +672,7 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
       gat(x.qo1 + gC, oC) = hu;
       gat(x.qo2 + gC, oC) = hv;
       if (!(fg & GS_FLAG_CFL_SKIP)) {
-        const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
+        const uint64_t lb_ = lam_bits<T, true>(h, hu, hv, P);
         lam = lb_ > lam ? lb_ : lam;
       }
     }
@@ -731,8 +731,10 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(sizeof(
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
   uint64_t lam = 0ull;
+  // PRE (the adjoint's recomputation) also compiles the extended forcings in, so that a
+  // replay reads rain through the per-cell lookup exactly as the forward run's extended sweep
   for (int64_t t = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv; t < a.n_tiles; t += nwaves)
-    flat_tile<T, PRE, false>(x, t, lane, lam);
+    flat_tile<T, PRE, PRE>(x, t, lane, lam);
   grid_reduce_and_plan(lam, a.clock, a.plan, true);
 }
 
@@ -873,7 +875,7 @@ __global__ __launch_bounds__(kFBlock) void k_flat_cfl(FlatK<T> a) {
     for (int t = threadIdx.x & 63; t < len; t += 64) {
       const int64_t k = k0 + t;
       if (a.flags && (a.flags[k] & GS_FLAG_CFL_SKIP)) continue;
-      const uint64_t l = lam_bits<T>(q[0][k], q[1][k], q[2][k], a.P);
+      const uint64_t l = lam_bits<T, true>(q[0][k], q[1][k], q[2][k], a.P);
       lam = l > lam ? l : lam;
     }
   }
@@ -1023,6 +1025,26 @@ static unsigned sweep_ext_grid(int64_t ntiles) {
   return sweep_grid(k_flat_sweep_ext<T>, cache, ntiles);
 }
 
+// The launch gs_flat_step makes for `a` (without the pre-friction state): which kernel, its
+// grid and the kernel's resident grid on this device.
+template <typename T>
+static void flat_launch_choice(const FlatK<T>& a, int* kernel, int64_t* blocks, int64_t* resident) {
+  static std::atomic<int> c_plain[64], c_ext[64];
+  if (a.n_tiles <= 0) {
+    *kernel = 0;
+    *blocks = flat_grid<T, 1>(a.nchunks);
+    *resident = *blocks;
+  } else if (a.ga_ks || a.ga_inv_tau || a.rain_mode == 3) {
+    *kernel = 2;
+    *blocks = sweep_ext_grid<T>(a.n_tiles);
+    *resident = resident_grid(k_flat_sweep_ext<T>, c_ext, a.n_tiles);
+  } else {
+    *kernel = 1;
+    *blocks = sweep_grid_for<T, false>(a.n_tiles);
+    *resident = resident_grid(k_flat_sweep<T, false>, c_plain, a.n_tiles);
+  }
+}
+
 // The fused step: row-sweep tiles when the descriptor has them, else chunks of the run table.
 template <typename T>
 static int flat_step_launch(const FlatK<T>& a, hipStream_t s) {
@@ -1058,6 +1080,9 @@ static int flat_step_t(const GsFlat* f, const GsPhys* p, GsForcing fo, GsClock* 
   FlatK<T> a;
   int rc = build_flat<T>(f, p, fo, clk, a, true);
   if (rc) return rc;
+  if ((pre0 || pre1 || pre2) && !(pre0 && pre1 && pre2)) return fail(GS_E_ARG, "pre: all three planes or none");
+  if (pre0 && a.n_tiles == 0)
+    return fail(GS_E_ARG, "the pre-friction state is written by the row-sweep traversal only (n_tiles > 0)");
   a.pre[0] = (T*)pre0;
   a.pre[1] = (T*)pre1;
   a.pre[2] = (T*)pre2;
@@ -1225,6 +1250,26 @@ int gs_flat_cfl(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* cl
   return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
 }
 
+int gs_flat_launch_info(const GsFlat* f, const GsPhys* phys, GsForcing forc, int* kernel, int64_t* blocks,
+                        int64_t* resident) {
+  using namespace gs;
+  if (!f || !kernel || !blocks || !resident) return fail(GS_E_ARG, "gs_flat_launch_info: NULL argument");
+  if (f->dtype == GS_F32) {
+    FlatK<float> a;
+    const int rc = build_flat<float>(f, phys, forc, nullptr, a, false);
+    if (rc) return rc;
+    flat_launch_choice<float>(a, kernel, blocks, resident);
+  } else if (f->dtype == GS_F64) {
+    FlatK<double> a;
+    const int rc = build_flat<double>(f, phys, forc, nullptr, a, false);
+    if (rc) return rc;
+    flat_launch_choice<double>(a, kernel, blocks, resident);
+  } else {
+    return fail(GS_E_DTYPE, "unsupported dtype %d", f->dtype);
+  }
+  return GS_OK;
+}
+
 int gs_flat_step_pre(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev, void* pre0,
                      void* pre1, void* pre2, void* stream) {
   using namespace gs;
@@ -1291,24 +1336,40 @@ struct GsEns {
 
 namespace gs {
 
+// Member m's view for k_ens_step, with the checks of what the ensemble step supports.
+template <typename T>
+static int ens_member_view(const GsFlat* members, const GsPhys* phys, const GsForcing* forcings,
+                           GsClock* const* clocks, int m, FlatK<T>& v) {
+  const int rc = build_flat<T>(&members[m], &phys[m], forcings[m], clocks[m], v, true);
+  if (rc) {
+    char msg[512];
+    snprintf(msg, sizeof msg, "%s", g_err);
+    return fail(rc, "member %d: %s", m, msg);
+  }
+  if (v.n_runs > 0 && v.n_tiles < 1) return fail(GS_E_ARG, "member %d has active cells but no row-sweep tiles", m);
+  if (v.n_rmode > 0) return fail(GS_E_ARG, "member %d has ring modes; the ensemble step does not apply them", m);
+  if (v.n_fring > 0)
+    return fail(GS_E_ARG, "member %d forces ring cells (gauge ring, or sponge or clamp outside the active set); the ensemble "
+                "step updates active cells only", m);
+  return GS_OK;
+}
+
+// Grid of one k_ens_step launch: `blocks` resident workgroups (0: one occupancy wave of the
+// device), never more than the tiles need.
+static int64_t ens_grid(int64_t blocks, int occ, int cus, int64_t ntiles, int tpp) {
+  if (blocks <= 0) blocks = (int64_t)occ * cus;
+  const int64_t need = (ntiles + 4LL * tpp - 1) / (4LL * tpp);
+  if (blocks > need) blocks = need;
+  return blocks < 1 ? 1 : blocks;
+}
+
 template <typename T>
 static int ens_build_views(GsEns* e, const GsFlat* members, const GsPhys* phys, const GsForcing* forcings) {
   std::vector<FlatK<T>> views((size_t)e->n);
   std::vector<int64_t> c0((size_t)e->n + 1, 0);
   for (int m = 0; m < e->n; ++m) {
-    const int rc = build_flat<T>(&members[m], &phys[m], forcings[m], e->clocks[m], views[m], true);
-    if (rc) {
-      char msg[512];
-      snprintf(msg, sizeof msg, "%s", g_err);
-      return fail(rc, "member %d: %s", m, msg);
-    }
-    if (views[m].n_runs > 0 && views[m].n_tiles < 1)
-      return fail(GS_E_ARG, "member %d has active cells but no row-sweep tiles", m);
-    if (views[m].n_rmode > 0)
-      return fail(GS_E_ARG, "member %d has ring modes; the ensemble step does not apply them", m);
-    if (views[m].n_fring > 0)
-      return fail(GS_E_ARG, "member %d forces ring cells (gauge ring, or sponge or clamp outside the active set); the ensemble "
-                  "step updates active cells only", m);
+    const int rc = ens_member_view<T>(members, phys, forcings, e->clocks.data(), m, views[m]);
+    if (rc) return rc;
     c0[m + 1] = c0[m] + views[m].n_tiles;
   }
   e->ntiles = c0[e->n];
@@ -1331,10 +1392,7 @@ static int ens_launch(GsEns* e, hipStream_t s) {
   k.tpp = e->tpp;
   k.ntiles = e->ntiles;
   k.done = e->done;
-  int64_t blocks = e->blocks > 0 ? e->blocks : (int64_t)e->occ * e->cus;
-  const int64_t need = (e->ntiles + 4LL * e->tpp - 1) / (4LL * e->tpp);
-  if (blocks > need) blocks = need;
-  if (blocks < 1) blocks = 1;
+  const int64_t blocks = ens_grid(e->blocks, e->occ, e->cus, e->ntiles, e->tpp);
   hipLaunchKernelGGL((k_ens_step<T>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k, k.mem, k.tile0);
   GS_HIP(hipGetLastError(), "k_ens_step launch");
   return GS_OK;
@@ -1494,7 +1552,12 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
   hipStream_t s = (hipStream_t)stream;
   const size_t es = e->dtype == GS_F64 ? 8 : 4;
   GS_HIP(hipStreamSynchronize(s), "sync");
-  // snapshot: the current state buffer, the running max and the clock of every member
+  // snapshot: the current state buffer, the running max, the Green-Ampt cumulative
+  // infiltration and the clock of every member (everything a step writes)
+  constexpr int kSnap = 5;
+  auto plane = [&](int m, int par, int k) -> void* {
+    return k < 3 ? e->flat[m].q[par][k] : (k == 3 ? e->flat[m].max_h : e->flat[m].ga_F);
+  };
   std::vector<GsClock> clk((size_t)e->n);
   std::vector<void*> snap;
   int rc = GS_OK;
@@ -1507,8 +1570,8 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
     if (h != hipSuccess) { rc = hip_fail(h, "clock d2h"); break; }
     const int par = (int)(clk[m].nsteps & 1);
     const size_t bytes = (size_t)e->flat[m].n_stored * es;
-    for (int k = 0; k < 4 && rc == GS_OK; ++k) {
-      const void* src = k < 3 ? e->flat[m].q[par][k] : e->flat[m].max_h;
+    for (int k = 0; k < kSnap && rc == GS_OK; ++k) {
+      const void* src = plane(m, par, k);
       void* p = nullptr;
       if (src) {
         h = hipMalloc(&p, bytes);
@@ -1523,9 +1586,9 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
     for (int m = 0; m < e->n; ++m) {
       const int par = (int)(clk[m].nsteps & 1);
       const size_t bytes = (size_t)e->flat[m].n_stored * es;
-      for (int k = 0; k < 4; ++k) {
-        void* dst = k < 3 ? e->flat[m].q[par][k] : e->flat[m].max_h;
-        void* p = snap[(size_t)m * 4 + k];
+      for (int k = 0; k < kSnap; ++k) {
+        void* dst = plane(m, par, k);
+        void* p = snap[(size_t)m * kSnap + k];
         if (dst && p) GS_HIP(hipMemcpyAsync(dst, p, bytes, hipMemcpyDeviceToDevice, s), "restore");
       }
       GS_HIP(hipMemcpyAsync(e->clocks[m], &clk[m], sizeof(GsClock), hipMemcpyHostToDevice, s), "clock restore");

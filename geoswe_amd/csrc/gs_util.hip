@@ -215,4 +215,61 @@ int gs_bench_valu(float* out, int blocks, int iters, int mode, void* stream) {
   return GS_OK;
 }
 
+// Stream capture of step entry points into an executable HIP graph.
+struct GsGraph {
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int64_t nodes;
+};
+
+int gs_graph_begin(void* stream) {
+  if (!stream) return fail(GS_E_ARG, "gs_graph_begin: capture needs a created stream, not the null stream");
+  GS_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  return GS_OK;
+}
+
+int gs_graph_end(void* stream, GsGraph** out) {
+  if (!stream || !out) return fail(GS_E_ARG, "gs_graph_end: stream and out are required");
+  *out = nullptr;
+  hipGraph_t g = nullptr;
+  GS_HIP(hipStreamEndCapture((hipStream_t)stream, &g), "hipStreamEndCapture");
+  if (!g) return fail(GS_E_STATE, "gs_graph_end: the capture recorded no graph");
+  size_t nn = 0;
+  hipError_t e = hipGraphGetNodes(g, nullptr, &nn);
+  hipGraphExec_t x = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return hip_fail(e, "gs_graph_end: instantiate");
+  }
+  GsGraph* r = new GsGraph;
+  r->graph = g;
+  r->exec = x;
+  r->nodes = (int64_t)nn;
+  *out = r;
+  return GS_OK;
+}
+
+int gs_graph_nodes(const GsGraph* g, int64_t* nodes) {
+  if (!g || !nodes) return fail(GS_E_ARG, "gs_graph_nodes: graph and nodes are required");
+  *nodes = g->nodes;
+  return GS_OK;
+}
+
+int gs_graph_launch(const GsGraph* g, void* stream) {
+  if (!g) return fail(GS_E_ARG, "gs_graph_launch: graph is NULL");
+  GS_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream), "hipGraphLaunch");
+  return GS_OK;
+}
+
+int gs_graph_destroy(GsGraph* g) {
+  if (!g) return GS_OK;
+  hipError_t e1 = hipGraphExecDestroy(g->exec);
+  hipError_t e2 = hipGraphDestroy(g->graph);
+  delete g;
+  if (e1 != hipSuccess) return hip_fail(e1, "hipGraphExecDestroy");
+  if (e2 != hipSuccess) return hip_fail(e2, "hipGraphDestroy");
+  return GS_OK;
+}
+
 }  // extern "C"

@@ -80,6 +80,8 @@ class _Forcing:
     def __init__(self, cfg: Config, dmin: float, dtype, frame_shape, frame_pitch=None):
         self.keep = []
         self.s = L.GsForcing()
+        self.dtype = np.dtype(dtype)
+        self.cfl, self.g, self.dmin = float(cfg.cfl), float(cfg.g), float(dmin)
         self.rain_host = rain_tables(cfg.rainfall_forcing, cfg.rainfall, cfg.cfl, cfg.g, dmin, dtype)
         if self.rain_host is not None:
             t, rate, cap, frames = self.rain_host
@@ -122,6 +124,18 @@ class _Forcing:
             self.keep += [st, sv]
             self.s.n_stage = len(sb.time_s)
             self.s.stage_t, self.s.stage_v = st.ptr, sv.ptr
+
+    def use_global_rain_cap(self) -> None:
+        """One rain-film bound for the whole run, from the largest rate of the table as the step
+        reads it (state dtype): the flat solver's rule (compressed_solver.py:3429-3436), where
+        the dense one bounds each step by its own segment's rate (solver.py:3005-3027)."""
+        if self.rain_host is None:
+            return
+        t, rate, cap, frames = self.rain_host
+        rmax = float(frames.max()) if frames is not None else float(np.max(rate.astype(self.dtype)))
+        c = np.inf if rmax <= 0.0 else (self.cfl * self.dmin) ** (2.0 / 3.0) / (self.g * rmax) ** (1.0 / 3.0)
+        self.rain_host = (t, rate, np.full(len(t), c), frames)
+        self.keep[2].upload(np.full(self.s.n_rain, c))
 
     def drop_rain_cap(self) -> None:
         """No rain-film step bound (the reference's cached replay applies none by default)."""
@@ -510,6 +524,34 @@ class Solver2D:
         if sync:
             L.sync(self.stream)
             self._raise_if_bad(self._clockv())
+
+    def capture_steps(self, nsteps: int) -> L.StepGraph:
+        """Record ``nsteps`` CFL steps (gs_dense_step launches) into a HIP graph whose
+        ``replay()`` enqueues them on this solver's stream with no host work per step: the
+        reference's captured fixed-dt loop (tests/test_gpu_dense_no_host_sync.py). dt, t and
+        the wave speed stay in the device clock, so a replay is the same launches run
+        directly, bit for bit. Synchronise (``sync``) before reading the state."""
+        if int(nsteps) < 1:
+            raise ValueError("capture_steps: nsteps must be >= 1")
+        self._ensure_lam()
+        self._open_mode()
+        L.sync(self.stream)
+        d, p = self._desc(), self._phys()
+        clk = C.c_void_p(self._clock.ptr)
+        fo = self._fo.s
+
+        def record(cap):
+            for _ in range(int(nsteps)):
+                L.call("gs_dense_step", C.byref(d), C.byref(p), fo, clk, cap)
+
+        g = L.StepGraph(record, self.stream)
+        g._keep = (d, p, self)      # the recorded arguments point at this solver's buffers
+        self._any_step = True
+        return g
+
+    def sync(self) -> None:
+        L.sync(self.stream)
+        self._raise_if_bad(self._clockv())
 
     def run(self, t_end: float, max_steps: int = 10 ** 7, callback: Optional[Callable] = None,
             dt_max: Optional[float] = None, batch: int = 64) -> int:

@@ -90,10 +90,14 @@ typedef struct GsClock {
 typedef struct GsPhys {
   double g;            /* gravity */
   double h_min;        /* wet/dry threshold */
-  double h_cfl;        /* CFL depth floor (h_min_cfl, floored at 1e-6 in float32) */
+  double h_cfl;        /* dense: CFL depth below which a cell is skipped (h_min_cfl, floored at
+                          1e-6 in float32); flat: the velocity divisor floor max(h, h_cfl) of
+                          the wave speed, whose cells are skipped below h_min (h_min_cfl of the
+                          flat stepper, compressed_solver.py:200-240, 599-604) */
   double cfl;          /* Courant number */
   double dmin;         /* min(dx, dy) */
-  double lam_floor;    /* (g*h_cfl)**0.5 computed by the host, solver.py:2663-2665 */
+  double lam_floor;    /* wave-speed floor computed by the host: (g*h_cfl)**0.5 dense
+                          (solver.py:2663-2665), (g*h_min)**0.5 flat (compressed_solver.py:713-719) */
   double vcap;         /* friction velocity cap (m/s); inf disables it */
   int32_t friction;    /* 0 none, 1 Manning (point-implicit) */
   int32_t quadratic;   /* 1 quadratic-alpha friction root, 0 linearised */
@@ -331,6 +335,13 @@ int gs_flat_residual(const GsFlat* f, const GsPhys* phys, int which, void* rhs0,
 int gs_flat_cfl(const GsFlat* f, const GsPhys* phys, GsForcing forc, GsClock* clock_dev,
                 void* stream);
 
+/* No reference counterpart (a launch query for tests and tuning): the kernel gs_flat_step
+ * launches for this descriptor (0 chunk traversal, 1 plain row sweep, 2 extended row sweep),
+ * its grid in workgroups and that kernel's resident grid on the current device (the grid
+ * exceeds it when the step launches one wavefront per tile). No device work. */
+int gs_flat_launch_info(const GsFlat* f, const GsPhys* phys, GsForcing forc, int* kernel, int64_t* blocks,
+                        int64_t* resident);
+
 /* Replaces: nothing in the reference (the adjoint recomputation needs it): gs_flat_step
  * that also stores the pre-friction state h + dt*rr0, hu + dt*r1, hv + dt*r2 of active
  * cells into pre0..2 (flat arrays of n_stored), the input of the update's reverse. */
@@ -446,6 +457,25 @@ int gs_ens_run(GsEns* e, int64_t nsteps, int graph_batch, void* stream);
  * restored afterwards), keep the fastest; writes its mean ms per step. Allocates scratch
  * for the snapshot. */
 int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms);
+
+/* ---- graph capture of step entry points ---- */
+/*
+ * Replaces: the reference's K-step capture of its fixed-dt dense step in a CUDA graph
+ * (tests/test_gpu_dense_no_host_sync.py:1-96 pins "K captured steps = K launches" byte for
+ * byte). Begin capture on a created stream, call any step entry points (gs_dense_step,
+ * gs_flat_step, ...) on that stream, end the capture into an executable graph, then launch
+ * it on any stream as often as needed: each launch replays the recorded kernels with the
+ * arguments they were recorded with. The step entry points keep dt, t and the wave speed in
+ * the device clock and record no memset or host synchronisation, so a replay advances the
+ * clock exactly as the same direct launches would. The buffers and tables a recorded call
+ * names must outlive the graph.
+ */
+typedef struct GsGraph GsGraph;
+int gs_graph_begin(void* stream);
+int gs_graph_end(void* stream, GsGraph** out);
+int gs_graph_nodes(const GsGraph* g, int64_t* nodes);
+int gs_graph_launch(const GsGraph* g, void* stream);
+int gs_graph_destroy(GsGraph* g);
 
 /* ---- set-up and measurement utilities ---- */
 /* Constant fill of a (rows, cols) plane with row pitch `pitch` (elements). No reference

@@ -11,11 +11,12 @@ Modules
                 step: residual, fused update (rain, Manning friction, wet/dry), CFL
                 wave-speed reduction and the dt rule, ghost-cell boundary fill, stage
                 boundary and rain forcing lookup.
-``flat_mesh``   Restatement of the compressed active-cell ("flat") mesh: ring-2 cross
-                dilation, row-major flat order, neighbour ids and the run table that the
-                HIP flat kernel reads.
 ``generators``  Seeded synthetic inputs (Philox value-noise fBm beds, Voronoi Manning
                 classes, correlated rain) shared bit-for-bit with the device generators.
+
+The flat cell order is not restated here: the product's own builder
+(``geoswe_amd.flatmesh``) is pinned directly by the reference's golden ``CompressedMesh2D``
+order and neighbour table (tests/test_oracle_golden.py).
 
 Parity status of each function is stated in its docstring ("pinned by ...").
 """

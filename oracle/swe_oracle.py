@@ -67,7 +67,8 @@ def bc_fill(q: np.ndarray, ngh: int, bc_x: str, bc_y: str) -> None:
 
     Restates bc.py:50-98: extrapolate copies the last interior layer, periodic wraps,
     wall mirrors with the normal momentum negated, fall zeroes the ghosts.
-    Pinned by tests/test_oracle_bc.py (hand-built 1-cell cases per kind).
+    Pinned by tests/test_oracle_invariants.py::test_ghost_fill_by_hand (hand-built cases per
+    kind) and by the reference's golden residuals under wall, extrapolate and periodic edges.
     """
     for axis, kind in ((1, bc_x), (2, bc_y)):
         n = q.shape[axis]
@@ -174,8 +175,9 @@ def hllc(hL, uNL, uTL, hR, uNR, uTR, g, h_min):
 
     Differs from flux.py:80-151 exactly as the production kernel does: cell velocities
     gated on the cell depth, no SM clamp (rhs_cuda.py:1001-1007), Sigma = 0.
-    Pinned by tests/test_oracle_hllc.py: consistency F(q,q) = physical flux, x/y and
-    mirror symmetry, both-dry zero, and the reference NumPy flux on wet states.
+    Pinned by tests/test_oracle_invariants.py: consistency F(q,q) = physical flux, mirror
+    antisymmetry, both-dry zero, rest pressure; and through the residual by the reference's
+    golden NumPy residuals and steps (tests/test_oracle_golden.py).
     """
     T = type(g)
     z = T(0)
@@ -361,6 +363,7 @@ class Physics:
     quadratic: bool = True       # friction_quadratic_alpha
     keep_h: bool = True          # SWE_WETDRY_ZERO_H unset (solver.py:428)
     cfl_l2: bool = False         # velocity norm of the CFL: L-inf (dense) or L2 (flat default)
+    flat_cfl: bool = False       # the flat layout's wave-speed rule and floor (see lam_max, dt_from_lam)
 
 
 def update(q_int, r_int, dt: float, ph: Physics, n_cell=None, rain=None, active=None):
@@ -371,6 +374,11 @@ def update(q_int, r_int, dt: float, ph: Physics, n_cell=None, rain=None, active=
     (already in the state dtype), added to ``rhs_h`` in the state precision; ``active``
     marks cells whose residual and rain are kept (inactive cells update with a zero
     residual, like the reference's dense carry kernel).
+    Pinned by the reference's own step (tests/golden/step_vcap_f64.npz): from its residual,
+    a class table with a fifth of the cells above the 15 m/s velocity cap, the quadratic and
+    the linear root, equal to 1e-14 of each field's scale (tests/test_oracle_golden.py); by
+    the bowl step golden; by the friction root and dry-cell closed forms
+    (tests/test_oracle_invariants.py).
     """
     T = q_int.dtype.type
     h_min = T(ph.h_min)
@@ -419,21 +427,33 @@ def update(q_int, r_int, dt: float, ph: Physics, n_cell=None, rain=None, active=
 
 
 def lam_max(q_int, ph: Physics, skip=None) -> float:
-    """Max over cells of the CFL wave speed, as a Python float (solver.py:152-205 + NaN kept).
+    """Max over cells of the CFL wave speed, as a Python float, NaN kept.
 
-    The speed is evaluated in the state dtype and maximised as float64 IEEE bits, the
-    order-independent reduction the device performs. Cells with ``h < h_cfl`` (or flagged
-    in ``skip``) are skipped; a NaN depth passes the test and yields a NaN lambda, which
-    dominates the bit maximum.
+    Dense rule (lammax_fp32_lean, solver.py:152-205): cells with ``h < h_cfl`` are skipped
+    and ``u = hu/h``. Flat rule (``ph.flat_cfl``; cfl_lammax_flat_blk and the fused step's
+    lambda, compressed_solver.py:200-240, compressed_rhs.py:1041-1053): cells with
+    ``h < h_min`` are skipped and ``u = hu/max(h, h_cfl)`` (``h_cfl`` is the flat solver's
+    ``h_min_cfl``). Speed = velocity norm (L-inf ``max(|u|,|v|)`` or L2 ``sqrt(u*u+v*v)``)
+    + ``sqrt(g*h)``, evaluated in the state dtype and maximised as float64 IEEE bits, the
+    order-independent reduction the device performs; a NaN depth passes the skip test and
+    yields a NaN lambda, which dominates the bit maximum.
+    Pinned by closed forms on hand-built states (tests/test_oracle_invariants.py), by the
+    reference's C1 dt sequence (golden) for the dense L-inf rule, and by the reference's
+    rain-on-dry-bed run (golden) for the floor.
     """
     T = q_int.dtype.type
     h, hu, hv = q_int[0], q_int[1], q_int[2]
     with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
-        take = ~(h < T(ph.h_cfl))
+        if ph.flat_cfl:
+            take = ~(h < T(ph.h_min))
+            hs = np.where(h > T(ph.h_cfl), h, T(ph.h_cfl))
+        else:
+            take = ~(h < T(ph.h_cfl))
+            hs = h
         if skip is not None:
             take &= ~(np.asarray(skip) != 0)
-        u = hu / h
-        v = hv / h
+        u = hu / hs
+        v = hv / hs
         if ph.cfl_l2:
             vel = np.sqrt(u * u + v * v)
         else:
@@ -449,17 +469,22 @@ def lam_max(q_int, ph: Physics, skip=None) -> float:
 
 
 def dt_from_lam(lam_max: float, ph: Physics, dmin: float) -> float:
-    """dt = cfl*min(dx,dy)/(max(lam, sqrt(g*h_cfl)) + 1e-12) in float64 (solver.py:2657-2666)."""
+    """dt = cfl*min(dx,dy)/(max(lam, floor) + 1e-12) in float64, with the floor
+    sqrt(g*h_cfl) of the dense solver (solver.py:2657-2666) or sqrt(g*h_min) of the flat one
+    (compressed_solver.py:713-719, 741-745)."""
     if not math.isfinite(lam_max):
         raise FloatingPointError(f"non-finite max wave speed {lam_max}")
-    floor = (ph.g * ph.h_cfl) ** 0.5
+    floor = (ph.g * (ph.h_min if ph.flat_cfl else ph.h_cfl)) ** 0.5
     if lam_max < floor:
         lam_max = floor
     return ph.cfl * dmin / (lam_max + 1.0e-12)
 
 
 def rain_step_cap(rmax: float, ph: Physics, dmin: float) -> float:
-    """The rain-film step bound (cfl*dmin)^(2/3)/(g*R)^(1/3) (solver.py:3005-3027); inf if dry."""
+    """The rain-film step bound (cfl*dmin)^(2/3)/(g*R)^(1/3) (solver.py:3005-3027); inf if dry.
+    Pinned by its fixed point dt = cfl*dmin/sqrt(g*R*dt) (tests/test_oracle_invariants.py) and
+    by the reference's rain-on-dry-bed run, whose first step it sets
+    (tests/golden/rain_dry_run_f64.npz)."""
     if rmax <= 0.0:
         return math.inf
     return (ph.cfl * dmin) ** (2.0 / 3.0) / (ph.g * rmax) ** (1.0 / 3.0)
@@ -857,10 +882,23 @@ class DenseOracle:
             coastal_force(self.q[I], c.sponge, c.clamp, c.ring, t0, c.drain, inf, ga)
         self.nsteps += 1
 
+    def rain_max_all(self) -> float:
+        """The largest rain rate of the whole table as the step reads it (state dtype, m/s)."""
+        r = self.case.rain
+        if r is None:
+            return 0.0
+        if np.isscalar(r):
+            return float(self.dtype.type(r))
+        rate = np.asarray(r[1])
+        return float((rate / 3.6e6).astype(self.dtype).max())
+
     def next_dt(self, t_end=None, dt_max=None) -> float:
-        """The step Solver2D.run takes next: CFL, rain cap, dt_max, clip to t_end."""
+        """The step a run takes next: CFL, rain-film bound, dt_max, clip to t_end. The dense
+        solver bounds each step by the current segment's largest rate (solver.py:3005-3027);
+        the flat one (``ph.flat_cfl``) by the largest rate of the whole table, rounded to the
+        state dtype (compressed_solver.py:3429-3436)."""
         dt = self.cfl_dt()
-        _, rmax = self.rain_now()
+        rmax = self.rain_max_all() if self.ph.flat_cfl else self.rain_now()[1]
         dt = min(dt, rain_step_cap(rmax, self.ph, self.dmin))
         if dt_max is not None and dt > dt_max:
             dt = dt_max

@@ -15,6 +15,14 @@ Imports GeoSWE from /root/reference/src with GEOSWE_BACKEND=numpy and records, a
 * flatmesh_small.npz        -- CompressedMesh2D flat order and E/W/N/S neighbour table on a
   small irregular mask after the ring-2 dilation of CompressedSWE.
 * forcing_samples.npz       -- RainfallForcing.rate_at_time / StageBoundary.stage_at_time.
+* step_vcap_f64.npz         -- one full Solver2D.step on a rough 48x40 state with a Manning
+  class table and a fifth of the cells above the 15 m/s velocity cap, quadratic and linear
+  friction roots, with the residual it starts from.
+* rain_dry_run_f64.npz      -- Solver2D.run on a dry sloping bed under a two-segment rain
+  series: the rain-film step bound sets the first steps (step times and final state).
+
+c1_oracle_2000.json (the oracle's own C1 state after 2000 steps) is written by
+make_c1_oracle_digest.py, which runs the oracle, not the reference.
 
 The reference is not available where the GPU tests run; only these data files are.
 Usage: python tests/golden/make_golden.py
@@ -143,7 +151,75 @@ def main():
         stages = np.array([sb.stage_at_time(t) for t in ts2])
     np.savez_compressed(OUT / "forcing_samples.npz", rain_t=rf.time_s, rain_mm_h=rf.rate_mm_h, t=ts, rates=rates,
                         stage_t=sb.time_s, stage_m=sb.stage_m, t2=ts2, stages=stages)
+
+    step_vcap_classes(Config, Mesh2D, Solver2D)
+    rain_dry_bed_run(Config, Mesh2D, Solver2D, RainfallForcing)
     print("golden vectors written to", OUT)
+
+
+def vcap_state():
+    """48 x 40 rough state: beds of a few decimetres, depths 2 cm - 82 cm with 8 % dry cells,
+    velocities up to 4 m/s and a fifth of the cells at 16-24 m/s (above the 15 m/s cap)."""
+    nx, ny = 48, 40
+    rng = np.random.default_rng(11)
+    i = np.arange(nx)[:, None]
+    j = np.arange(ny)[None, :]
+    bed = 0.2 * np.sin(i / 6.0) * np.cos(j / 5.0) + 0.05 * rng.random((nx, ny))
+    h = 0.02 + 0.8 * rng.random((nx, ny))
+    h[rng.random((nx, ny)) < 0.08] = 0.0
+    u = rng.uniform(-4.0, 4.0, (nx, ny))
+    v = rng.uniform(-4.0, 4.0, (nx, ny))
+    fast = rng.random((nx, ny)) < 0.2
+    u[fast] = rng.uniform(16.0, 24.0, fast.sum()) * np.sign(rng.standard_normal(fast.sum()))
+    cls = rng.integers(0, 5, (nx, ny)).astype(np.uint8)
+    tab = np.array([0.01, 0.025, 0.04, 0.07, 0.12])
+    return np.stack([h, h * u, h * v]), bed, cls, tab
+
+
+def step_vcap_classes(Config, Mesh2D, Solver2D):
+    """One full Solver2D.step (BC, residual, Euler, point-implicit Manning friction from a
+    class table, velocity cap, wet/dry; solver.py:2695-2973, friction 2917-2950) in float64,
+    with the quadratic and with the linearised friction root, and the residual it starts from."""
+    q0, bed, cls, tab = vcap_state()
+    ngh = 4
+    out = dict(q0=q0, bed=bed, cls=cls, tab=tab, ngh=ngh, dx=2.0, cfl=0.45, h_min=1e-6, vcap=15.0)
+    for quad in (True, False):
+        mesh = Mesh2D(nx=q0.shape[1], ny=q0.shape[2], dx=2.0, dy=2.0, ngh=ngh)
+        cfg = Config(bc_x="wall", bc_y="extrapolate", dtype="float64", h_min=1e-6, cfl=0.45,
+                     friction="manning_implicit", friction_quadratic_alpha=quad)
+        s = Solver2D(mesh, cfg, q0.copy(), bed)
+        s.set_manning_table(np.pad(cls, ngh, mode="edge"), tab)
+        dt = float(s.cfl_dt())
+        s._apply_bc()
+        rhs = s._rhs(s.q)[:, ngh:-ngh, ngh:-ngh].copy()
+        s.step(dt)
+        tag = "quad" if quad else "lin"
+        out[f"rhs_{tag}"] = rhs
+        out[f"dt_{tag}"] = dt
+        out[f"q1_{tag}"] = s.q_interior.copy()
+        out[f"tab32_{tag}"] = np.asarray(s._manning_tab)
+    np.savez_compressed(OUT / "step_vcap_f64.npz", **out)
+
+
+def rain_dry_bed_run(Config, Mesh2D, Solver2D, RainfallForcing):
+    """Solver2D.run (solver.py:2975-3027) on a dry sloping bed under a two-segment rain series,
+    float64: the rain-film step bound (cfl*dmin)^(2/3)/(g*R)^(1/3) of the current segment sets
+    the first steps, the CFL step the later ones; the time of every step and the final state."""
+    nx, ny = 32, 24
+    rng = np.random.default_rng(5)
+    i = np.arange(nx)[:, None]
+    bed = 0.02 * (nx - i) + 0.01 * rng.random((nx, ny))
+    q0 = np.zeros((3, nx, ny))
+    rf = RainfallForcing(time_s=np.array([0.0, 60.0]), rate_mm_h=np.array([50.0, 150.0]))
+    mesh = Mesh2D(nx=nx, ny=ny, dx=2.0, dy=2.0, ngh=4)
+    cfg = Config(bc_x="wall", bc_y="wall", dtype="float64", h_min=1e-6, cfl=0.5, friction="manning_implicit",
+                 manning_n=0.03, rainfall_forcing=rf)
+    s = Solver2D(mesh, cfg, q0, bed)
+    ts = []
+    n = s.run(t_end=150.0, callback=lambda sv, k: ts.append(float(sv.t)))
+    np.savez_compressed(OUT / "rain_dry_run_f64.npz", bed=bed, rain_t=rf.time_s, rain_mm_h=rf.rate_mm_h,
+                        t_end=150.0, steps=n, ts=np.array(ts), q=s.q_interior.copy(), dx=2.0, cfl=0.5,
+                        h_min=1e-6, manning=0.03)
 
 
 if __name__ == "__main__":

@@ -113,3 +113,47 @@ def test_gradient_independent_of_checkpoint_interval(gpu):
         other.simulate()
         gn, gb = other.gradient()
         assert np.array_equal(gn, ref_n) and np.array_equal(gb, ref_b), K
+
+
+def test_gradient_with_native_grid_rain(gpu):
+    """Rain read through a per-cell lookup into a native-grid table (the run driver's bundle)
+    reaches the adjoint's recomputation as it reaches the forward run: the misfit and its
+    gradient equal, bit for bit, those of the same rates given as full (nx, ny) frames."""
+    from geoswe_amd import CompressedSolver, Config, Mesh2D, NativeRain, RainfallForcing, Solver2D
+    from geoswe_amd.adjoint import GaugeMisfit
+    n = 36
+    i = np.arange(n)[:, None].astype(np.float64)
+    j = np.arange(n)[None, :].astype(np.float64)
+    bed = 0.3 * np.sin(i / 6.0) * np.cos(j / 5.0) + 0.004 * i
+    h = 1.2 - bed + 0.05 * np.sin(i / 4.0 + j / 7.0)
+    q0 = np.stack([h, h * 0.3, h * -0.2])
+    t_s = np.array([0.0, 0.4, 0.9])
+    lk = (np.arange(n)[:, None] // 6) * 6 + (np.arange(n)[None, :] // 6)
+    mmh = np.random.default_rng(2).random((3, 36)) * 400.0
+    frames = np.stack([mmh[k][lk] for k in range(3)])
+
+    def solver(native):
+        cfg = Config(bc_x="extrapolate", bc_y="extrapolate", dtype="float64", friction="manning", manning_n=0.03,
+                     h_min=1e-6)
+        s = Solver2D(Mesh2D(n, n, 2.0, 2.0, ngh=4), cfg, q0, bed)
+        cls = np.zeros((n + 8, n + 8), np.uint8)
+        cls[4:-4, 4:-4] = ((i + 2 * j) % 4).astype(np.uint8)
+        s.set_manning_table(cls, np.array([0.02, 0.035, 0.05, 0.07]))
+        c = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=False)
+        c.set_rain(NativeRain(time_s=t_s, table=mmh / 3.6e6, lookup=lk) if native
+                   else RainfallForcing(time_s=t_s, rate_mm_h=frames))
+        return c
+
+    ref = solver(False)
+    dts = np.array([ref.step() for _ in range(20)])
+    assert ref.t > t_s[-1]                                    # the run crosses both rain changes
+    out = []
+    for native in (True, False):
+        f = solver(native)
+        gauges = np.sort(np.random.default_rng(4).choice(np.flatnonzero(f.fm.active), 6, replace=False))
+        prob = GaugeMisfit(f, gauges, dts, h_star=np.full((20, 6), 1.0), checkpoint_every=7)
+        prob.simulate()
+        out.append((prob.loss(), *prob.gradient()))
+    (Ja, na, ba), (Jb, nb, bb) = out
+    assert Ja == Jb and np.array_equal(na, nb) and np.array_equal(ba, bb)
+    assert np.abs(nb).max() > 0

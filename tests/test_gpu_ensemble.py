@@ -4,6 +4,8 @@ hipGraph replays, a t_end run and after the launch-shape autotuner."""
 import numpy as np
 import pytest
 
+from oracle import swe_oracle as O
+
 pytestmark = pytest.mark.gpu
 
 SIDES = [64, 192, 96, 128, 64]
@@ -123,4 +125,33 @@ def test_ensemble_with_taller_tiles_than_members(cases, monkeypatch, target):
     assert all(np.array_equal(o, m.fm.tile_irc) for o, m in zip(own, mem))
     ens.advance(25)
     assert_same(plain, mem)
+    ens.close()
+
+
+def test_c4_generator_members_match_the_oracle(gpu):
+    """Members of the batched-ensemble workload's generator at the small end of its size range
+    (256^2 - 512^2 boxes, per-member rain scaling and Manning perturbation, gridded rain,
+    tidal stage), stepped together as one ensemble: each member bit for bit the dense oracle
+    of its own case on every active cell, with its own step count and time."""
+    from geoswe_amd.cases import ensemble_c4
+    from geoswe_amd.ensemble import Ensemble
+    members = ensemble_c4(sides=[256, 384, 512], rain_frames=2)
+    mem = [c.flat() for c in members]
+    ens = Ensemble(mem, graph_batch=8)
+    ens.advance(25)
+    assert len({m.t for m in mem}) == len(mem)
+    for case, m in zip(members, mem):
+        sb, rf = case.cfg.stage_boundary, case.cfg.rainfall_forcing
+        oc = O.DenseCase(h=case.q0[0], hu=case.q0[1], hv=case.q0[2], bed=case.bed, dx=3.0, dy=3.0, bc_x="fall",
+                         bc_y="fall", manning=case.ntab.astype(np.float32)[case.cls], rain=(rf.time_s, rf.rate_mm_h),
+                         active=case.active,
+                         stage={"cells": np.argwhere(case.sea_edge), "time_s": sb.time_s, "stage_m": sb.stage_m})
+        o = O.DenseOracle(oc, O.Physics(friction=True, flat_cfl=True), np.float32)
+        o.cfl_skip = ~case.active
+        o.run_steps(25)
+        assert m.nsteps == 25 and m.t == o.t
+        g = case.mesh.ngh
+        got = m.fm.unpack(m.q_flat())[:, g:-g, g:-g]
+        a = case.active
+        assert np.array_equal(got[:, a], o.interior[:, a])
     ens.close()

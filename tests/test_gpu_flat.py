@@ -280,13 +280,18 @@ def test_per_tile_launch_equals_chunk_traversal(gpu):
         for trav in ("tiles", "chunks"):
             s = dense(q0, bed, friction="manning", manning_n=0.03, rainfall=20.0 / 3.6e6)
             c = CompressedSolver.from_dense(s, cfl_linf=True, sweep_rows=1)
-            assert len(c.fm.tile_e0) > 4 * 7168
             if ext:
                 g = s.mesh.ngh
                 c.set_ga_drain({"cls_pad": np.pad(cls, g), "Ks_t": [0.0, 1e-5, 4e-5], "psi_t": [0.1, 0.1, 0.2],
                                 "dth_t": [0.3, 0.3, 0.3], "F_pad": None,
                                 "inv_tau_pad": np.full((nx + 2 * g, ny + 2 * g), 1.0 / 600.0), "mode": "fused"})
             c.traversal = trav
+            info = c.launch_info()
+            if trav == "tiles":
+                assert info["kernel"] == ("sweep_ext" if ext else "sweep")
+                assert info["blocks"] > info["resident"], info     # one wavefront per tile
+            else:
+                assert info["kernel"] == "chunks"
             c.advance(6)
             runs.append((c.q_flat(), c.nsteps, c.t))
         (qa, na, ta), (qb, nb, tb) = runs

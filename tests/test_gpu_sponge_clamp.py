@@ -478,6 +478,46 @@ def test_green_ampt_resume_and_ensemble(gpu, tmp_path):
     ens.close()
 
 
+def test_green_ampt_ensemble_after_autotune(gpu):
+    """The launch-shape autotuner restores everything its trial steps write, the Green-Ampt
+    cumulative infiltration F included: members with Green-Ampt stepped after an autotune
+    equal their plain runs bit for bit (state and F)."""
+    from geoswe_amd.ensemble import Ensemble
+    mk = lambda: _ga_setup("float32", "fused", seed=43)[0]
+    plain = [mk(), mk()]
+    for p in plain:
+        p.advance(9)
+    mem = [mk(), mk()]
+    F0 = [m.ga_F().copy() for m in mem]
+    ens = Ensemble(mem, graph_batch=4)
+    ens.autotune(trial_steps=3)
+    for m, f0 in zip(mem, F0):
+        assert m.nsteps == 0 and np.array_equal(m.ga_F(), f0)
+    ens.advance(9)
+    for p, e in zip(plain, mem):
+        assert p.nsteps == e.nsteps == 9
+        assert np.array_equal(p.q_flat(), e.q_flat()) and np.array_equal(p.ga_F(), e.ga_F())
+    ens.close()
+
+
+def test_checkpoint_without_F_is_refused_with_green_ampt(gpu, tmp_path):
+    """A checkpoint written without Green-Ampt (no F array) does not load into a solver with
+    Green-Ampt: resuming would reset the cumulative infiltration (the reference raises too)."""
+    c, g, m, case = _ga_setup("float32", "fused", seed=44)
+    from geoswe_amd import CompressedSolver
+    q0, bed = rough_state(NX, NY, seed=44, depth=0.7, rough=1.0, vel=0.3)
+    s = dense(q0 * m, bed, friction="manning", manning_n=0.03, rainfall=30.0 / 3.6e6)
+    s.set_inside_mask(m)
+    plain = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=True)
+    plain.advance(3)
+    d = str(tmp_path / "noF")
+    plain.save_checkpoint(d)
+    before = c.q_flat()
+    with pytest.raises(RuntimeError, match="cumulative infiltration"):
+        c.load_checkpoint(d)
+    assert c.nsteps == 0 and np.array_equal(c.q_flat(), before)
+
+
 def _native_rain(seed=61):
     rng = np.random.default_rng(seed)
     bi, bj = 7, 9

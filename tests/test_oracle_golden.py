@@ -135,3 +135,121 @@ def test_cache_neighbour_table_is_the_reference_one():
     want = np.where(nb >= 0, nb - k, -32768).astype(np.int16)
     got = _nbr_int16(fm)
     assert got.dtype == np.int16 and np.array_equal(got, want)
+
+
+def _vcap_oracle(z, quadratic, vcap=15.0, dtype=np.float64):
+    q0, bed, cls = z["q0"], z["bed"], z["cls"]
+    n = z["tab32_quad"][cls]                        # the reference casts the class table to float32
+    case = O.DenseCase(h=q0[0], hu=q0[1], hv=q0[2], bed=bed, dx=2.0, dy=2.0, bc_x="wall", bc_y="extrapolate",
+                       manning=n)
+    ph = O.Physics(h_min=1e-6, h_cfl=1e-6, cfl=0.45, friction=True, vcap=vcap, quadratic=quadratic)
+    return O.DenseOracle(case, ph, dtype)
+
+
+def _face_velocity_divergence(q0, bed):
+    """Cells touching a face where a side's reconstructed depth is <= h_min while its cell is
+    wet: there the reference's NumPy flux gates the velocity on the face depth and its kernel
+    (which the oracle follows) on the cell depth (flux.py:98-101 vs rhs_cuda.py:931-938)."""
+    g, hm = 4, 1e-6
+    qp = np.zeros((3, q0.shape[1] + 2 * g, q0.shape[2] + 2 * g))
+    qp[:, g:-g, g:-g] = q0
+    O.bc_fill(qp, g, "wall", "extrapolate")
+    b = O.pad_bed(bed, g, "wall", "extrapolate", np.float64)
+    h = qp[0]
+    gzx = np.zeros_like(b)
+    gzy = np.zeros_like(b)
+    gzx[1:-1] = (0.5 * (b[2:] - b[:-2])) * 0.5
+    gzy[:, 1:-1] = (0.5 * (b[:, 2:] - b[:, :-2])) * 0.5
+    bad = np.zeros(h.shape, bool)
+    xl, xr, *_ = O.srm_face(b[:-1], b[1:], gzx[:-1], gzx[1:], h[:-1], h[1:], 1.0, 9.81, hm)
+    fx = ((xl <= hm) & (h[:-1] > hm)) | ((xr <= hm) & (h[1:] > hm))
+    bad[:-1] |= fx
+    bad[1:] |= fx
+    yl, yr, *_ = O.srm_face(b[:, :-1], b[:, 1:], gzy[:, :-1], gzy[:, 1:], h[:, :-1], h[:, 1:], 1.0, 9.81, hm)
+    fy = ((yl <= hm) & (h[:, :-1] > hm)) | ((yr <= hm) & (h[:, 1:] > hm))
+    bad[:, :-1] |= fy
+    bad[:, 1:] |= fy
+    return bad[g:-g, g:-g]
+
+
+@pytest.mark.parametrize("quadratic", [True, False])
+def test_update_with_class_table_and_velocity_cap_f64(quadratic):
+    """The update tail of one reference Solver2D.step (solver.py:2917-2962: Euler, point-implicit
+    Manning friction from a class table, the 15 m/s velocity cap, wet/dry) applied to the
+    reference's own residual: the oracle's ``update`` equals it to float64 round-off, with the
+    quadratic and with the linear root. A fifth of the cells are above the cap. The fixture
+    discriminates: without the cap, or with the other root, the update misses it by orders of
+    magnitude more than the tolerance."""
+    z = np.load(G / "step_vcap_f64.npz")
+    tag = "quad" if quadratic else "lin"
+    n = z["tab32_quad"][z["cls"]]
+    dt = float(z[f"dt_{tag}"])
+    want = z[f"q1_{tag}"]
+    scale = np.abs(want).max(axis=(1, 2), keepdims=True)
+
+    def upd(vcap, quad):
+        ph = O.Physics(h_min=1e-6, h_cfl=1e-6, friction=True, vcap=vcap, quadratic=quad)
+        return O.update(z["q0"], z[f"rhs_{tag}"], dt, ph, n_cell=n)
+
+    err = np.abs(upd(15.0, quadratic) - want) / scale
+    assert err.max() < 1e-14, err.max()
+    h, hu, hv = z["q0"]
+    assert (np.hypot(hu, hv) > 15.0 * np.maximum(h, 1e-6)).sum() > 200
+    for other in (upd(np.inf, quadratic), upd(15.0, not quadratic)):
+        assert (np.abs(other - want) / scale).max() > 1e-6
+
+
+@pytest.mark.parametrize("quadratic", [True, False])
+def test_step_with_class_table_and_velocity_cap_f64(quadratic):
+    """The whole step (BC fill, residual, update) against the reference's: the dt exactly, the
+    state to float64 round-off on every cell except those next to a face where the NumPy flux
+    and the kernel gate the velocity differently (a documented divergence of the reference's
+    own two paths; 4 of 1920 cells here)."""
+    z = np.load(G / "step_vcap_f64.npz")
+    tag = "quad" if quadratic else "lin"
+    o = _vcap_oracle(z, quadratic)
+    dt = o.cfl_dt()
+    assert abs(dt - float(z[f"dt_{tag}"])) <= 1e-15 * dt
+    o.step(dt)
+    want = z[f"q1_{tag}"]
+    scale = np.abs(want).max(axis=(1, 2), keepdims=True)
+    err = (np.abs(o.interior - want) / scale).max(axis=0)
+    skip = _face_velocity_divergence(z["q0"], z["bed"])
+    assert skip.sum() <= 8
+    assert err[~skip].max() < 1e-13, err[~skip].max()
+
+
+def test_step_with_class_table_and_velocity_cap_f32_close():
+    """The float32 oracle step stays within float32 round-off of the reference's float64 step
+    (away from the face-velocity divergence)."""
+    z = np.load(G / "step_vcap_f64.npz")
+    o = _vcap_oracle(z, True, dtype=np.float32)
+    o.step(float(z["dt_quad"]))
+    want = z["q1_quad"]
+    scale = np.abs(want).max(axis=(1, 2), keepdims=True)
+    err = (np.abs(o.interior.astype(np.float64) - want) / scale).max(axis=0)
+    assert err[~_face_velocity_divergence(z["q0"], z["bed"])].max() < 2e-6
+
+
+def test_rain_on_dry_bed_run_f64():
+    """Solver2D.run on a dry bed under a two-segment rain (solver.py:2975-3027): the reference's
+    step times, step count and final state. The first two steps sit at the rain-film bound
+    (cfl*dmin)^(2/3)/(g*R)^(1/3) of the first segment, the later ones at the CFL step."""
+    z = np.load(G / "rain_dry_run_f64.npz")
+    bed = z["bed"]
+    zero = np.zeros_like(bed)
+    case = O.DenseCase(h=zero, hu=zero, hv=zero, bed=bed, dx=2.0, dy=2.0, bc_x="wall", bc_y="wall",
+                       manning=0.03, rain=(z["rain_t"], z["rain_mm_h"]))
+    ph = O.Physics(h_min=1e-6, h_cfl=1e-6, cfl=0.5, friction=True)
+    o = O.DenseOracle(case, ph, np.float64)
+    n = o.run(float(z["t_end"]))
+    assert n == int(z["steps"])
+    ts = np.cumsum(o.dts)
+    assert np.abs(ts - z["ts"]).max() < 1e-11
+    cap = O.rain_step_cap(50.0 / 3.6e6, ph, 2.0)
+    assert o.dts[0] == cap and o.dts[1] <= cap and o.dts[2] < 0.6 * cap
+    # thin films (1e-4 - 2e-2 m) crossing the wet/dry threshold: the reference's NumPy flux and its
+    # kernel (which the oracle follows) differ below round-off at such faces (velocity gate,
+    # both-dry faces), which the run amplifies to ~2e-8 of each field's scale in 36 steps
+    scale = np.abs(z["q"]).max(axis=(1, 2), keepdims=True)
+    assert (np.abs(o.interior - z["q"]) / scale).max() < 1e-7

@@ -165,6 +165,44 @@ def test_cfl_closed_form():
     assert math.isnan(O.lam_max(qn, ph))
 
 
+def test_cfl_l2_and_flat_rules_closed_form():
+    """The L2 velocity norm and the flat layout's skip/divisor rule on hand-built cells
+    (3-4-5 velocities, a thin film between h_min and h_min_cfl, a sub-h_min film)."""
+    c = math.sqrt(G * 4.0)
+    #            3-4-5 cell      thin film          below h_min
+    h = np.array([[4.0,          2.0e-6,            5.0e-7]])
+    hu = np.array([[12.0,        2.0e-6,            1.0]])
+    hv = np.array([[-16.0,       0.0,               1.0]])
+    q = np.stack([h, hu, hv])
+    dense = O.Physics(h_min=1e-6, h_cfl=1e-3)
+    assert O.lam_max(q, dense) == 4.0 + c                                   # L-inf; films skipped
+    assert O.lam_max(q, O.Physics(h_min=1e-6, h_cfl=1e-3, cfl_l2=True)) == 5.0 + c
+    # flat: only h < h_min is skipped; the film's velocity divides by h_min_cfl = 1e-3, not by h
+    flat = O.Physics(h_min=1e-6, h_cfl=1e-3, flat_cfl=True)
+    film = 2.0e-6 / 1.0e-3 + math.sqrt(G * 2.0e-6)
+    assert O.lam_max(q[:, :, 1:], flat) == np.float64(film)
+    assert O.lam_max(q, flat) == 4.0 + c
+    assert O.lam_max(q, O.Physics(h_min=1e-6, h_cfl=1e-3, flat_cfl=True, cfl_l2=True)) == 5.0 + c
+    # with h_min_cfl = h_min the flat rule is the dense one
+    same = q[:, :, 1:].copy()
+    assert O.lam_max(same, O.Physics(h_min=1e-6, h_cfl=1e-6, flat_cfl=True)) == \
+        O.lam_max(same, O.Physics(h_min=1e-6, h_cfl=1e-6))
+    # floors: sqrt(g*h_cfl) dense, sqrt(g*h_min) flat
+    assert O.dt_from_lam(0.0, dense, 3.0) == 0.5 * 3.0 / (math.sqrt(G * 1e-3) + 1e-12)
+    assert O.dt_from_lam(0.0, flat, 3.0) == 0.5 * 3.0 / (math.sqrt(G * 1e-6) + 1e-12)
+
+
+@pytest.mark.parametrize("R,dmin,cfl", [(50 / 3.6e6, 3.0, 0.5), (1e-3, 1.0, 0.9), (2e-7, 30.0, 0.25)])
+def test_rain_step_cap_is_the_film_cfl_step(R, dmin, cfl):
+    """The rain-film bound (solver.py:3005-3027) is the fixed point dt = cfl*dmin/sqrt(g*R*dt):
+    the CFL step of the film R*dt that the step itself lays down."""
+    ph = O.Physics(cfl=cfl)
+    dt = O.rain_step_cap(R, ph, dmin)
+    assert dt == pytest.approx(cfl * dmin / math.sqrt(G * R * dt), rel=1e-14)
+    assert O.rain_step_cap(0.0, ph, dmin) == math.inf
+    assert O.rain_step_cap(-1.0, ph, dmin) == math.inf
+
+
 def test_ghost_fill_by_hand():
     q = np.arange(3 * 6 * 5, dtype=float).reshape(3, 6, 5) + 1.0
     for kind in ("wall", "fall", "extrapolate", "periodic"):
