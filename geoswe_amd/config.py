@@ -27,41 +27,61 @@ _ENUM_OK = {"time": {"euler", "ssprk3"}, "flux": {"lf", "hllc"},
             "bc_x": _BC_OK, "bc_y": _BC_OK, "friction": {"manning_implicit"}}
 
 
+# Numeric bounds of the configuration (the accept/reject rules of solver.py:564-626): each
+# row is a field, the interval its value must lie in, and what breaks outside it. A value of
+# None is not checked; every value given must be finite except the velocity cap (inf, or a
+# NaN that no speed exceeds, switches the cap off); a value is refused when it is known to lie
+# outside its interval.
+_BOUNDS = (
+    ("cfl", (0.0, False, 1.0, True), "the explicit step is unstable above the Courant limit"),
+    ("g", (0.0, False, math.inf, True), "gravity must be positive"),
+    ("h_min", (0.0, False, math.inf, True), "it is the wet/dry threshold and a divisor"),
+    ("h_min_cfl", (0.0, True, math.inf, True), "0 means: use h_min"),
+    ("manning_n", (0.0, True, math.inf, True), "a roughness cannot be negative"),
+    ("friction_velocity_cap_ms", (0.0, False, math.inf, True), "inf switches the cap off"),
+    ("storage_courant", (0.0, True, math.inf, True), "a Courant number cannot be negative"),
+)
+# Values that are legal but usually a mistake: (field, threshold above which to warn, why).
+_ODD = (
+    ("cfl", 0.5, lambda v: "the 2D limit of the max(|u|,|v|) norm is about 0.5"),
+    ("rainfall", 1.0e-3, lambda v: f"rainfall is in m/s; this is {v * 3.6e6:,.0f} mm/h"),
+)
+
+
+# spellings of the friction switch: "none" in any case means off, "manning" the point-implicit
+# Manning law
+_FRICTION_NAMES = {"manning": "manning_implicit"}
+# Settings that a configuration accepts but that have no effect in it (warnings, as the
+# reference gives them): (condition, message).
+_IGNORED = (
+    (lambda c: c.friction is None and (c.manning_n > 0.0 or c.manning_field is not None),
+     lambda c: "Manning roughness given with friction off: the run is frictionless (friction='manning' "
+               "applies it)"),
+    (lambda c: c.well_balanced and c.recon != "first",
+     lambda c: f"recon={c.recon!r} is ignored: the well-balanced face states are first order"),
+)
+
+
+def _interval(lo, lo_closed, hi, hi_closed) -> str:
+    return f"{'[' if lo_closed else '('}{lo:g}, {hi:g}{']' if hi_closed else ')'}"
+
+
 def check_numeric_inputs(where="Config", **vals):
-    """Bound the numeric inputs (solver.py:564-626): raises ValueError, warns on odd values."""
-    for name, v in vals.items():
-        if v is not None and not math.isfinite(float(v)) and name != "friction_velocity_cap_ms":
-            raise ValueError(f"{where}.{name}={v!r} is not finite")
-    f = lambda n: None if vals.get(n) is None else float(vals[n])
-    cfl = f("cfl")
-    if cfl is not None and not (0.0 < cfl <= 1.0):
-        raise ValueError(f"{where}.cfl={cfl} is outside (0, 1]: the explicit step is unstable above "
-                         f"the Courant limit")
-    if cfl is not None and cfl > 0.5:
-        warnings.warn(f"{where}.cfl={cfl} is above 0.5; with the max(|u|,|v|) norm the 2D limit is "
-                      f"near 0.5", stacklevel=3)
-    g = f("g")
-    if g is not None and g <= 0.0:
-        raise ValueError(f"{where}.g={g} must be positive")
-    h_min = f("h_min")
-    if h_min is not None and h_min <= 0.0:
-        raise ValueError(f"{where}.h_min={h_min} must be positive (wet/dry threshold and divisor)")
-    hmc = f("h_min_cfl")
-    if hmc is not None and hmc < 0.0:
-        raise ValueError(f"{where}.h_min_cfl={hmc} must be >= 0 (0 couples it to h_min)")
-    mn = f("manning_n")
-    if mn is not None and mn < 0.0:
-        raise ValueError(f"{where}.manning_n={mn} must be >= 0")
-    vcap = f("friction_velocity_cap_ms")
-    if vcap is not None and vcap <= 0.0:
-        raise ValueError(f"{where}.friction_velocity_cap_ms={vcap} must be positive (inf disables the cap)")
-    sc = f("storage_courant")
-    if sc is not None and sc < 0.0:
-        raise ValueError(f"{where}.storage_courant={sc} must be >= 0")
-    rain = f("rainfall")
-    if rain is not None and rain > 1.0e-3:
-        warnings.warn(f"{where}.rainfall={rain} is in m/s, not mm/h ({rain * 3.6e6:,.0f} mm/h)",
-                      stacklevel=3)
+    """Check numeric configuration values against ``_BOUNDS`` (ValueError) and ``_ODD``
+    (a warning), after solver.py:564-626."""
+    given = {k: float(v) for k, v in vals.items() if v is not None}
+    for name, v in given.items():
+        if not math.isfinite(v) and name != "friction_velocity_cap_ms":
+            raise ValueError(f"{where}.{name}={vals[name]!r}: a finite value is required")
+    for name, (lo, lo_closed, hi, hi_closed), why in _BOUNDS:
+        if name not in given:
+            continue
+        v = given[name]
+        if v < lo or (v == lo and not lo_closed) or v > hi or (v == hi and not hi_closed):
+            raise ValueError(f"{where}.{name}={v!r} lies outside {_interval(lo, lo_closed, hi, hi_closed)}: {why}")
+    for name, limit, why in _ODD:
+        if name in given and given[name] > limit:
+            warnings.warn(f"{where}.{name}={given[name]!r}: {why(given[name])}", stacklevel=3)
 
 
 @dataclass
@@ -102,38 +122,36 @@ class Config:
     sigma_halo_every: int = 1
 
     def __post_init__(self):
+        # the float32 build is the default here (the package computes on the GPU); other
+        # spellings of a dtype are normalised to its name
         if self.dtype is None:
-            self.dtype = "float32"          # this package computes on the GPU
-        if not isinstance(self.dtype, str):
+            self.dtype = "float32"
+        elif not isinstance(self.dtype, str):
             try:
                 self.dtype = np.dtype(self.dtype).name
             except TypeError:
-                pass
-        # float32 wet/dry floor defaults (solver.py:799-806)
+                pass                         # refused by the choice check below
         if self.dtype == "float32" and self.h_min == H_MIN:
+            # float32 wet/dry floor, with the CFL floor coupled to it (solver.py:799-806)
             self.h_min = 1.0e-6
             if self.h_min_cfl == 0.0:
                 self.h_min_cfl = self.h_min
-        if isinstance(self.friction, str) and self.friction.lower() == "none":
-            self.friction = None
-        elif self.friction == "manning":
-            self.friction = "manning_implicit"
-        for k, ok in _ENUM_OK.items():
-            v = getattr(self, k, None)
-            if v is None and k == "friction":
-                continue
-            if not isinstance(v, str) or v not in ok:
-                raise ValueError(f"Config.{k}={v!r} invalid; expected one of {sorted(ok)}")
-        if self.friction is None and (self.manning_n > 0.0 or self.manning_field is not None):
-            warnings.warn("Config: a Manning roughness is set but friction is off, so the run is "
-                          "frictionless; pass friction='manning' to apply it", stacklevel=3)
+        if isinstance(self.friction, str):
+            self.friction = None if self.friction.lower() == "none" else _FRICTION_NAMES.get(self.friction,
+                                                                                           self.friction)
+        for name, allowed in _ENUM_OK.items():
+            v = getattr(self, name, None)
+            if name == "friction" and v is None:
+                continue                     # no friction
+            if not (isinstance(v, str) and v in allowed):
+                raise ValueError(f"Config.{name}={v!r} is not one of {sorted(allowed)}")
         check_numeric_inputs(where="Config", cfl=self.cfl, g=self.g, h_min=self.h_min,
                              h_min_cfl=self.h_min_cfl, manning_n=self.manning_n,
                              friction_velocity_cap_ms=self.friction_velocity_cap_ms,
                              storage_courant=self.storage_courant, rainfall=self.rainfall)
-        if self.well_balanced and self.recon != "first":
-            warnings.warn(f"Config.recon={self.recon!r} has no effect with well_balanced=True",
-                          stacklevel=3)
+        for ignored, why in _IGNORED:
+            if ignored(self):
+                warnings.warn(f"Config: {why(self)}", stacklevel=3)
 
     def h_cfl(self) -> float:
         """The CFL depth floor cfl_dt uses (solver.py:2578-2585)."""

@@ -24,6 +24,23 @@ def _require_time_order(who, t):
         raise ValueError(f"{who}: time_s must be non-decreasing; time_s[{k}]={t[k]} < time_s[{k-1}]={t[k-1]}")
 
 
+def _series(who, time_s, values, ranks, what):
+    """A forcing's (times, values) pair: lists become float64 arrays; the values must have one
+    of the ranks in ``ranks`` (any rank for None) and one leading row per time; the times must
+    be a non-empty, non-decreasing 1-D array (forcing.py:113-137, 247-254)."""
+    t = np.asarray(time_s, dtype=np.float64) if isinstance(time_s, (list, tuple)) else time_s
+    v = np.asarray(values, dtype=np.float64) if isinstance(values, (list, tuple)) else values
+    rank = int(getattr(v, "ndim", 1))
+    if ranks is not None and rank not in ranks:
+        want = " or ".join({1: "(nt,)", 3: "(nt, nx, ny)"}[r] for r in ranks)
+        raise ValueError(f"{who}: {what} of shape {tuple(np.shape(v))} is not {want}")
+    rows = int(np.shape(v)[0]) if rank else 1
+    if rows != len(t):
+        raise ValueError(f"{who}: {what} has {rows} rows for {len(t)} times")
+    _require_time_order(who, t)
+    return t, v
+
+
 @dataclass
 class RainfallForcing:
     """Time-varying rainfall: uniform ``rate_mm_h`` (nt,) or gridded (nt, nx, ny)."""
@@ -31,17 +48,7 @@ class RainfallForcing:
     rate_mm_h: np.ndarray
 
     def __post_init__(self):
-        if isinstance(self.time_s, (list, tuple)):
-            self.time_s = np.asarray(self.time_s, dtype=np.float64)
-        if isinstance(self.rate_mm_h, (list, tuple)):
-            self.rate_mm_h = np.asarray(self.rate_mm_h, dtype=np.float64)
-        nd = int(getattr(self.rate_mm_h, "ndim", 1))
-        if nd not in (1, 3):
-            raise ValueError(f"RainfallForcing: rate_mm_h has shape {tuple(self.rate_mm_h.shape)}; it must be "
-                             f"(nt,) or (nt, nx, ny)")
-        if int(self.rate_mm_h.shape[0]) != len(self.time_s):
-            raise ValueError(f"RainfallForcing: {len(self.time_s)} times but {int(self.rate_mm_h.shape[0])} rates")
-        _require_time_order("RainfallForcing", self.time_s)
+        self.time_s, self.rate_mm_h = _series("RainfallForcing", self.time_s, self.rate_mm_h, (1, 3), "rate_mm_h")
 
     @classmethod
     def from_uniform_constant(cls, rate_mm_h: float, t_end: float = 1e9):
@@ -96,13 +103,7 @@ class StageBoundary:
     bed_b: np.ndarray
 
     def __post_init__(self):
-        if isinstance(self.time_s, (list, tuple)):
-            self.time_s = np.asarray(self.time_s, dtype=np.float64)
-        if isinstance(self.stage_m, (list, tuple)):
-            self.stage_m = np.asarray(self.stage_m, dtype=np.float64)
-        if len(self.time_s) != len(self.stage_m):
-            raise ValueError(f"StageBoundary: {len(self.time_s)} times but {len(self.stage_m)} stage values")
-        _require_time_order("StageBoundary", self.time_s)
+        self.time_s, self.stage_m = _series("StageBoundary", self.time_s, self.stage_m, None, "stage_m")
         self.cells = np.asarray(self.cells, dtype=np.int64).reshape(-1, 2)
 
     @classmethod
