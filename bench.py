@@ -59,6 +59,39 @@ def timed_steps(s, steps):
     return wall, float(ms.value)
 
 
+def fp64_error(nx, ny, bed, steps):
+    """Max |h32 - h64| after `steps` steps of the rain-on-slope case: the float32 run against
+    the float64 build of the kernel on the float32 run's inputs and wet/dry thresholds (its
+    own CFL steps), and against the float64 build replaying the float32 run's dt sequence
+    (which takes the dt schedule out of the comparison)."""
+    from geoswe_amd.cases import rain_on_slope_c2
+    log(f"float64 comparison: float32 and float64 runs of {steps} steps")
+    s = rain_on_slope_c2(nx, ny, "float32", bed=bed)
+    dts = [s.step() for _ in range(steps)]
+    h32 = s.depth().astype(np.float64)
+    t32 = s.t
+    del s
+    out = {"steps": steps}
+    for name, dt_seq in (("own_dt", None), ("frozen_dt", dts)):
+        s64 = rain_on_slope_c2(nx, ny, "float64", bed=bed, f32_inputs=True)
+        if dt_seq is None:
+            s64.advance(steps)
+        else:
+            for dt in dt_seq:
+                s64.step(dt)
+        e = np.abs(h32 - s64.depth())
+        k = int(np.argmax(e))
+        i, j = np.unravel_index(k, e.shape)
+        out[name] = {"max": float(e.max()), "at": [int(i), int(j)], "p99999": float(np.percentile(e, 99.999)),
+                     "median": float(np.median(e)), "cells_above_1e-5": int((e > 1e-5).sum()), "t64": s64.t}
+        log(f"float64 ({name}): max |h32 - h64| {e.max():.3e} m at ({i}, {j}); 99.999th percentile "
+            f"{out[name]['p99999']:.3e} m, median {out[name]['median']:.3e} m")
+        del s64, e
+    out["t32"] = t32
+    out["max_depth_err_vs_fp64_m"] = out["own_dt"]["max"]
+    return out
+
+
 def cpu_baseline(bed_spec, crop=(1024, 1000), steps=80):
     """The NumPy oracle (float32, one thread) on a crop of the same workload."""
     from oracle import generators as GN
@@ -143,17 +176,24 @@ def copy_calibration(nbytes: int, reps: int, quiet: bool = False) -> dict:
 
 
 def coastal(args):
+    """--workload coastal: the coastal leg as its own JSON line."""
+    out = coastal_leg(args.n, args.steps, args.sweep_rows, args.forcings)
+    print(json.dumps(out), flush=True)
+    if not out["dense_equals_flat"]:
+        sys.exit(1)
+
+
+def coastal_leg(n: int, steps: int, sweep_rows=None, forcings: bool = False) -> dict:
     """The compressed coastal case: flat and dense layouts stepped side by side, depth and
     residual compared bit for bit at steps 1, 500 and the last; both timed with HIP events.
-    Prints one JSON line for the flat layout (the production layout of this case)."""
+    The line is the flat layout's (the production layout of this case)."""
     from geoswe_amd import _lib as L
     from geoswe_amd.cases import coastal_c3
-    n = args.n
     t0 = time.perf_counter()
     log(f"building the coastal case {n}x{n}")
     case = coastal_c3(n=n)
     d = case.dense(track_max_depth=False)
-    f = case.flat(track_max_depth=False, sweep_rows=args.sweep_rows)
+    f = case.flat(track_max_depth=False, sweep_rows=sweep_rows)
     fm = f.fm
     act = fm.active
     ij = fm.ij[act] - case.mesh.ngh
@@ -174,7 +214,7 @@ def coastal(args):
         return same_h and same_r and same_clock
 
     ok = compare("t=0")
-    marks = [1, 500, args.steps] if args.steps > 500 else [1, args.steps]
+    marks = [1, 500, steps] if steps > 500 else [1, steps]
     tf = td = 0.0
     done = 0
     for m in marks:
@@ -222,25 +262,23 @@ def coastal(args):
         "dense_ms_per_step": ms_d,
         "dense_equals_flat": ok,
     }
-    if args.forcings:
-        out["forcing_tail"] = coastal_forcing_tail(case, args, ms_f)
-    print(json.dumps(out), flush=True)
-    if not ok:
-        sys.exit(1)
+    if forcings:
+        out["forcing_tail"] = coastal_forcing_tail(case, steps, sweep_rows, ms_f)
+    return out
 
 
-def coastal_forcing_tail(case, args, plain_ms):
+def coastal_forcing_tail(case, steps, sweep_rows, plain_ms):
     """The coastal step with a driver's whole forcing tail attached (infiltration, sponge,
     IDW gauge ring, Green-Ampt with drain-tau, clamp, karst drain): a fresh flat solver, one
     warm-up step, then the same number of timed steps as the plain run."""
     from geoswe_amd import _lib as L
     from geoswe_amd.cases import attach_coastal_forcings
     t0 = time.perf_counter()
-    f = case.flat(track_max_depth=False, sweep_rows=args.sweep_rows)
+    f = case.flat(track_max_depth=False, sweep_rows=sweep_rows)
     cells = attach_coastal_forcings(case, f)["cells"]
     log(f"forcing tail attached in {time.perf_counter() - t0:.0f} s: {cells}")
     timed_steps(f, 1)
-    k = max(args.steps - 1, 1)
+    k = max(steps - 1, 1)
     _, ms = timed_steps(f, k)
     if f._clockv().status & L.ST_NONFINITE:
         raise FloatingPointError("non-finite state in the forced coastal run")
@@ -252,13 +290,21 @@ def coastal_forcing_tail(case, args, plain_ms):
 
 
 def ensemble(args):
+    """--workload ensemble: the ensemble leg as its own JSON line."""
+    out = ensemble_leg(args.members, args.steps, args.graph_batch)
+    print(json.dumps(out), flush=True)
+    if not out["members_bitwise_equal_plain"]:
+        sys.exit(1)
+
+
+def ensemble_leg(members: int, steps: int, graph_batch: int) -> dict:
     """The batched ensemble: seeded uneven coastal members stepped one member after another
     (plain CompressedSolver runs) and together (one launch per step, hipGraph replay, the
-    autotuned launch shape); every member compared bit for bit. One JSON line."""
+    autotuned launch shape); every member compared bit for bit."""
     from geoswe_amd import _lib as L
     from geoswe_amd.cases import ensemble_c4, ensemble_sides
     from geoswe_amd.ensemble import Ensemble
-    sides = ensemble_sides(args.members)
+    sides = ensemble_sides(members)
     t0 = time.perf_counter()
     log(f"building {len(sides)} members, sides {min(sides)}..{max(sides)}, {sum(x * x for x in sides) / 1e6:.1f} M "
         f"box cells")
@@ -268,7 +314,6 @@ def ensemble(args):
     del cases
     act = sum(m.n_active for m in mem)
     log(f"built in {time.perf_counter() - t0:.0f} s; {act / 1e6:.1f} M active cells")
-    steps = args.steps
     for m in plain:                    # first dt and set-up outside the timed region
         m._ensure_lam()
         m._open_mode()
@@ -287,7 +332,7 @@ def ensemble(args):
     L.call("gs_event_elapsed_ms", C.byref(ms), e0, e1)
     ev_plain = ms.value
     log(f"sequential plain runs: {w_plain:.2f} s wall for {steps} steps x {len(plain)} members")
-    ens = Ensemble(mem, graph_batch=args.graph_batch)
+    ens = Ensemble(mem, graph_batch=graph_batch)
     tune = ens.autotune(trial_steps=3)
     log(f"autotuned launch: {tune}")
     ens.advance(1, graph_batch=0)       # first launch outside the timed region
@@ -321,7 +366,7 @@ def ensemble(args):
     achieved = 33.0 * act * steps / (ev_ens * 1e-3) / 1e9
     traffic = None
     tp = ROOT / "profiles" / "ens_step_64_members_traffic.json"
-    if args.members == 64 and tp.exists():
+    if members == 64 and tp.exists():
         traffic = json.loads(tp.read_text()).get("bytes_per_launch")
     out = {
         "metric": "cell-updates/s (batched ensemble of flat domains, 1 MI355X)",
@@ -339,19 +384,25 @@ def ensemble(args):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_ens_step<float>",
                      "bytes_per_cell": 33.0},
     }
-    print(json.dumps(out), flush=True)
-    if not same:
-        sys.exit(1)
+    L.call("gs_event_destroy", e0)
+    L.call("gs_event_destroy", e1)
+    return out
 
 
 def adjoint(args):
+    """--workload adjoint: the adjoint leg as its own JSON line."""
+    out = adjoint_leg(args.n_adjoint, args.steps, args.checkpoint, args.sweep_rows, fd=not args.no_fd,
+                      fd_bed_cells=args.fd_bed_cells)
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_leg(n: int, S: int, K: int, sweep_rows=None, fd: bool = True, fd_bed_cells: int = 64) -> dict:
     """Gauge-misfit calibration gradients on the coastal case: forward and adjoint cost,
-    central finite differences (eps 1e-3) on every class n and on seeded bed cells near the
-    gauges, and float64 torch autograd on a small coastal case. One JSON line."""
+    central finite differences (eps 1e-3 and 1e-4) on every class n and on seeded bed cells
+    near the gauges, and float64 torch autograd on a small coastal case."""
     from geoswe_amd import _lib as L
     from geoswe_amd.adjoint import GaugeMisfit
     from geoswe_amd.cases import coastal_c3
-    n, S, K = args.n_adjoint, args.steps, args.checkpoint
     t0 = time.perf_counter()
     case = coastal_c3(n=n)
     ref = case.flat(track_max_depth=False)
@@ -361,7 +412,7 @@ def adjoint(args):
     for _ in range(S):
         dts.append(ref.step())
     del ref
-    f = case.flat(track_max_depth=False, sweep_rows=args.sweep_rows)
+    f = case.flat(track_max_depth=False, sweep_rows=sweep_rows)
     fm = f.fm
     h0 = f.q_flat()[0]
     rng = np.random.default_rng(20261205)
@@ -409,14 +460,13 @@ def adjoint(args):
         e[cell] = eps
         return (J(bb=b0 + e) - J(bb=b0 - e)) / (2 * eps)
 
-    if args.no_fd:
-        print(json.dumps({"forward_ms_per_step": 1e3 * t_fwd / S, "adjoint_ms_per_step": 1e3 * t_adj / S,
-                          "plain_forward_ms_per_step": 1e3 * t_plain / S}), flush=True)
-        return
+    if not fd:
+        return {"forward_ms_per_step": 1e3 * t_fwd / S, "adjoint_ms_per_step": 1e3 * t_adj / S,
+                "plain_forward_ms_per_step": 1e3 * t_plain / S, "value": t_adj / t_fwd}
     fd_n = np.array([[fd_at(eps, nn_dir=k) for k in range(len(n0))] for eps in (1e-3, 1e-4)])
     gij = fm.ij[gauges]
     near = np.flatnonzero(fm.active & (np.abs(fm.ij[:, None, :] - gij[None]).max(axis=2).min(axis=1) <= 4))
-    cells = np.sort(rng.choice(near, size=min(64, len(near)), replace=False))
+    cells = np.sort(rng.choice(near, size=min(fd_bed_cells, len(near)), replace=False))
     fd_b = np.array([[fd_at(eps, cell=c) for c in cells] for eps in (1e-3, 1e-4)])
     prob.set_params(ntab=n0, bed=b0)
 
@@ -455,7 +505,7 @@ def adjoint(args):
         "fd_vs_adjoint_n": st_n, "fd_vs_adjoint_bed": st_b,
         "torch_f64_autograd": torch_err,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def torch_check():
@@ -479,15 +529,19 @@ def torch_check():
 
 
 def dambreak(args):
+    """--workload dambreak: the dam-break leg as its own JSON line."""
+    print(json.dumps(dambreak_leg(args.steps)), flush=True)
+
+
+def dambreak_leg(steps: int) -> dict:
     """The 256x256 circular dam break (walls, no friction, no rain), 2000 steps of
     step(cfl_dt()): the float32 GPU run against the float64 GPU run and against the float64
     NumPy oracle run; max |dh|, relative L2 of h, hu, hv at steps 1, 100 and the last, the
     final time and the dt sequences. The float32 oracle is timed on the same run (one
-    thread) as the CPU baseline. One JSON line."""
+    thread) as the CPU baseline."""
     from geoswe_amd import Solver2D
     from geoswe_amd.cases import dam_break_c1
     from oracle import swe_oracle as O
-    steps = args.steps
     marks = sorted({1, min(100, steps), steps})
     runs = {}
     for dt_name in ("float32", "float64"):
@@ -548,7 +602,7 @@ def dambreak(args):
         "cpu_baseline": {"value": cells * steps / cpu, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                          "sample": f"NumPy oracle float32, the full run ({steps} steps) in {cpu:.1f} s"},
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -579,6 +633,13 @@ def main():
     ap.add_argument("--n-adjoint", type=int, default=2048)
     ap.add_argument("--checkpoint", type=int, default=100)
     ap.add_argument("--no-fd", action="store_true", help="adjoint: timings only (no finite differences)")
+    ap.add_argument("--fd-bed-cells", type=int, default=64, help="adjoint: bed cells checked by finite differences")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="default workload: leave out the ensemble and adjoint legs of the line")
+    ap.add_argument("--ens-steps", type=int, default=1000, help="ensemble leg: steps (the configuration's 1000)")
+    ap.add_argument("--adj-steps", type=int, default=4000, help="adjoint leg: steps (the configuration's 4000)")
+    ap.add_argument("--err-steps", type=int, default=220,
+                    help="float64 comparison: steps of both runs (the configuration's 20 warm-up + 200 timed)")
     ap.add_argument("--n", type=int, default=16384, help="coastal box side")
     ap.add_argument("--forcings", action="store_true",
                     help="coastal: also time the step with a driver's whole forcing tail attached")
@@ -654,30 +715,17 @@ def main():
     copy = copy_calibration(1 << 31, 10, quiet=True)
     log(f"{ms_step:.3f} ms/step wall, {kern_ms:.3f} ms/launch (events), {value / 1e9:.2f} G cell-updates/s, "
         f"{achieved:.0f} GB/s algorithmic; copy kernel {copy['GBps_read_plus_write']:.0f} GB/s")
-    h32 = None
     err64 = None
-    if not args.no_fp64:
-        log("float64 run of the same workload (FP64 oracle build of the kernel)")
-        h32 = s.depth() if args.layout == "dense" else None
-        if h32 is not None:
-            del s
-            s64 = rain_on_slope_c2(nx, ny, "float64", bed=bed, f32_inputs=True)
-            s64.advance(args.warmup + args.steps)
-            h64 = s64.depth()
-            e = np.abs(h32.astype(np.float64) - h64)
-            err64 = float(e.max())
-            k = int(np.argmax(e))
-            log(f"max error at cell {np.unravel_index(k, e.shape)}, h64 there {h64.flat[k]:.4g} m; "
-                f"99.999th percentile {np.percentile(e, 99.999):.3e} m, median {np.median(e):.3e} m")
-            log(f"max |h32 - h64| after {args.warmup + args.steps} steps: {err64:.3e} m")
-            del s64, h64
+    if not args.no_fp64 and args.layout == "dense":
+        del s
+        err64 = fp64_error(nx, ny, bed, args.err_steps)
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (NumPy oracle on a crop)")
         cpu = cpu_baseline(bed)
         log(f"CPU {cpu['value'] / 1e6:.2f} M cell-updates/s")
     out = {
-        "metric": "cell-updates/s (fused FV step, 1 MI355X)",
+        "metric": "cell-updates/s & %HBM roofline, 1 MI355X: plain vs batched ensemble vs adjoint",
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -692,15 +740,41 @@ def main():
         "config": {"workload": f"rain-on-slope {args.layout} {nx}x{ny} ({cells / 1e6:.0f} M cells), dx=3 m, "
                                f"fBm H=0.8 0.5 m + slope 1e-3, h0=0.1 m, 50 mm/h, n=0.03, fall edges, CFL 0.5",
                    "layout": args.layout, "cells": cells},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        # the step is bound by instruction issue (VALU), not by HBM: "valu" carries its issue rate
+        # against the device's measured float32 issue rate; achieved/peak/frac are the HBM figures
+        "roofline": {"bound": "valu" if valu else "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "gs::k_dense_sweep<float, 1>" if args.layout == "dense" else "gs::k_flat_sweep<float>",
                      "kernel_ms": kern_ms, "bytes_per_cell": BYTES_PER_CELL[args.layout], "valu": valu,
                      "copy_GBps": copy["GBps_read_plus_write"],
                      "frac_of_copy": achieved / copy["GBps_read_plus_write"]},
-        "max_depth_err_vs_fp64_m": err64,
+        "max_depth_err_vs_fp64_m": err64["max_depth_err_vs_fp64_m"] if err64 else None,
+        "fp64_comparison": err64,
         "cpu_baseline": cpu,
     }
+    if not args.no_legs and (nx, ny) == (25600, 25000) and args.layout == "dense":
+        log(f"ensemble leg: 64 coastal members, {args.ens_steps} steps")
+        e = ensemble_leg(64, args.ens_steps, args.graph_batch)
+        out["ensemble"] = {k: e[k] for k in ("value", "unit", "steps", "ms_per_step", "config", "sequential_plain_value",
+                                             "speedup_vs_sequential", "sequential_readback_value",
+                                             "speedup_vs_sequential_readback", "launch", "members_bitwise_equal_plain",
+                                             "distinct_member_dt", "roofline")}
+        log(f"adjoint leg: coastal 2048x2048, {args.adj_steps} steps, checkpoint every {args.checkpoint}")
+        a = adjoint_leg(2048, args.adj_steps, args.checkpoint, fd=True, fd_bed_cells=16)
+        log("coastal leg: 16384x16384 flat and dense, 2000 steps")
+        c = coastal_leg(16384, 2000)
+        out["coastal"] = {k: c[k] for k in ("value", "unit", "steps", "ms_per_step", "config", "roofline",
+                                            "dense_ms_per_step", "dense_equals_flat")}
+        log("dam-break leg: 256x256, 2000 steps, GPU float32/float64 and the NumPy oracle")
+        d = dambreak_leg(2000)
+        out["dambreak"] = {k: d[k] for k in ("value", "unit", "steps", "ms_per_step", "config", "float32_vs_float64_gpu",
+                                             "float64_gpu_vs_float64_oracle", "float32_gpu_bitwise_oracle",
+                                             "cpu_baseline")}
+        out["adjoint"] = {k: a[k] for k in ("value", "unit", "steps", "config", "forward_ms_per_step",
+                                            "adjoint_ms_per_step", "plain_forward_ms_per_step",
+                                            "adjoint_over_plain_forward", "forward_cell_updates_per_s", "grad_n",
+                                            "fd_n_eps1e-3", "fd_n_eps1e-4", "fd_vs_adjoint_n", "fd_vs_adjoint_bed",
+                                            "torch_f64_autograd") if k in a}
     print(json.dumps(out), flush=True)
 
 

@@ -148,6 +148,7 @@ _SIGS = {
                              C.c_int, _P]),
     "gs_bench_copy": (C.c_int, [_P, _P, C.c_size_t, _P]),
     "gs_bench_valu": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P]),
+    "gs_crmath_check": (C.c_int, [C.c_int, _P, _P]),
     "gs_flat_mesh_build": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _P]),
     "gs_flat_mesh_free": (C.c_int, [_P]),
     "gs_gen_voronoi": (C.c_int, [_P, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, _P, _P, _P, C.c_int, _P]),

@@ -16,14 +16,42 @@
 // hand-written adjoint against finite differences of these functions).
 #define GS_HD __host__ __device__ __forceinline__
 
+#include "gs_crmath.h"
+
 namespace gs {
 
 template <typename T> GS_HD T vmax(T a, T b) { return fmax(a, b); }
 template <typename T> GS_HD T vmin(T a, T b) { return fmin(a, b); }
 template <> GS_HD float vmax<float>(float a, float b) { return fmaxf(a, b); }
 template <> GS_HD float vmin<float>(float a, float b) { return fminf(a, b); }
-GS_HD float vsqrt(float x) { return sqrtf(x); }
+// Correctly rounded float32 division and square root. On the device they are the short
+// refinements of gs_crmath.h (the IEEE results bit for bit, in 9 instructions where the
+// compiler's general expansions take 11 and 16); on the host, the IEEE operators.
+GS_HD float vsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return sqrt_cr(x);
+#else
+  return sqrtf(x);
+#endif
+}
 GS_HD double vsqrt(double x) { return sqrt(x); }
+// sqrt of an operand that is >= 2^-96, +-0, +inf or NaN (no range branch on the device)
+GS_HD float vsqrt_big(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return sqrt_cr_big(x);
+#else
+  return sqrtf(x);
+#endif
+}
+GS_HD double vsqrt_big(double x) { return sqrt(x); }
+GS_HD float vdiv(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return div_cr(a, b);
+#else
+  return a / b;
+#endif
+}
+GS_HD double vdiv(double a, double b) { return a / b; }
 GS_HD float vabs(float x) { return fabsf(x); }
 GS_HD double vabs(double x) { return fabs(x); }
 
@@ -60,7 +88,7 @@ GS_HD void hllc(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min,
   const T denom = hR * (uNR - SR) - hL * (uNL - SL);
   T SM;
   if (vabs(denom) > T(1.0e-14)) {
-    SM = ((SL * hR) * (uNR - SR) - (SR * hL) * (uNL - SL)) / denom;
+    SM = vdiv((SL * hR) * (uNR - SR) - (SR * hL) * (uNL - SL), denom);
   } else {
     SM = T(0.5) * (uNL + uNR);
   }
@@ -82,7 +110,7 @@ GS_HD void hllc(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min,
     const T hh = lft ? hL : hR;
     const T huN = lft ? huNL : huNR;
     const T F_N = lft ? FL_N : FR_N;
-    const T fac = (S - uN) / ((S - SM) + T(1.0e-30));
+    const T fac = vdiv(S - uN, (S - SM) + T(1.0e-30));
     const T hs = hh * fac;
     Fh = huN + S * (hs - hh);
     FN = F_N + S * (hs * SM - huN);
@@ -131,8 +159,8 @@ template <typename T>
 GS_HD void prims(T h, T hu, T hv, T h_min, T& u, T& v) {
   const bool wet = h > h_min;
   const T hs = wet ? h : h_min;
-  u = wet ? hu / hs : T(0);
-  v = wet ? hv / hs : T(0);
+  u = wet ? vdiv(hu, hs) : T(0);
+  v = wet ? vdiv(hv, hs) : T(0);
 }
 
 // h^(-1/3) from IEEE +,-,* only: Newton on f(y) = y^-3 - h, y <- y*(4/3 - (h/3)*y^3),
@@ -186,55 +214,72 @@ GS_HD void update(T q0, T q1, T q2, T rr0, T r1, T r2, T dt, T n,
     hv = T(0);
   } else if (P.friction) {
     const T hs = (h > P.h_min) ? h : P.h_min;
-    const T u = hu / hs;
-    const T v = hv / hs;
+    const T u = vdiv(hu, hs);
+    const T v = vdiv(hv, hs);
     const T modU = vsqrt(u * u + v * v);
     const T h43 = pow_m43(hs);
     if (P.has_vcap && modU > P.vcap) {
-      const T n_cri = vsqrt(T(1) / ((((T(1.0e-10) + dt) * P.g) * h43) * (modU + T(1.0e-30))));
+      const T n_cri = vsqrt(vdiv(T(1), (((T(1.0e-10) + dt) * P.g) * h43) * (modU + T(1.0e-30))));
       if (n_cri > n) n = n_cri;
     }
     const T Cf = ((P.g * n) * n) * h43;
     T alpha;
     if (P.quadratic) {
       const T two = ((T(2) * dt) * Cf) * modU;
-      alpha = T(2) / (vsqrt(T(1) + T(2) * two) + T(1));
+      // the root's operand is >= 1
+      alpha = vdiv(T(2), vsqrt_big(T(1) + T(2) * two) + T(1));
     } else {
-      alpha = T(1) / (T(1) + ((dt * Cf) * modU));
+      alpha = vdiv(T(1), T(1) + ((dt * Cf) * modU));
     }
     hu = hu * alpha;
     hv = hv * alpha;
   }
 }
 
-// CFL wave speed of one cell as float64 bits; 0 for a skipped cell, canonical NaN bits
-// for a NaN speed. The two layouts follow their own reference kernels:
+// CFL wave speed of one cell as an unsigned key whose maximum is the maximum speed: the
+// IEEE bits of the speed in the state's precision (non-negative floats order as their
+// bits, and a NaN of either sign lies above +inf, so NaN dominates), 0 for a skipped
+// cell. One integer max per cell; lam_key_bits64 turns the reduced key into the float64
+// bits the clock holds (canonical NaN). The two layouts follow their own reference kernels:
 //   dense (FLAT = false), lammax_fp32_lean (solver.py:152-205): cells with h < h_cfl are
 //     skipped and the velocity divides by h;
 //   flat (FLAT = true), cfl_lammax_flat_blk and the fused step's lambda
 //     (compressed_solver.py:200-240, compressed_rhs.py:1041-1053): cells with h < h_min are
 //     skipped and the velocity divides by max(h, h_cfl) (h_min_cfl there).
 // Both keep a NaN speed (compressed_solver.py:200-240 NaN rule).
+template <typename T> struct LamKey { using type = uint64_t; };
+template <> struct LamKey<float> { using type = uint32_t; };
+template <typename T> using lam_t = typename LamKey<T>::type;
+
+__device__ __forceinline__ uint64_t lam_key_bits64(uint32_t k) {
+  if ((k & 0x7FFFFFFFu) > 0x7F800000u) return 0x7FF8000000000000ull;
+  return (uint64_t)__double_as_longlong((double)__uint_as_float(k));
+}
+__device__ __forceinline__ uint64_t lam_key_bits64(uint64_t k) {
+  if ((k & 0x7FFFFFFFFFFFFFFFull) > 0x7FF0000000000000ull) return 0x7FF8000000000000ull;
+  return k;
+}
+
 template <typename T, bool FLAT = false>
-__device__ __forceinline__ uint64_t lam_bits(T h, T hu, T hv, const Phys<T>& P) {
-  if (h < (FLAT ? P.h_min : P.h_cfl)) return 0ull;
+__device__ __forceinline__ lam_t<T> lam_key(T h, T hu, T hv, const Phys<T>& P) {
+  if (h < (FLAT ? P.h_min : P.h_cfl)) return 0;
   const T hs = FLAT ? (h > P.h_cfl ? h : P.h_cfl) : h;
   T vel;
   if (P.cfl_l2) {
-    const T u = hu / hs;
-    const T v = hv / hs;
+    const T u = vdiv(hu, hs);
+    const T v = vdiv(hv, hs);
     vel = vsqrt(u * u + v * v);
   } else {
     // max(|hu/hs|, |hv/hs|) = max(|hu|, |hv|)/hs bit for bit: correctly rounded division
     // by the same positive hs is monotonic, so one division serves both components
     const T au = vabs(hu), av = vabs(hv);
-    vel = ((au > av) ? au : av) / hs;
+    vel = vdiv((au > av) ? au : av, hs);
   }
-  // h >= h_min > 0 (or NaN) here, so sqrt(g*max(h, 0)) of the flat kernels is sqrt(g*h)
+  // h >= h_min > 0 (or NaN) here, so sqrt(g*max(h, 0)) of the flat kernels is sqrt(g*h);
+  // the speed is +0 or more, or NaN
   const T lam = vel + vsqrt(P.g * h);
-  const double l = (double)lam;
-  if (l != l) return 0x7FF8000000000000ull;
-  return (uint64_t)__double_as_longlong(l);
+  if (sizeof(T) == 4) return (lam_t<T>)__float_as_uint((float)lam);
+  return (lam_t<T>)__double_as_longlong((double)lam);
 }
 
 }  // namespace gs

@@ -404,7 +404,7 @@ __device__ __forceinline__ void active_force(const FlatCtx<T>& x, uint8_t fg, in
 }
 
 template <typename T, int MODE>
-__device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, int lane, uint64_t& lam) {
+__device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, int lane, lam_t<T>& lam) {
   const FlatCtx<T>& a = x;
   const Phys<T>& P = a.P;
   const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
@@ -511,7 +511,7 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
   x.qo1[k] = hu;
   x.qo2[k] = hv;
   if (!(fg & GS_FLAG_CFL_SKIP)) {
-    const uint64_t lb_ = lam_bits<T, true>(h, hu, hv, P);
+    const lam_t<T> lb_ = lam_key<T, true>(h, hu, hv, P);
     lam = lb_ > lam ? lb_ : lam;
   }
 }
@@ -560,7 +560,7 @@ __device__ __forceinline__ T sweep_b(const FlatCtx<T>& x, int64_t base, uint32_t
 // must call it. PRE: also write the pre-friction state (the adjoint's recomputation); a
 // separate instantiation keeps those pointers out of the plain step's scalar registers.
 template <typename T, bool PRE, bool EXT>
-__device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int lane, uint64_t& lam) {
+__device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int lane, lam_t<T>& lam) {
   const Phys<T>& P = x.P;
   const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
   const GS_G int32_t* irc = x.tile_irc + 3 * t;
@@ -672,7 +672,7 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
       gat(x.qo1 + gC, oC) = hu;
       gat(x.qo2 + gC, oC) = hv;
       if (!(fg & GS_FLAG_CFL_SKIP)) {
-        const uint64_t lb_ = lam_bits<T, true>(h, hu, hv, P);
+        const lam_t<T> lb_ = lam_key<T, true>(h, hu, hv, P);
         lam = lb_ > lam ? lb_ : lam;
       }
     }
@@ -706,11 +706,11 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
   x.ntab = s_ntab;
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
-  uint64_t lam = 0ull;
+  lam_t<T> lam = 0;
   for (int64_t chunk = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + (threadIdx.x >> 6);
        chunk < a.nchunks; chunk += nwaves)
     flat_chunk<T, MODE>(x, chunk, lane, lam);
-  if (MODE == 1) grid_reduce_and_plan(lam, a.clock, a.plan, true);
+  if (MODE == 1) grid_reduce_and_plan(lam_key_bits64(lam), a.clock, a.plan, true);
 }
 
 // One step of one flat domain by row-sweep tiles, one wavefront per tile in band-major tile
@@ -730,12 +730,12 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(sizeof(
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
-  uint64_t lam = 0ull;
+  lam_t<T> lam = 0;
   // PRE (the adjoint's recomputation) also compiles the extended forcings in, so that a
   // replay reads rain through the per-cell lookup exactly as the forward run's extended sweep
   for (int64_t t = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv; t < a.n_tiles; t += nwaves)
     flat_tile<T, PRE, PRE>(x, t, lane, lam);
-  grid_reduce_and_plan(lam, a.clock, a.plan, true);
+  grid_reduce_and_plan(lam_key_bits64(lam), a.clock, a.plan, true);
 }
 
 // The sweep with the extended forcings compiled in: Green-Ampt infiltration, drain-tau and
@@ -754,10 +754,10 @@ __global__ __launch_bounds__(kFBlock) void k_flat_sweep_ext(FlatK<T> a) {
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
-  uint64_t lam = 0ull;
+  lam_t<T> lam = 0;
   for (int64_t t = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv; t < a.n_tiles; t += nwaves)
     flat_tile<T, false, true>(x, t, lane, lam);
-  grid_reduce_and_plan(lam, a.clock, a.plan, true);
+  grid_reduce_and_plan(lam_key_bits64(lam), a.clock, a.plan, true);
 }
 
 // The forcings on the ring cells of the band (stored, not active): out = force(in), one
@@ -867,7 +867,7 @@ template <typename T>
 __global__ __launch_bounds__(kFBlock) void k_flat_cfl(FlatK<T> a) {
   const int par = (int)(a.clock->nsteps & 1);
   T* const* q = a.buf[par];
-  uint64_t lam = 0ull;
+  lam_t<T> lam = 0;
   for (int64_t r = (int64_t)blockIdx.x * (kFBlock / 64) + (threadIdx.x >> 6); r < a.n_runs;
        r += (int64_t)gridDim.x * (kFBlock / 64)) {
     const int64_t k0 = a.run_k0[r];
@@ -875,11 +875,11 @@ __global__ __launch_bounds__(kFBlock) void k_flat_cfl(FlatK<T> a) {
     for (int t = threadIdx.x & 63; t < len; t += 64) {
       const int64_t k = k0 + t;
       if (a.flags && (a.flags[k] & GS_FLAG_CFL_SKIP)) continue;
-      const uint64_t l = lam_bits<T, true>(q[0][k], q[1][k], q[2][k], a.P);
+      const lam_t<T> l = lam_key<T, true>(q[0][k], q[1][k], q[2][k], a.P);
       lam = l > lam ? l : lam;
     }
   }
-  grid_reduce_and_plan(lam, a.clock, a.plan, false);
+  grid_reduce_and_plan(lam_key_bits64(lam), a.clock, a.plan, false);
 }
 
 template <typename T>
@@ -1158,7 +1158,9 @@ __device__ __forceinline__ void ens_flush(const EnsK<T>& e, int m, uint64_t lam)
 // clock values and the wave's speed maximum change only there. The last workgroup to
 // finish plans each member's next dt from its own maximum (plan_finish), as the plain
 // step's last workgroup does for its single domain.
-template <typename T>
+// EXT compiles the extended forcings (Green-Ampt, drain-tau, rain through a per-cell lookup)
+// in; without them the kernel keeps the plain sweep's register budget.
+template <typename T, bool EXT>
 __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>* __restrict__ mem,
                                                       const int64_t* __restrict__ tile0) {
   const int lane = threadIdx.x & 63;
@@ -1168,7 +1170,7 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>*
   const int64_t tpp = e.tpp;
   int m = -1;
   int64_t c0 = 0, c1 = 0;
-  uint64_t lam = 0ull;
+  lam_t<T> lam = 0;
   FlatClk<T> ck;
   ck.live = false;
   ck.par = 0;
@@ -1178,7 +1180,7 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>*
     const int64_t stop = base + tpp < e.ntiles ? base + tpp : e.ntiles;
     for (int64_t gc = base; gc < stop; ++gc) {
       if (gc >= c1) {
-        ens_flush(e, m, lam);
+        ens_flush(e, m, lam_key_bits64(lam));
         lam = 0ull;
         do {
           ++m;
@@ -1192,10 +1194,10 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>*
       // it stays a register temporary
       FlatCtx<T> x;
       flat_ctx<T, 1>(mem[m], ck, x);
-      flat_tile<T, false, true>(x, gc - c0, lane, lam);
+      flat_tile<T, false, EXT>(x, gc - c0, lane, lam);
     }
   }
-  ens_flush(e, m, lam);
+  ens_flush(e, m, lam_key_bits64(lam));
   __threadfence();   // every wave's maxima are performed before the hand-off below
   __syncthreads();
   __shared__ int s_last;
@@ -1215,10 +1217,11 @@ __global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>*
 }
 
 template <typename T>
-static int ens_occupancy() {
+static int ens_occupancy(bool ext) {
   int occ = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ens_step<T>, kFBlock, 0) != hipSuccess || occ < 1)
-    occ = 1;
+  const hipError_t h = ext ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ens_step<T, true>, kFBlock, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ens_step<T, false>, kFBlock, 0);
+  if (h != hipSuccess || occ < 1) occ = 1;
   return occ;
 }
 
@@ -1325,6 +1328,7 @@ struct GsEns {
   int tpp = 1;
   int cus = 256;
   int occ = 1;                  // resident k_ens_step workgroups per CU
+  bool ext = false;             // some member needs the extended forcings
   std::vector<GsFlat> flat;
   std::vector<GsClock*> clocks;
   hipStream_t gstream = nullptr;
@@ -1371,8 +1375,10 @@ static int ens_build_views(GsEns* e, const GsFlat* members, const GsPhys* phys, 
     const int rc = ens_member_view<T>(members, phys, forcings, e->clocks.data(), m, views[m]);
     if (rc) return rc;
     c0[m + 1] = c0[m] + views[m].n_tiles;
+    if (views[m].ga_ks || views[m].ga_inv_tau || views[m].rain_mode == 3) e->ext = true;
   }
   e->ntiles = c0[e->n];
+  e->occ = ens_occupancy<T>(e->ext);
   GS_HIP(hipMalloc(&e->mem, sizeof(FlatK<T>) * (size_t)e->n), "ensemble views");
   GS_HIP(hipMalloc((void**)&e->tile0, sizeof(int64_t) * ((size_t)e->n + 1)), "ensemble tile table");
   GS_HIP(hipMalloc((void**)&e->done, 256), "ensemble counter");
@@ -1393,7 +1399,10 @@ static int ens_launch(GsEns* e, hipStream_t s) {
   k.ntiles = e->ntiles;
   k.done = e->done;
   const int64_t blocks = ens_grid(e->blocks, e->occ, e->cus, e->ntiles, e->tpp);
-  hipLaunchKernelGGL((k_ens_step<T>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k, k.mem, k.tile0);
+  if (e->ext)
+    hipLaunchKernelGGL((k_ens_step<T, true>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k, k.mem, k.tile0);
+  else
+    hipLaunchKernelGGL((k_ens_step<T, false>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k, k.mem, k.tile0);
   GS_HIP(hipGetLastError(), "k_ens_step launch");
   return GS_OK;
 }
@@ -1455,7 +1464,6 @@ int gs_ens_create(const GsFlat* members, const GsPhys* phys, const GsForcing* fo
   int dev = 0;
   if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e->cus < 1) e->cus = 256;
-  e->occ = dt == GS_F64 ? ens_occupancy<double>() : ens_occupancy<float>();
   int rc = dt == GS_F64 ? ens_build_views<double>(e, members, phys, forcings)
                         : ens_build_views<float>(e, members, phys, forcings);
   if (rc == GS_OK) {
@@ -1641,5 +1649,6 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
   if (best_ms) *best_ms = best;
   return GS_OK;
 }
+
 
 }  // extern "C"

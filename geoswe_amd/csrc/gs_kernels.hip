@@ -41,6 +41,8 @@ __global__ void k_clock_reset(GsClock* c, double t0) {
   c->rain_idx = 0;
   c->use_host_dt = 0;
   c->run_mode = 0;
+  // the padding too, so that a clock's bytes are a function of its history alone
+  for (int i = 0; i < 6; ++i) c->pad_[i] = 0;
 }
 
 __global__ void k_clock_target(GsClock* c, Plan p, double t_end, double dt_max, long long max_steps,
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
   const int lane = threadIdx.x & 63;
   // the wavefront index as a scalar: tile, rows and row addresses are then uniform values
   const int tile = (int)(lb * 4u) + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-  uint64_t lam = 0ull;
+  lam_t<T> lam = 0;
 
   if (tile < a.ntiles) {
     const int rs = tile / a.ncs, cs = tile - rs * a.ncs;
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
           qout[1][idx] = hu;
           qout[2][idx] = hv;
           if (!(fl & GS_FLAG_CFL_SKIP)) {
-            const uint64_t lb_ = lam_bits<T>(h, hu, hv, P);
+            const lam_t<T> lb_ = lam_key<T>(h, hu, hv, P);
             lam = lb_ > lam ? lb_ : lam;
           }
         }
@@ -302,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
       xFh = nFh; xFN = nFN; xFT = nFT; xsR = fn.srcR * ix; xg = fn.gate;
     }
   }
-  if (MODE == 1) grid_reduce_and_plan(lam, a.clock, a.plan, true);
+  if (MODE == 1) grid_reduce_and_plan(lam_key_bits64(lam), a.clock, a.plan, true);
 }
 
 // CFL wave speed of the current dense state, then plan dt (no time advance).
@@ -310,17 +312,17 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_dense_cfl(DenseK<T> a) {
   const int par = (int)(a.clock->nsteps & 1);
   T* const* q = a.buf[par];
-  uint64_t lam = 0ull;
+  lam_t<T> lam = 0;
   const int64_t n = (int64_t)a.nx * a.ny;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
        k += (int64_t)gridDim.x * blockDim.x) {
     const int64_t i = k / a.ny, j = k - i * a.ny;
     const int64_t idx = i * a.pitch + j;
     if (a.flags && (a.flags[idx] & GS_FLAG_CFL_SKIP)) continue;
-    const uint64_t l = lam_bits<T>(q[0][idx], q[1][idx], q[2][idx], a.P);
+    const lam_t<T> l = lam_key<T>(q[0][idx], q[1][idx], q[2][idx], a.P);
     lam = l > lam ? l : lam;
   }
-  grid_reduce_and_plan(lam, a.clock, a.plan, false);
+  grid_reduce_and_plan(lam_key_bits64(lam), a.clock, a.plan, false);
 }
 
 static int pick_tile_rows(int nx, int ncs, int req) {

@@ -147,6 +147,48 @@ __global__ __launch_bounds__(256) void k_valu(float* out, int iters, float m, fl
   out[gid] = sum;
 }
 
+// The short correctly rounded division and square root of gs_crmath.h against the
+// compiler's general IEEE expansions, over every float32 input (or 2^32 seeded pairs).
+// CASE 0: 1/b for every b; 1: sqrt(x) for every x (the guarded form), and the short form for
+// every x with |x| outside (0, 2^-96); 2: a/b for 2^32 pairs of seeded bit patterns (every exponent);
+// 3: a/b for every a over four divisors with the hardest significands (all ones, 1 + ulp,
+// 1 - ulp/2, 3). counts[2*k] = mismatches of check k, counts[2*k + 1] = 1 + the largest
+// offending index (0: none).
+__device__ __forceinline__ bool cr_same(float a, float b) {
+  return (a != a && b != b) || __float_as_uint(a) == __float_as_uint(b);
+}
+__device__ __forceinline__ uint32_t cr_hash(uint64_t i, uint32_t k) {
+  return philox2x32((uint32_t)i, (uint32_t)(i >> 32), k);
+}
+
+template <int CASE>
+__global__ __launch_bounds__(256) void k_crmath_check(unsigned long long* counts) {
+  const uint64_t n = CASE == 3 ? (4ull << 32) : (1ull << 32);
+  unsigned long long bad0 = 0, bad1 = 0, last0 = 0, last1 = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256) {
+    if (CASE == 0) {
+      const float b = __uint_as_float((uint32_t)i);
+      if (!cr_same(div_cr(1.0f, b), 1.0f / b)) { bad0++; last0 = i + 1; }
+    } else if (CASE == 1) {
+      const float x = __uint_as_float((uint32_t)i);
+      const float want = __builtin_sqrtf(x);
+      if (!cr_same(sqrt_cr(x), want)) { bad0++; last0 = i + 1; }
+      if (!(x != 0.0f && __builtin_fabsf(x) < 0x1p-96f) && !cr_same(sqrt_cr_big(x), want)) { bad1++; last1 = i + 1; }
+    } else if (CASE == 2) {
+      const float a = __uint_as_float(cr_hash(i, 0x2545F491u));
+      const float b = __uint_as_float(cr_hash(i, 0x9E3779B9u));
+      if (!cr_same(div_cr(a, b), a / b)) { bad0++; last0 = i + 1; }
+    } else {
+      const uint32_t dv[4] = {0x3FFFFFFFu, 0x3F800001u, 0x3F7FFFFFu, 0x40400000u};
+      const float a = __uint_as_float((uint32_t)i);
+      const float b = __uint_as_float(dv[i >> 32]);
+      if (!cr_same(div_cr(a, b), a / b)) { bad0++; last0 = i + 1; }
+    }
+  }
+  if (bad0) { atomicAdd(&counts[0], bad0); atomicMax(&counts[1], last0); }
+  if (bad1) { atomicAdd(&counts[2], bad1); atomicMax(&counts[3], last1); }
+}
+
 extern "C" {
 
 int gs_fill(void* dst, int dtype, int64_t rows, int64_t cols, int64_t pitch, double value, void* stream) {
@@ -212,6 +254,20 @@ int gs_bench_valu(float* out, int blocks, int iters, int mode, void* stream) {
   else
     hipLaunchKernelGGL(k_valu<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, out, iters, 0.9999f, 1.0e-4f);
   GS_HIP(hipGetLastError(), "k_valu launch");
+  return GS_OK;
+}
+
+int gs_crmath_check(int which, unsigned long long* counts_dev, void* stream) {
+  if (!counts_dev || which < 0 || which > 3) return fail(GS_E_ARG, "gs_crmath_check: counts and which in 0..3 required");
+  hipStream_t s = (hipStream_t)stream;
+  const dim3 g(8192), b(256);
+  switch (which) {
+    case 0: hipLaunchKernelGGL(k_crmath_check<0>, g, b, 0, s, counts_dev); break;
+    case 1: hipLaunchKernelGGL(k_crmath_check<1>, g, b, 0, s, counts_dev); break;
+    case 2: hipLaunchKernelGGL(k_crmath_check<2>, g, b, 0, s, counts_dev); break;
+    default: hipLaunchKernelGGL(k_crmath_check<3>, g, b, 0, s, counts_dev); break;
+  }
+  GS_HIP(hipGetLastError(), "k_crmath_check launch");
   return GS_OK;
 }
 

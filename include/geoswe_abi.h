@@ -458,6 +458,7 @@ int gs_ens_run(GsEns* e, int64_t nsteps, int graph_batch, void* stream);
  * for the snapshot. */
 int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms);
 
+
 /* ---- graph capture of step entry points ---- */
 /*
  * Replaces: the reference's K-step capture of its fixed-dt dense step in a CUDA graph
@@ -508,6 +509,13 @@ int gs_bench_copy(void* dst, const void* src, size_t bytes, void* stream);
  * mode 1: hardware square root, multiply, hardware reciprocal, add (32 per step). `out`
  * holds blocks * 256 floats. No reference counterpart. */
 int gs_bench_valu(float* out, int blocks, int iters, int mode, void* stream);
+/* Check of the short correctly rounded float32 division and square root the kernels use
+ * against the compiler's general IEEE expansions on this device. which 0: 1/b for every
+ * float32 b; 1: sqrt(x) for every x (the guarded form; the short form where |x| is outside (0, 2^-96));
+ * 2: a/b for 2^32 seeded bit-pattern pairs; 3: a/b for every a over four divisors of hard
+ * significands. counts_dev (4 x uint64, zeroed by the caller) receives, per check, the
+ * mismatch count and 1 + the largest offending index. No reference counterpart. */
+int gs_crmath_check(int which, unsigned long long* counts_dev, void* stream);
 
 #ifdef __cplusplus
 }

@@ -157,3 +157,34 @@ def test_gradient_with_native_grid_rain(gpu):
     (Ja, na, ba), (Jb, nb, bb) = out
     assert Ja == Jb and np.array_equal(na, nb) and np.array_equal(ba, bb)
     assert np.abs(nb).max() > 0
+
+
+def test_float32_adjoint_against_float64_and_torch(gpu):
+    """The float32 build's gradient (the precision of the calibration benchmark) against the
+    float64 build on the same problem (the float32 run's inputs, dt sequence, gauges and
+    truth depths) and against float64 autograd through the torch restatement: the coastal
+    generator at 128x128 over 200 steps, 32 gauges, 8 Manning classes."""
+    torch = pytest.importorskip("torch")
+    from geoswe_amd.adjoint import GaugeMisfit
+    case, p32 = AC.coastal_problem(n=128, S=200, K=50, dtype="float32", n_gauges=32)
+    n0, b0 = p32.params()
+    p32.simulate()
+    gn32, gb32 = p32.gradient()
+    case.cfg.dtype = "float64"
+    s64 = case.flat(track_max_depth=False)
+    p64 = GaugeMisfit(s64, p32.gauges, p32.dts, h_star=p32.h_star, checkpoint_every=50)
+    p64.set_params(ntab=n0, bed=b0)
+    p64.simulate()
+    gn64, gb64 = p64.gradient()
+    assert np.abs(p32.samples - p64.samples).max() <= 1e-5 * np.abs(p64.samples).max()
+    nt = torch.tensor(n0, dtype=torch.float64, requires_grad=True)
+    bt = torch.tensor(b0, dtype=torch.float64, requires_grad=True)
+    Jt, _ = AC.torch_misfit(case, p64, nt, bt)
+    Jt.backward()
+    tn, tb = nt.grad.numpy(), bt.grad.numpy()
+    assert np.abs(gn64 - tn).max() <= 1e-7 * np.abs(tn).max()
+    assert np.linalg.norm(gb64 - tb) <= 1e-7 * np.linalg.norm(tb)
+    print(f"float32 vs float64: class n max rel {np.abs(gn32 - gn64).max() / np.abs(gn64).max():.3e}, "
+          f"bed normwise {np.linalg.norm(gb32 - gb64) / np.linalg.norm(gb64):.3e}")
+    assert np.abs(gn32 - gn64).max() <= 1e-2 * np.abs(gn64).max()
+    assert np.linalg.norm(gb32 - gb64) <= 5e-2 * np.linalg.norm(gb64)
