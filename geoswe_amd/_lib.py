@@ -141,6 +141,8 @@ _SIGS = {
     "gs_ens_step": (C.c_int, [_P, _P]),
     "gs_ens_run": (C.c_int, [_P, C.c_int64, C.c_int, _P]),
     "gs_ens_autotune": (C.c_int, [_P, C.c_int, _P, C.POINTER(C.c_double)]),
+    "gs_ens_run_noalloc": (C.c_int, [C.POINTER(GsFlat), C.POINTER(GsPhys), C.POINTER(GsForcing), C.POINTER(_P),
+                                     C.c_int, C.c_int64, _P]),
     "gs_fill": (C.c_int, [_P, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_double, _P]),
     "gs_gen_bed": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_uint32,
                              C.c_int,

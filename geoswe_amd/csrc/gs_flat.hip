@@ -15,6 +15,7 @@
 // cross-lane shuffle. The face arithmetic is gs_device.h's, shared with the dense sweep,
 // so the residual is bitwise the dense one.
 #include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "gs_common.h"
@@ -1650,5 +1651,123 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms) {
   return GS_OK;
 }
 
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// the ensemble run with no allocation
+// ---------------------------------------------------------------------------
+// The device memory k_ens_step needs beyond the members' own buffers (the last-workgroup
+// counter, the tile prefix and the member views) for gs_ens_run_noalloc, which allocates
+// nothing: layout [counter, 256 B][tile prefix (n + 1) x int64, padded to 256 B][views].
+// Every device holds its own copy at its own address (looked up per device); host code uses
+// that address, never the array's name.
+constexpr size_t kScratchBytes = 256 * 1024;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace gs {
+
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;                    // guards the table and the host staging copy
+static void* g_scratch_dev[kScratchDevices];       // scratch's address on each device
+alignas(256) static unsigned char g_scratch_host[kScratchBytes];   // what a call copies into scratch
+
+// The address of `scratch` on the current device; the caller holds g_scratch_mu.
+static int scratch_addr(void** out) {
+  int dev = 0;
+  GS_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kScratchDevices) return fail(GS_E_RANGE, "device %d: the scratch table covers %d devices", dev,
+                                                     kScratchDevices);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    GS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(GS_E_STATE, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return GS_OK;
+}
+
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Bytes of scratch the no-allocation run needs for n members (0 if they do not fit).
+template <typename T>
+static size_t noalloc_bytes(int n) {
+  const size_t need = align256(256 + 8 * ((size_t)n + 1)) + sizeof(FlatK<T>) * (size_t)n;
+  return need <= kScratchBytes ? need : 0;
+}
+
+template <typename T>
+static int ens_run_noalloc_t(const GsFlat* members, const GsPhys* phys, const GsForcing* forcings,
+                             GsClock* const* clocks, int n, int64_t nsteps, hipStream_t s) {
+  const size_t need = noalloc_bytes<T>(n);
+  if (!need) {
+    int cap = 0;
+    while (noalloc_bytes<T>(cap + 1)) ++cap;
+    return fail(GS_E_RANGE, "gs_ens_run_noalloc: %d members exceed the %zu-byte scratch (at most %d members of "
+                "this dtype)", n, kScratchBytes, cap);
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  void* base = nullptr;
+  int rc = scratch_addr(&base);
+  if (rc) return rc;
+  const size_t off_tile = 256, off_views = align256(256 + 8 * ((size_t)n + 1));
+  memset(g_scratch_host, 0, off_views);            // the counter starts at zero
+  int64_t* c0 = (int64_t*)(g_scratch_host + off_tile);
+  FlatK<T>* views = (FlatK<T>*)(g_scratch_host + off_views);
+  bool ext = false;
+  c0[0] = 0;
+  for (int m = 0; m < n; ++m) {
+    new (&views[m]) FlatK<T>();
+    rc = ens_member_view<T>(members, phys, forcings, clocks, m, views[m]);
+    if (rc) return rc;
+    c0[m + 1] = c0[m] + views[m].n_tiles;
+    if (views[m].ga_ks || views[m].ga_inv_tau || views[m].rain_mode == 3) ext = true;
+  }
+  // steps of an earlier call on this stream may still read the scratch: let them finish
+  // before it is overwritten (a synchronous copy, so the host staging is free on return)
+  GS_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+  GS_HIP(hipMemcpy(base, g_scratch_host, need, hipMemcpyHostToDevice), "scratch h2d");
+  int cus = 256, dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (cus < 1) cus = 256;
+  EnsK<T> k;
+  k.mem = (const FlatK<T>*)((unsigned char*)base + off_views);
+  k.tile0 = (const int64_t*)((unsigned char*)base + off_tile);
+  k.n_members = n;
+  k.tpp = 1;
+  k.ntiles = c0[n];
+  k.done = (unsigned*)base;
+  const int64_t blocks = ens_grid(0, ens_occupancy<T>(ext), cus, k.ntiles, 1);
+  for (int64_t i = 0; i < nsteps; ++i) {
+    if (ext)
+      hipLaunchKernelGGL((k_ens_step<T, true>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k, k.mem, k.tile0);
+    else
+      hipLaunchKernelGGL((k_ens_step<T, false>), dim3((unsigned)blocks), dim3(kFBlock), 0, s, k, k.mem, k.tile0);
+    GS_HIP(hipGetLastError(), "k_ens_step launch");
+  }
+  return GS_OK;
+}
+
+}  // namespace gs
+
+extern "C" {
+
+int gs_ens_run_noalloc(const GsFlat* members, const GsPhys* phys, const GsForcing* forcings, GsClock* const* clocks,
+                       int n_members, int64_t nsteps, void* stream) {
+  using namespace gs;
+  if (n_members < 1) return fail(GS_E_RANGE, "gs_ens_run_noalloc: n_members must be >= 1");
+  if (!members || !phys || !forcings || !clocks) return fail(GS_E_ARG, "gs_ens_run_noalloc: member arrays are NULL");
+  if (nsteps < 0) return fail(GS_E_ARG, "gs_ens_run_noalloc: nsteps < 0");
+  const int dt = members[0].dtype;
+  if (dt != GS_F32 && dt != GS_F64) return fail(GS_E_DTYPE, "unsupported dtype %d", dt);
+  for (int m = 0; m < n_members; ++m) {
+    if (members[m].dtype != dt) return fail(GS_E_DTYPE, "member %d dtype differs from member 0", m);
+    if (!clocks[m]) return fail(GS_E_ARG, "clock of member %d is NULL", m);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  return dt == GS_F64 ? ens_run_noalloc_t<double>(members, phys, forcings, clocks, n_members, nsteps, s)
+                      : ens_run_noalloc_t<float>(members, phys, forcings, clocks, n_members, nsteps, s);
+}
 
 }  // extern "C"

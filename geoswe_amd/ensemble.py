@@ -158,3 +158,35 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+def run_noalloc(members: Sequence[CompressedSolver], nsteps: int, sync: bool = True) -> None:
+    """nsteps steps of every member through ``gs_ens_run_noalloc``: the ensemble step with
+    no device allocation (its member table lives in a static device array of the library),
+    for callers that cannot allocate. Members use their own tile tables; bitwise the steps
+    an ``Ensemble`` of the same members takes."""
+    members = list(members)
+    if not members:
+        raise ValueError("run_noalloc needs at least one member")
+    for k, m in enumerate(members):
+        if not isinstance(m, CompressedSolver):
+            raise TypeError(f"member {k} is {type(m).__name__}, not CompressedSolver")
+        if m._rmode is not None or m._force_ring is not None:
+            raise NotImplementedError(f"member {k} has ring modes or forced ring cells; the ensemble step updates "
+                                      f"active cells only")
+        m._tables_taken = True
+        m._ensure_lam()
+        m._open_mode()
+    n = len(members)
+    descs = (L.GsFlat * n)(*[m._desc() for m in members])
+    phys = (L.GsPhys * n)(*[m._phys() for m in members])
+    forc = (L.GsForcing * n)(*[m._fo.s for m in members])
+    clocks = (C.c_void_p * n)(*[m._clock.ptr for m in members])
+    L.call("gs_ens_run_noalloc", descs, phys, forc, clocks, n, C.c_int64(int(nsteps)), None)
+    for m in members:
+        m._any_step = True
+    if sync:
+        L.sync(None)
+        bad = [k for k, m in enumerate(members) if m._clockv().status & L.ST_NONFINITE]
+        if bad:
+            raise FloatingPointError(f"run_noalloc: non-finite wave speed in member(s) {bad}")

@@ -458,6 +458,18 @@ int gs_ens_run(GsEns* e, int64_t nsteps, int graph_batch, void* stream);
  * for the snapshot. */
 int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms);
 
+/* No reference counterpart: for callers that cannot allocate device memory (tight serving
+ * loops that step a batch of domains, harnesses that forbid allocation). nsteps steps of
+ * every member, bitwise what gs_ens_create + gs_ens_run(nsteps, 0) + gs_ens_destroy give,
+ * without any device allocation: the member table, the tile prefix and the last-workgroup
+ * counter live in a static 256 KiB device array of the library (each GPU holds its own
+ * until the process ends), so at most 414 float32 or 389 float64 members (GS_E_RANGE
+ * beyond). Members use their own tile tables and one occupancy wave of workgroups. The call
+ * waits for earlier work on `stream` (an earlier call's steps may still read the array),
+ * copies the table synchronously, then enqueues the steps; calls that overlap on one GPU
+ * (other streams or threads) race on the array. */
+int gs_ens_run_noalloc(const GsFlat* members, const GsPhys* phys, const GsForcing* forcings,
+                       GsClock* const* clocks, int n_members, int64_t nsteps, void* stream);
 
 /* ---- graph capture of step entry points ---- */
 /*
