@@ -485,6 +485,9 @@ def adjoint_leg(n: int, S: int, K: int, sweep_rows=None, fd: bool = True, fd_bed
     st_b = stats(gb[cells], fd_b)
     log(f"FD (eps 1e-3 / 1e-4) vs adjoint, class n: {st_n}")
     log(f"FD (eps 1e-3 / 1e-4) vs adjoint, bed cells near the gauges: {st_b}")
+    del prob, f
+    horizon = fd64_horizon_check(n, 500)
+    log(f"float64 central differences vs adjoint at 500 steps: {horizon}")
     torch_err = None
     try:
         torch_err = torch_check()
@@ -504,8 +507,61 @@ def adjoint_leg(n: int, S: int, K: int, sweep_rows=None, fd: bool = True, fd_bed
         "grad_n": gn.tolist(), "fd_n_eps1e-3": fd_n[0].tolist(), "fd_n_eps1e-4": fd_n[1].tolist(),
         "fd_vs_adjoint_n": st_n, "fd_vs_adjoint_bed": st_b,
         "torch_f64_autograd": torch_err,
+        "fd64_500_steps": horizon,
     }
     return out
+
+
+def fd64_horizon_check(n: int, S: int) -> dict:
+    """The adjoint against float64 central differences on the calibration case over a
+    horizon where the loss is smooth: the float64 build of the same coastal problem (its own
+    CFL dt sequence, gauges and truth run, as the float32 leg builds them) over S steps;
+    per class, differences at eps = 1e-5, 1e-6, 1e-7; a class counts as smooth where two
+    consecutive eps agree to 1e-3, and the adjoint is compared with the finer of the two."""
+    from geoswe_amd.adjoint import GaugeMisfit
+    from geoswe_amd.cases import coastal_c3
+    case = coastal_c3(n=n)
+    case.cfg.dtype = "float64"
+    ref = case.flat(track_max_depth=False)
+    dts = [ref.step() for _ in range(S)]
+    del ref
+    f = case.flat(track_max_depth=False)
+    fm = f.fm
+    rng = np.random.default_rng(20261205)
+    gauges = np.sort(rng.choice(np.flatnonzero(fm.active & (f.q_flat()[0] > 0.05)), size=32, replace=False))
+    prob = GaugeMisfit(f, gauges, np.array(dts), checkpoint_every=100)
+    n0, _ = prob.params()
+    prob.set_params(ntab=n0 * (1.0 + 0.3 * rng.uniform(-1.0, 1.0, len(n0))))
+    prob.h_star = prob.simulate(keep_checkpoints=False).copy()
+    prob.set_params(ntab=n0)
+    prob.simulate()
+    gn, _ = prob.gradient()
+
+    def J(nn):
+        prob.set_params(ntab=nn)
+        prob.simulate(keep_checkpoints=False)
+        return prob.loss()
+
+    epss = (1e-5, 1e-6, 1e-7)
+    rows, rel = [], []
+    for c in range(len(n0)):
+        fd = []
+        for eps in epss:
+            e = np.zeros_like(n0)
+            e[c] = eps
+            fd.append((J(n0 + e) - J(n0 - e)) / (2 * eps))
+        smooth = None
+        for a_, b_ in zip(fd[:-1], fd[1:]):
+            if abs(a_ - b_) <= 1e-3 * abs(b_):
+                smooth = b_
+        r = None if smooth is None else abs(gn[c] - smooth) / abs(smooth)
+        if r is not None:
+            rel.append(r)
+        rows.append({"class": c, "adjoint": float(gn[c]), "fd": [float(x) for x in fd], "rel_err": r})
+    prob.set_params(ntab=n0)
+    return {"steps": S, "eps": list(epss), "classes": rows, "smooth_classes": len(rel),
+            "max_rel_err_smooth": max(rel) if rel else None,
+            "within_1e-3": float(np.mean(np.array(rel) <= 1e-3)) if rel else None}
 
 
 def torch_check():
