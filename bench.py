@@ -60,35 +60,64 @@ def timed_steps(s, steps):
 
 
 def fp64_error(nx, ny, bed, steps):
-    """Max |h32 - h64| after `steps` steps of the rain-on-slope case: the float32 run against
-    the float64 build of the kernel on the float32 run's inputs and wet/dry thresholds (its
-    own CFL steps), and against the float64 build replaying the float32 run's dt sequence
-    (which takes the dt schedule out of the comparison)."""
+    """Max |h - h64| after `steps` steps of the rain-on-slope case, h64 from the float64 build
+    of the kernel on the float32 run's inputs and wet/dry thresholds: the float32 run (h32)
+    against the float64 run with its own CFL steps and against the float64 build replaying
+    the float32 run's dt sequence (which takes the dt schedule out of the comparison), and the
+    float32 planes stepped with float64 arithmetic against the float64 run."""
     from geoswe_amd.cases import rain_on_slope_c2
-    log(f"float64 comparison: float32 and float64 runs of {steps} steps")
+    log(f"float64 comparison: float32, float64 and mixed runs of {steps} steps")
+
+    def err(h, h64, name):
+        e = np.abs(h - h64)
+        k = int(np.argmax(e))
+        i, j = np.unravel_index(k, e.shape)
+        r = {"max": float(e.max()), "at": [int(i), int(j)], "p99999": float(np.percentile(e, 99.999)),
+             "median": float(np.median(e)), "cells_above_1e-5": int((e > 1e-5).sum())}
+        log(f"{name}: max |h - h64| {e.max():.3e} m at ({i}, {j}); 99.999th percentile {r['p99999']:.3e} m, "
+            f"median {r['median']:.3e} m")
+        return r
+
     s = rain_on_slope_c2(nx, ny, "float32", bed=bed)
     dts = [s.step() for _ in range(steps)]
     h32 = s.depth().astype(np.float64)
     t32 = s.t
     del s
-    out = {"steps": steps}
-    for name, dt_seq in (("own_dt", None), ("frozen_dt", dts)):
-        s64 = rain_on_slope_c2(nx, ny, "float64", bed=bed, f32_inputs=True)
-        if dt_seq is None:
-            s64.advance(steps)
-        else:
-            for dt in dt_seq:
-                s64.step(dt)
-        e = np.abs(h32 - s64.depth())
-        k = int(np.argmax(e))
-        i, j = np.unravel_index(k, e.shape)
-        out[name] = {"max": float(e.max()), "at": [int(i), int(j)], "p99999": float(np.percentile(e, 99.999)),
-                     "median": float(np.median(e)), "cells_above_1e-5": int((e > 1e-5).sum()), "t64": s64.t}
-        log(f"float64 ({name}): max |h32 - h64| {e.max():.3e} m at ({i}, {j}); 99.999th percentile "
-            f"{out[name]['p99999']:.3e} m, median {out[name]['median']:.3e} m")
-        del s64, e
-    out["t32"] = t32
+    s = rain_on_slope_c2(nx, ny, "float32", bed=bed, compute_dtype="float64")
+    s.advance(steps)
+    hmx = s.depth().astype(np.float64)
+    del s
+    out = {"steps": steps, "t32": t32}
+    s64 = rain_on_slope_c2(nx, ny, "float64", bed=bed, f32_inputs=True)
+    s64.advance(steps)
+    h64 = s64.depth()
+    out["own_dt"] = err(h32, h64, "float32 vs float64")
+    out["own_dt"]["t64"] = s64.t
+    out["mixed_vs_fp64"] = err(hmx, h64, "float32 planes with float64 arithmetic vs float64")
+    del s64, h64, hmx
+    s64 = rain_on_slope_c2(nx, ny, "float64", bed=bed, f32_inputs=True)
+    for dt in dts:
+        s64.step(dt)
+    out["frozen_dt"] = err(h32, s64.depth(), "float32 vs float64 replaying the float32 dt sequence")
+    out["frozen_dt"]["t64"] = s64.t
+    del s64
     out["max_depth_err_vs_fp64_m"] = out["own_dt"]["max"]
+    return out
+
+
+def mixed_timing(nx, ny, bed, steps, warmup):
+    """The rain-on-slope step with float32 planes and float64 arithmetic (GS_F32X64): wall
+    and HIP-event time over `steps` steps after `warmup`."""
+    from geoswe_amd.cases import rain_on_slope_c2
+    s = rain_on_slope_c2(nx, ny, "float32", bed=bed, compute_dtype="float64")
+    s.advance(warmup)
+    wall, ev_ms = timed_steps(s, steps)
+    cells = nx * ny
+    out = {"value": cells * steps / wall, "unit": "cell-updates/s", "steps": steps, "ms_per_step": 1e3 * wall / steps,
+           "kernel_ms": ev_ms / steps, "kernel": "gs::k_dense_sweep<double, 1, float>",
+           "hbm_frac": BYTES_PER_CELL["dense"] * cells / (ev_ms / steps * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"float32 planes, float64 arithmetic: {out['ms_per_step']:.3f} ms/step, {out['value'] / 1e9:.2f} G "
+        f"cell-updates/s")
     return out
 
 
@@ -772,9 +801,12 @@ def main():
     log(f"{ms_step:.3f} ms/step wall, {kern_ms:.3f} ms/launch (events), {value / 1e9:.2f} G cell-updates/s, "
         f"{achieved:.0f} GB/s algorithmic; copy kernel {copy['GBps_read_plus_write']:.0f} GB/s")
     err64 = None
+    mixed = None
     if not args.no_fp64 and args.layout == "dense":
         del s
+        mixed = mixed_timing(nx, ny, bed, args.steps, args.warmup)
         err64 = fp64_error(nx, ny, bed, args.err_steps)
+        mixed["max_depth_err_vs_fp64_m"] = err64["mixed_vs_fp64"]["max"]
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (NumPy oracle on a crop)")
@@ -806,6 +838,9 @@ def main():
                      "frac_of_copy": achieved / copy["GBps_read_plus_write"]},
         "max_depth_err_vs_fp64_m": err64["max_depth_err_vs_fp64_m"] if err64 else None,
         "fp64_comparison": err64,
+        # the same step with float32 planes and float64 arithmetic (Config.compute_dtype):
+        # the configuration meeting the float64 depth tolerance, timed over the same steps
+        "float64_arithmetic": mixed,
         "cpu_baseline": cpu,
     }
     if not args.no_legs and (nx, ny) == (25600, 25000) and args.layout == "dense":

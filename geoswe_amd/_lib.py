@@ -17,7 +17,7 @@ _HERE = Path(__file__).resolve().parent
 LIB_PATH = _HERE / "_lib" / "libgeoswe_hip.so"
 
 GS_BC = {"extrapolate": 0, "periodic": 1, "wall": 2, "fall": 3}
-GS_F32, GS_F64 = 0, 1
+GS_F32, GS_F64, GS_F32X64 = 0, 1, 2
 FLAG_CLASS_MASK, FLAG_CFL_SKIP, FLAG_INACTIVE, FLAG_STAGE = 0x1F, 0x20, 0x40, 0x80
 FLAG_FORCE = 0x40          # flat layout: sponge / clamp cell (the dense layout's inactive bit)
 MAX_CLASSES = 32

@@ -71,20 +71,21 @@ def dam_break_c1(n: int = 256, dtype: str = "float32"):
 
 
 def rain_on_slope_c2(nx: int = 25600, ny: int = 25000, dtype: str = "float32", track_max_depth: bool = False,
-                     tile_rows: int = 0, bed: FbmBed = None, f32_inputs: bool = False):
+                     tile_rows: int = 0, bed: FbmBed = None, f32_inputs: bool = False, compute_dtype: str = None):
     """Solver2D of the rain-on-slope case, state and bed built on the device.
 
     ``f32_inputs`` rounds every input (bed, h0, n, rain rate) to float32 first and takes the
     float32 run's wet/dry thresholds (h_min = h_min_cfl = 1e-6 instead of the float64
     default 1e-10): a float64 run on exactly the inputs and parameters of the float32 run,
-    which isolates the arithmetic error."""
+    which isolates the arithmetic error. ``compute_dtype="float64"`` with dtype float32 steps
+    the float32 planes with float64 arithmetic."""
     from .solver import Solver2D
     bed = bed or FbmBed(seed=C2_SEED).normalise(nx, ny)
     r = (lambda x: float(np.float32(x))) if f32_inputs else (lambda x: x)
     thr = dict(h_min=1.0e-6, h_min_cfl=1.0e-6) if f32_inputs else {}
     mesh = Mesh2D(nx, ny, 3.0, 3.0, ngh=4)
     cfg = Config(bc_x="fall", bc_y="fall", cfl=0.5, dtype=dtype, friction="manning", manning_n=r(0.03),
-                 rainfall=r(50.0 / 3.6e6), **thr)
+                 rainfall=r(50.0 / 3.6e6), compute_dtype=compute_dtype, **thr)
     s = Solver2D(mesh, cfg, None, None, track_max_depth=track_max_depth, tile_rows=tile_rows,
                  device_init=(r(0.1), lambda ptr, pitch, dcode: bed.write(ptr, pitch, dcode, nx, ny,
                                                                           round_f32=f32_inputs)))

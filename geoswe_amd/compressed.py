@@ -88,6 +88,9 @@ class CompressedSolver:
         the CFL velocity-divisor floor of the flat wave speed (the reference's
         CompressedStepper argument; None: SWE_HMIN_CFL, else h_min)."""
         mesh, cfg = s.mesh, s.cfg
+        if getattr(s, "compute64", False):
+            raise NotImplementedError("CompressedSolver.from_dense: float64 arithmetic on float32 planes "
+                                      "(Config.compute_dtype='float64') is a dense-layout mode")
         g, nx, ny = mesh.ngh, mesh.nx, mesh.ny
         nxp, nyp = nx + 2 * g, ny + 2 * g
         inside = np.zeros((nxp, nyp), bool)

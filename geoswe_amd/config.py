@@ -120,6 +120,9 @@ class Config:
     rk_storage: str = "low_storage"
     dtype: Optional[str] = None
     sigma_halo_every: int = 1
+    # No reference counterpart: "float64" with dtype "float32" steps the float32 planes with
+    # float64 arithmetic (each stored value rounded once to float32); dense layout only.
+    compute_dtype: Optional[str] = None
 
     def __post_init__(self):
         # the float32 build is the default here (the package computes on the GPU); other
@@ -145,6 +148,10 @@ class Config:
                 continue                     # no friction
             if not (isinstance(v, str) and v in allowed):
                 raise ValueError(f"Config.{name}={v!r} is not one of {sorted(allowed)}")
+        if self.compute_dtype not in (None, self.dtype) and not (self.dtype == "float32"
+                                                                 and self.compute_dtype == "float64"):
+            raise ValueError(f"Config.compute_dtype={self.compute_dtype!r} with dtype={self.dtype!r}: the arithmetic "
+                             f"is the state's dtype, or float64 for a float32 state")
         check_numeric_inputs(where="Config", cfl=self.cfl, g=self.g, h_min=self.h_min,
                              h_min_cfl=self.h_min_cfl, manning_n=self.manning_n,
                              friction_velocity_cap_ms=self.friction_velocity_cap_ms,

@@ -72,22 +72,24 @@ __global__ void k_clock_dt(GsClock* c, Plan p, double dt) {
 constexpr int kOut = 60;      // columns written per strip (lanes 2..61)
 constexpr int kBlock = 256;   // 4 strips per workgroup
 
-template <typename T>
+// T is the arithmetic type, S the type of the planes in memory (S = float with T = double:
+// float32 planes stepped with float64 arithmetic, each value rounded once when it is stored).
+template <typename T, typename S = T>
 struct DenseK {
   int nx, ny, bc_x, bc_y, tile_rows, ncs, nrs, ntiles;
   int64_t pitch;
-  T* buf[2][3];
-  const T* bed;
+  S* buf[2][3];
+  const S* bed;
   const uint8_t* flags;
-  const T* ntab;
+  const S* ntab;
   int n_classes;
-  const T* n_field;
-  T* max_h;
+  const S* n_field;
+  S* max_h;
   T* rhs[3];
   int which;
   Phys<T> P;
   int rain_mode;  // 0 none, 1 uniform, 2 gridded
-  const T* rain_frames;
+  const S* rain_frames;
   int64_t frame_stride, rain_pitch;
   int stage;
   GsClock* clock;
@@ -127,30 +129,30 @@ __device__ __forceinline__ ColMap col_map(int c, int ny, int bc) {
 }
 
 // h, hu, hv of cell (i, column map cm); the row map is uniform across the wavefront.
-template <typename T>
-__device__ __forceinline__ void load_q(const DenseK<T>& a, T* const* q, int i, const ColMap& cm, T& h, T& hu,
+template <typename T, typename S>
+__device__ __forceinline__ void load_q(const DenseK<T, S>& a, S* const* q, int i, const ColMap& cm, T& h, T& hu,
                                        T& hv) {
   bool zx = false, fx = false;
   const int mi = map_state(i, a.nx, a.bc_x, zx, fx);
   const int64_t idx = (int64_t)mi * a.pitch + cm.mj;
-  h = q[0][idx];
-  hu = q[1][idx];
-  hv = q[2][idx];
+  h = (T)q[0][idx];
+  hu = (T)q[1][idx];
+  hv = (T)q[2][idx];
   if (fx) hu = -hu;
   if (cm.flip) hv = -hv;
   if (zx || cm.zero) { h = T(0); hu = T(0); hv = T(0); }
 }
 
-template <typename T>
-__device__ __forceinline__ T load_b(const DenseK<T>& a, int i, const ColMap& cm) {
+template <typename T, typename S>
+__device__ __forceinline__ T load_b(const DenseK<T, S>& a, int i, const ColMap& cm) {
   const int mi = map_bed(i, a.nx, a.bc_x);
-  return a.bed[(int64_t)mi * a.pitch + cm.mjb];
+  return (T)a.bed[(int64_t)mi * a.pitch + cm.mjb];
 }
 
 
 // MODE 0: residual into a.rhs; MODE 1: fused step.
-template <typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
+template <typename T, int MODE, typename S = T>
+__global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T, S> a) {
   int par = 0;
   T dt = T(0), rain_u = T(0), eta = T(0);
   int ridx = 0;
@@ -170,14 +172,14 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
     }
   }
   __shared__ T s_ntab[32];
-  if (MODE == 1 && threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
+  if (MODE == 1 && threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = (T)a.ntab[threadIdx.x];
   __syncthreads();
 
-  T* const* qin = MODE == 1 ? a.buf[par] : a.buf[a.which];
-  T* const* qout = a.buf[par ^ 1];
+  S* const* qin = MODE == 1 ? a.buf[par] : a.buf[a.which];
+  S* const* qout = a.buf[par ^ 1];
   const Phys<T>& P = a.P;
   const T ix = P.inv_dx, iy = P.inv_dy, hdx = P.half_dx, hdy = P.half_dy, g = P.g, hm = P.h_min;
-  const T* frame = (MODE == 1 && a.rain_mode == 2) ? a.rain_frames + (int64_t)ridx * a.frame_stride : nullptr;
+  const S* frame = (MODE == 1 && a.rain_mode == 2) ? a.rain_frames + (int64_t)ridx * a.frame_stride : nullptr;
 
   // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs; give each XCD a
   // contiguous range of tiles so neighbouring strips share that XCD's L2.
@@ -275,10 +277,10 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
           T rr0 = rh;
           if (!inactive) {
             if (a.rain_mode == 1) rr0 = rr0 + rain_u;
-            else if (a.rain_mode == 2) rr0 = rr0 + frame[(int64_t)i * a.rain_pitch + c];
+            else if (a.rain_mode == 2) rr0 = rr0 + (T)frame[(int64_t)i * a.rain_pitch + c];
           }
           T n = T(0);
-          if (P.friction) n = a.n_field ? a.n_field[idx] : s_ntab[fl & GS_FLAG_CLASS_MASK];
+          if (P.friction) n = a.n_field ? (T)a.n_field[idx] : s_ntab[fl & GS_FLAG_CLASS_MASK];
           T h, hu, hv;
           update<T>(hC, huC, hvC, rr0, rhu, rhv, dt, n, P, h, hu, hv);
           if (fl & GS_FLAG_STAGE) {
@@ -287,12 +289,15 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
             hu = T(0);
             hv = T(0);
           }
-          if (a.max_h && h > a.max_h[idx]) a.max_h[idx] = h;
-          qout[0][idx] = h;
-          qout[1][idx] = hu;
-          qout[2][idx] = hv;
+          // the stored values (one rounding each when S is narrower than T) are the state the
+          // running max and the next step's wave speed see
+          const S hS = (S)h, huS = (S)hu, hvS = (S)hv;
+          if (a.max_h && hS > a.max_h[idx]) a.max_h[idx] = hS;
+          qout[0][idx] = hS;
+          qout[1][idx] = huS;
+          qout[2][idx] = hvS;
           if (!(fl & GS_FLAG_CFL_SKIP)) {
-            const lam_t<T> lb_ = lam_key<T>(h, hu, hv, P);
+            const lam_t<T> lb_ = lam_key<T>((T)hS, (T)huS, (T)hvS, P);
             lam = lb_ > lam ? lb_ : lam;
           }
         }
@@ -308,10 +313,10 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T> a) {
 }
 
 // CFL wave speed of the current dense state, then plan dt (no time advance).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_dense_cfl(DenseK<T> a) {
+template <typename T, typename S = T>
+__global__ __launch_bounds__(kBlock) void k_dense_cfl(DenseK<T, S> a) {
   const int par = (int)(a.clock->nsteps & 1);
-  T* const* q = a.buf[par];
+  S* const* q = a.buf[par];
   lam_t<T> lam = 0;
   const int64_t n = (int64_t)a.nx * a.ny;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
@@ -319,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_cfl(DenseK<T> a) {
     const int64_t i = k / a.ny, j = k - i * a.ny;
     const int64_t idx = i * a.pitch + j;
     if (a.flags && (a.flags[idx] & GS_FLAG_CFL_SKIP)) continue;
-    const lam_t<T> l = lam_key<T>(q[0][idx], q[1][idx], q[2][idx], a.P);
+    const lam_t<T> l = lam_key<T>((T)q[0][idx], (T)q[1][idx], (T)q[2][idx], a.P);
     lam = l > lam ? l : lam;
   }
   grid_reduce_and_plan(lam_key_bits64(lam), a.clock, a.plan, false);
@@ -335,8 +340,8 @@ static int pick_tile_rows(int nx, int ncs, int req) {
   return 16;
 }
 
-template <typename T>
-static int build_dense(const GsDense* d, const GsPhys* p, const GsForcing& f, GsClock* clk, DenseK<T>& a,
+template <typename T, typename S = T>
+static int build_dense(const GsDense* d, const GsPhys* p, const GsForcing& f, GsClock* clk, DenseK<T, S>& a,
                        bool need_clock) {
   if (!d) return fail(GS_E_ARG, "dense descriptor is NULL");
   if (d->nx < 4 || d->ny < 4) return fail(GS_E_RANGE, "dense grid needs nx, ny >= 4 (got %d x %d)", d->nx, d->ny);
@@ -365,16 +370,16 @@ static int build_dense(const GsDense* d, const GsPhys* p, const GsForcing& f, Gs
   if ((int64_t)a.ncs * a.nrs > (int64_t)INT32_MAX / 2) return fail(GS_E_RANGE, "too many tiles");
   a.ntiles = a.ncs * a.nrs;
   for (int s = 0; s < 2; ++s)
-    for (int k = 0; k < 3; ++k) a.buf[s][k] = (T*)d->q[s][k];
-  a.bed = (const T*)d->bed;
+    for (int k = 0; k < 3; ++k) a.buf[s][k] = (S*)d->q[s][k];
+  a.bed = (const S*)d->bed;
   a.flags = d->flags;
-  a.ntab = (const T*)d->ntab;
+  a.ntab = (const S*)d->ntab;
   a.n_classes = d->ntab ? (p->n_classes > 0 ? p->n_classes : 1) : 0;
-  a.n_field = (const T*)d->n_field;
-  a.max_h = (T*)d->max_h;
+  a.n_field = (const S*)d->n_field;
+  a.max_h = (S*)d->max_h;
   a.P = make_phys<T>(*p, d->dx, d->dy);
   a.rain_mode = f.n_rain > 0 ? (f.rain_gridded ? 2 : 1) : 0;
-  a.rain_frames = (const T*)f.rain_frames;
+  a.rain_frames = (const S*)f.rain_frames;
   a.frame_stride = f.rain_frame_stride;
   a.rain_pitch = f.rain_row_pitch;
   a.stage = f.n_stage > 0;
@@ -383,24 +388,24 @@ static int build_dense(const GsDense* d, const GsPhys* p, const GsForcing& f, Gs
   return GS_OK;
 }
 
-template <typename T>
+template <typename T, typename S = T>
 static int dense_step_t(const GsDense* d, const GsPhys* p, GsForcing f, GsClock* clk, hipStream_t s) {
-  DenseK<T> a;
-  int rc = build_dense<T>(d, p, f, clk, a, true);
+  DenseK<T, S> a;
+  int rc = build_dense<T, S>(d, p, f, clk, a, true);
   if (rc) return rc;
   const unsigned blocks = (unsigned)((a.ntiles + 3) / 4);
-  hipLaunchKernelGGL((k_dense_sweep<T, 1>), dim3(blocks), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_dense_sweep<T, 1, S>), dim3(blocks), dim3(kBlock), 0, s, a);
   GS_HIP(hipGetLastError(), "k_dense_sweep launch");
   return GS_OK;
 }
 
-template <typename T>
+template <typename T, typename S = T>
 static int dense_residual_t(const GsDense* d, const GsPhys* p, int which, void* r0, void* r1, void* r2,
                             hipStream_t s) {
   GsForcing f;
   memset(&f, 0, sizeof f);
-  DenseK<T> a;
-  int rc = build_dense<T>(d, p, f, nullptr, a, false);
+  DenseK<T, S> a;
+  int rc = build_dense<T, S>(d, p, f, nullptr, a, false);
   if (rc) return rc;
   if (which < 0 || which > 1) return fail(GS_E_ARG, "which must be 0 or 1");
   if (!r0 || !r1 || !r2) return fail(GS_E_ARG, "residual planes are NULL");
@@ -410,20 +415,20 @@ static int dense_residual_t(const GsDense* d, const GsPhys* p, int which, void* 
   a.rhs[2] = (T*)r2;
   // MODE 0 writes qout = buf[which ^ 1] never; buf[par ^ 1] is unused there
   const unsigned blocks = (unsigned)((a.ntiles + 3) / 4);
-  hipLaunchKernelGGL((k_dense_sweep<T, 0>), dim3(blocks), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_dense_sweep<T, 0, S>), dim3(blocks), dim3(kBlock), 0, s, a);
   GS_HIP(hipGetLastError(), "k_dense_sweep(residual) launch");
   return GS_OK;
 }
 
-template <typename T>
+template <typename T, typename S = T>
 static int dense_cfl_t(const GsDense* d, const GsPhys* p, GsForcing f, GsClock* clk, hipStream_t s) {
-  DenseK<T> a;
-  int rc = build_dense<T>(d, p, f, clk, a, true);
+  DenseK<T, S> a;
+  int rc = build_dense<T, S>(d, p, f, clk, a, true);
   if (rc) return rc;
   const int64_t n = (int64_t)a.nx * a.ny;
   int64_t blocks = (n + kBlock - 1) / kBlock;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL((k_dense_cfl<T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_dense_cfl<T, S>), dim3((unsigned)blocks), dim3(kBlock), 0, s, a);
   GS_HIP(hipGetLastError(), "k_dense_cfl launch");
   return GS_OK;
 }
@@ -559,6 +564,7 @@ int gs_dense_step(const GsDense* d, const GsPhys* phys, GsForcing forc, GsClock*
   if (!d) return fail(GS_E_ARG, "dense descriptor is NULL");
   if (d->dtype == GS_F32) return dense_step_t<float>(d, phys, forc, clock_dev, (hipStream_t)stream);
   if (d->dtype == GS_F64) return dense_step_t<double>(d, phys, forc, clock_dev, (hipStream_t)stream);
+  if (d->dtype == GS_F32X64) return dense_step_t<double, float>(d, phys, forc, clock_dev, (hipStream_t)stream);
   return fail(GS_E_DTYPE, "unsupported dtype %d", d->dtype);
 }
 
@@ -567,6 +573,8 @@ int gs_dense_residual(const GsDense* d, const GsPhys* phys, int which, void* rhs
   if (!d) return fail(GS_E_ARG, "dense descriptor is NULL");
   if (d->dtype == GS_F32) return dense_residual_t<float>(d, phys, which, rhs0, rhs1, rhs2, (hipStream_t)stream);
   if (d->dtype == GS_F64) return dense_residual_t<double>(d, phys, which, rhs0, rhs1, rhs2, (hipStream_t)stream);
+  if (d->dtype == GS_F32X64)
+    return dense_residual_t<double, float>(d, phys, which, rhs0, rhs1, rhs2, (hipStream_t)stream);
   return fail(GS_E_DTYPE, "unsupported dtype %d", d->dtype);
 }
 
@@ -574,6 +582,7 @@ int gs_dense_cfl(const GsDense* d, const GsPhys* phys, GsForcing forc, GsClock* 
   if (!d) return fail(GS_E_ARG, "dense descriptor is NULL");
   if (d->dtype == GS_F32) return dense_cfl_t<float>(d, phys, forc, clock_dev, (hipStream_t)stream);
   if (d->dtype == GS_F64) return dense_cfl_t<double>(d, phys, forc, clock_dev, (hipStream_t)stream);
+  if (d->dtype == GS_F32X64) return dense_cfl_t<double, float>(d, phys, forc, clock_dev, (hipStream_t)stream);
   return fail(GS_E_DTYPE, "unsupported dtype %d", d->dtype);
 }
 

@@ -194,6 +194,8 @@ class Solver2D:
             raise ValueError(f"gridded rain must be (nt, nx, ny) = (nt, {mesh.nx}, {mesh.ny}); got "
                              f"{tuple(rf.rate_mm_h.shape)}")
         self.dtype = np.dtype(cfg.dtype)
+        # float32 planes stepped with float64 arithmetic (Config.compute_dtype="float64")
+        self.compute64 = self.dtype == np.float32 and cfg.compute_dtype == "float64"
         nx, ny = mesh.nx, mesh.ny
         self.pitch = _pitch(ny)
         self._plane = (nx, self.pitch)
@@ -260,7 +262,7 @@ class Solver2D:
         d = L.GsDense()
         d.nx, d.ny, d.pitch = self.mesh.nx, self.mesh.ny, self.pitch
         d.bc_x, d.bc_y = L.GS_BC[self.cfg.bc_x], L.GS_BC[self.cfg.bc_y]
-        d.dtype = L.GS_F32 if self.dtype == np.float32 else L.GS_F64
+        d.dtype = L.GS_F64 if self.dtype == np.float64 else (L.GS_F32X64 if self.compute64 else L.GS_F32)
         d.tile_rows = self._tile_rows
         d.dx, d.dy = float(self.mesh.dx), float(self.mesh.dy)
         for s in range(2):
@@ -598,7 +600,7 @@ class Solver2D:
     def residual(self) -> np.ndarray:
         """SRM-HLLC residual (3, nx, ny) of the current state (inactive cells 0)."""
         nx, ny = self.mesh.nx, self.mesh.ny
-        r = [L.DeviceArray(self._plane, self.dtype, zero=True) for _ in range(3)]
+        r = [L.DeviceArray(self._plane, np.float64 if self.compute64 else self.dtype, zero=True) for _ in range(3)]
         L.call("gs_dense_residual", C.byref(self._desc()), C.byref(self._phys()), self._cur(),
                C.c_void_p(r[0].ptr), C.c_void_p(r[1].ptr), C.c_void_p(r[2].ptr), self.stream)
         return np.stack([x.download()[:, :ny] for x in r])

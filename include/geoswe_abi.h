@@ -47,6 +47,10 @@ extern "C" {
 /* dtypes */
 #define GS_F32 0
 #define GS_F64 1
+/* dense layout only: float32 planes (state, bed, Manning, rain frames, running max) stepped
+ * with float64 arithmetic, every stored value rounded once to float32. No reference
+ * counterpart (the reference computes a float32 state in float32). */
+#define GS_F32X64 2
 
 /* per-cell flags byte (dense "flags" plane, flat "flags" array) */
 #define GS_FLAG_CLASS_MASK 0x1F  /* Manning class id (0..31) into the ntab array */
@@ -132,7 +136,7 @@ typedef struct GsDense {
   int32_t nx, ny;            /* interior cells */
   int64_t pitch;             /* elements between rows of every plane */
   int32_t bc_x, bc_y;        /* GS_BC_* */
-  int32_t dtype;             /* GS_F32 or GS_F64 */
+  int32_t dtype;             /* GS_F32, GS_F64 or GS_F32X64 */
   int32_t tile_rows;         /* rows swept per wavefront (0 = library default) */
   double dx, dy;
   void* q[2][3];             /* double-buffered state planes h, hu, hv */

@@ -801,9 +801,13 @@ class DenseOracle:
     -> stage -> infiltration -> sponge -> gauge ring (at the step's start time) -> Green-Ampt
     and drain-tau -> clamp -> karst drain -> (next) CFL. Works in float32 or float64."""
 
-    def __init__(self, case: DenseCase, ph: Physics, dtype=np.float32):
+    def __init__(self, case: DenseCase, ph: Physics, dtype=np.float32, store_dtype=None):
+        """``store_dtype`` narrower than ``dtype`` (float32 with float64): the state is kept in
+        ``store_dtype``, i.e. rounded to it after every step and read back exactly, and the
+        arithmetic is ``dtype`` (the library's float32-planes, float64-arithmetic mode)."""
         self.case, self.ph = case, ph
         self.dtype = np.dtype(dtype)
+        self.store = None if store_dtype is None else np.dtype(store_dtype)
         nx, ny = case.h.shape
         ngh = case.ngh
         self.nx, self.ny, self.ngh = nx, ny, ngh
@@ -880,6 +884,8 @@ class DenseOracle:
                 inf = (c.infil[0], c.infil[1], dt, ph.h_min, act)
             ga = None if c.ga is None else (c.ga, self.ga_F, dt, act)
             coastal_force(self.q[I], c.sponge, c.clamp, c.ring, t0, c.drain, inf, ga)
+        if self.store is not None:
+            self.q[I] = self.q[I].astype(self.store).astype(self.dtype)
         self.nsteps += 1
 
     def rain_max_all(self) -> float:

@@ -435,3 +435,15 @@ def test_native_mesh_builder_equals_numpy_builder():
             build_flat_mesh(bad, 4, native=native)
         with pytest.raises(ValueError):
             build_flat_mesh(masks[2][0], 4, ring=1, native=native)
+
+
+def test_config_compute_dtype_rule():
+    """float64 arithmetic is offered for a float32 state only."""
+    import pytest
+    from geoswe_amd.config import Config
+    assert Config(dtype="float32", compute_dtype="float64").compute_dtype == "float64"
+    assert Config(dtype="float64", compute_dtype="float64").compute_dtype == "float64"
+    for kw in (dict(dtype="float64", compute_dtype="float32"), dict(dtype="float32", compute_dtype="float16")):
+        with pytest.raises(ValueError, match="compute_dtype"):
+            Config(**kw)
+
