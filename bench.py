@@ -286,7 +286,7 @@ def coastal_leg(n: int, steps: int, sweep_rows=None, forcings: bool = False) -> 
                                f"gridded rain every 300 s, tidal stage on the sea edge, fall edges, CFL 0.5 (L-inf)",
                    "layout": "flat", "active_cells": int(nact), "stored_cells": int(fm.n_stored)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_flat_sweep<float>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_flat_sweep<float, false, true>",
                      "kernel_ms": ms_f, "bytes_per_cell": bpc},
         "dense_ms_per_step": ms_d,
         "dense_equals_flat": ok,
@@ -410,7 +410,7 @@ def ensemble_leg(members: int, steps: int, graph_batch: int) -> dict:
         "sequential_plain_s": w_plain, "ensemble_s": w_ens, "event_ms_plain": ev_plain, "event_ms_ensemble": ev_ens,
         "launch": tune, "members_bitwise_equal_plain": same, "distinct_member_dt": len(dts),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_ens_step<float>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_ens_step<float, 2>",
                      "bytes_per_cell": 33.0},
     }
     L.call("gs_event_destroy", e0)
@@ -832,7 +832,7 @@ def main():
         # against the device's measured float32 issue rate; achieved/peak/frac are the HBM figures
         "roofline": {"bound": "valu" if valu else "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "gs::k_dense_sweep<float, 1>" if args.layout == "dense" else "gs::k_flat_sweep<float>",
+                     "kernel": "gs::k_dense_sweep<float, 1>" if args.layout == "dense" else "gs::k_flat_sweep<float, false, true>",
                      "kernel_ms": kern_ms, "bytes_per_cell": BYTES_PER_CELL[args.layout], "valu": valu,
                      "copy_GBps": copy["GBps_read_plus_write"],
                      "frac_of_copy": achieved / copy["GBps_read_plus_write"]},
