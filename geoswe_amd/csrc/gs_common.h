@@ -154,8 +154,6 @@ __device__ static void plan_finish(GsClock* c, const Plan& p, uint64_t lam, bool
   plan_dt(c, p);
 }
 
-// Block reduction of a per-thread wave-speed (IEEE bits, order independent) and the
-// last-workgroup-done hand-off of the grid maximum to plan_finish.
 // Neighbour-lane values for the y-faces: lane l receives lane l+1's value (lane_upper)
 // or lane l-1's (lane_lower); the end lanes keep their own. The shuffle runs on the LDS
 // crossbar, beside the VALU work of the faces: DPP wavefront shifts (one VALU move per 32
@@ -171,6 +169,19 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
   return v;
 }
 
+// Use an atomic's returned value: the wave waits until the atomic has been performed, and
+// the compiler moves no memory access across this point.
+__device__ __forceinline__ void await_atomic(uint64_t v) { __asm__ volatile("" ::"v"(v) : "memory"); }
+
+// Block reduction of a per-thread wave-speed (IEEE bits, order independent) and the
+// last-workgroup-done hand-off of the grid maximum to plan_finish. Only atomics cross
+// workgroups here (the state a workgroup stores is read by the next launch), so the
+// hand-off orders atomics and nothing else: thread 0 folds the workgroup's maximum into
+// the clock and waits until that atomic has been performed before it counts the
+// workgroup done, so the count that reveals the last workgroup follows every maximum, and
+// the last workgroup reads the total with a device-scope atomic load. A release on the
+// count would also write back the XCD's L2 (its dirty state lines) once per workgroup,
+// which cost 10 % of the coastal step at one workgroup per four tiles.
 __device__ static void grid_reduce_and_plan(uint64_t lam, GsClock* c, const Plan& p, bool advance) {
   __shared__ uint64_t s_w[16];
   __shared__ int s_last;
@@ -182,9 +193,9 @@ __device__ static void grid_reduce_and_plan(uint64_t lam, GsClock* c, const Plan
     uint64_t m = 0ull;
     const int nw = (blockDim.x + 63) >> 6;
     for (int w = 0; w < nw; ++w) m = s_w[w] > m ? s_w[w] : m;
-    if (m) __hip_atomic_fetch_max(&c->lam_acc, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (m) await_atomic(__hip_atomic_fetch_max(&c->lam_acc, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     const unsigned int prev =
-        __hip_atomic_fetch_add(&c->blocks_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(&c->blocks_done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = (prev == gridDim.x - 1) ? 1 : 0;
   }
   __syncthreads();

@@ -1165,69 +1165,105 @@ struct EnsK {
   unsigned* done;                     // workgroups finished in this launch
 };
 
-// Fold a wavefront's CFL speed bits of member m into that member's clock.
+// Read a value that the launch never writes through the constant address space: a
+// uniform address then gives scalar loads. The host pass sees an ordinary read.
+template <typename U>
+__device__ __forceinline__ U const_read(const U* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(4))) U*)p;
+#else
+  return *p;
+#endif
+}
+
+// Fold a wavefront's CFL speed bits of member m into that member's clock (lane 0 waits
+// until its atomic has been performed, as grid_reduce_and_plan's thread 0 does).
 template <typename T>
 __device__ __forceinline__ void ens_flush(const EnsK<T>& e, int m, uint64_t lam) {
   lam = wave_max_u64(lam);
   if (m >= 0 && (threadIdx.x & 63) == 0 && lam)
-    __hip_atomic_fetch_max(&e.mem[m].clock->lam_acc, lam, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    await_atomic(__hip_atomic_fetch_max(&e.mem[m].clock->lam_acc, lam, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// The member that owns global tile gc (tile0: the members' tile prefix, n + 1 entries).
+__device__ __forceinline__ int ens_member_of(const int64_t* tile0, int n, int64_t gc) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (const_read(tile0 + mid) <= gc) lo = mid;
+    else hi = mid;
+  }
+  return lo;
 }
 
 // One step of every live member. The global tile space is the members' sweep tiles end to
-// end; a wavefront takes `tpp` consecutive tiles per pass and strides by the grid, so a
-// member boundary is crossed at most a few times per wavefront and the member view, its
-// clock values and the wave's speed maximum change only there. The last workgroup to
-// finish plans each member's next dt from its own maximum (plan_finish), as the plain
-// step's last workgroup does for its single domain.
+// end; a wavefront takes `tpp` consecutive tiles per pass and strides by the grid (the
+// autotuner usually picks one wavefront per tile, as the plain sweep launches), finding a
+// tile's member by bisection of the tile prefix and reading the member's view and clock
+// values from the member table with scalar loads. A wavefront folds its speed maximum into
+// a member's clock when it leaves the member; its last maximum goes through the
+// workgroup's thread 0, which folds the workgroup's maxima (one atomic per member) before
+// counting the workgroup done. The last workgroup to finish plans each member's next dt
+// from its own maximum (plan_finish), as the plain step's last workgroup does for its
+// single domain; the hand-off orders atomics only, as grid_reduce_and_plan's does.
 // KIND kEnsExt compiles the extended forcings (Green-Ampt, drain-tau, rain through a
-// per-cell lookup) in; kEnsPlain leaves them out and keeps the plain sweep's register
-// budget; kEnsLean also leaves out the per-cell Manning field, the running maximum and the
-// active-cell forcings (every member has none of them).
+// per-cell lookup) in; kEnsPlain leaves them out; kEnsLean also leaves out the per-cell
+// Manning field, the running maximum and the active-cell forcings (every member has none
+// of them), and its float32 build is held to 72 VGPRs (7 wavefronts per SIMD: 1.81 against
+// 1.93 ms per step on the 64-member case at 82 VGPRs, though 16 VGPRs then spill, a few of
+// them reloaded in the row loop).
 enum { kEnsPlain = 0, kEnsExt = 1, kEnsLean = 2 };
 
 template <typename T, int KIND>
-__global__ __launch_bounds__(kFBlock) void k_ens_step(EnsK<T> e, const FlatK<T>* __restrict__ mem,
-                                                      const int64_t* __restrict__ tile0) {
+__global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && KIND == kEnsLean ? 7 : 1)))
+void k_ens_step(EnsK<T> e, const FlatK<T>* __restrict__ mem, const int64_t* __restrict__ tile0) {
   const int lane = threadIdx.x & 63;
   const int64_t nwaves = (int64_t)gridDim.x * (kFBlock / 64);
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t wave = (int64_t)xcd_rank(blockIdx.x, gridDim.x) * (kFBlock / 64) + wv;
   const int64_t tpp = e.tpp;
-  int m = -1;
-  int64_t c0 = 0, c1 = 0;
+  int m = -1;          // the member whose maximum `lam` holds
   lam_t<T> lam = 0;
-  FlatClk<T> ck;
-  ck.live = false;
-  ck.par = 0;
-  ck.dt = ck.rain_u = ck.eta = T(0);
-  ck.frame = nullptr;
   for (int64_t base = wave * tpp; base < e.ntiles; base += nwaves * tpp) {
     const int64_t stop = base + tpp < e.ntiles ? base + tpp : e.ntiles;
     for (int64_t gc = base; gc < stop; ++gc) {
-      if (gc >= c1) {
+      // the tile's member, its view and clock values, looked up per tile so that none of
+      // them is held across tiles
+      const int mt = ens_member_of(tile0, e.n_members, gc);
+      if (mt != m) {
         ens_flush(e, m, lam_key_bits64(lam));
-        lam = 0ull;
-        do {
-          ++m;
-          c0 = tile0[m];
-          c1 = tile0[m + 1];
-        } while (gc >= c1);
-        ck = flat_clock<T, 1>(mem[m]);   // the member's clock values, once per switch
+        lam = 0;
+        m = mt;
       }
+      const FlatK<T> v = const_read(mem + mt);
+      const FlatClk<T> ck = flat_clock<T, 1>(v);
       if (!ck.live) continue;
-      // the member's layout, rebuilt per tile from the (scalar-loaded) member table so that
-      // it stays a register temporary
       FlatCtx<T> x;
-      flat_ctx<T, 1>(mem[m], ck, x);
-      flat_tile<T, false, KIND == kEnsExt, KIND == kEnsLean>(x, gc - c0, lane, lam);
+      flat_ctx<T, 1>(v, ck, x);
+      flat_tile<T, false, KIND == kEnsExt, KIND == kEnsLean>(x, gc - const_read(tile0 + mt), lane, lam);
     }
   }
-  ens_flush(e, m, lam_key_bits64(lam));
-  __threadfence();   // every wave's maxima are performed before the hand-off below
-  __syncthreads();
+  __shared__ int s_m[kFBlock / 64];
+  __shared__ uint64_t s_l[kFBlock / 64];
   __shared__ int s_last;
+  const uint64_t lw = wave_max_u64(lam_key_bits64(lam));
+  if (lane == 0) {
+    s_m[wv] = m;
+    s_l[wv] = lw;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    const unsigned prev = __hip_atomic_fetch_add(e.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    for (int w = 0; w < kFBlock / 64; ++w) {
+      const int mw = s_m[w];
+      const uint64_t l = s_l[w];
+      if (mw < 0 || !l) continue;
+      if (w + 1 < kFBlock / 64 && s_m[w + 1] == mw) {   // the next wave's member: fold into its slot
+        s_l[w + 1] = s_l[w + 1] > l ? s_l[w + 1] : l;
+        continue;
+      }
+      await_atomic(__hip_atomic_fetch_max(&e.mem[mw].clock->lam_acc, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    const unsigned prev = __hip_atomic_fetch_add(e.done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_last = (prev == gridDim.x - 1) ? 1 : 0;
   }
   __syncthreads();
