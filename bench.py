@@ -832,7 +832,7 @@ def main():
         # against the device's measured float32 issue rate; achieved/peak/frac are the HBM figures
         "roofline": {"bound": "valu" if valu else "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "gs::k_dense_sweep<float, 1>" if args.layout == "dense" else "gs::k_flat_sweep<float, false, true>",
+                     "kernel": "gs::k_dense_sweep<float, 1, float>" if args.layout == "dense" else "gs::k_flat_sweep<float, false, true>",
                      "kernel_ms": kern_ms, "bytes_per_cell": BYTES_PER_CELL[args.layout], "valu": valu,
                      "copy_GBps": copy["GBps_read_plus_write"],
                      "frac_of_copy": achieved / copy["GBps_read_plus_write"]},

@@ -12,6 +12,19 @@
 
 namespace gs {
 
+// A quotient that only the reverse sweep uses (no recomputed forward value and no branch
+// depends on it): the float32 device build takes the hardware reciprocal (1 ulp) times the
+// numerator; float64 and the host build divide. The recomputed forward values keep the
+// forward's correctly rounded operations (vdiv, vsqrt), so the reverse follows exactly the
+// branches the forward took.
+template <typename T>
+GS_HD T rdiv(T a, T b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (sizeof(T) == 4) return a * __builtin_amdgcn_rcpf(b);
+#endif
+  return a / b;
+}
+
 // Cotangents of the 10 face inputs.
 template <typename T>
 struct FaceGrad {
@@ -42,7 +55,7 @@ GS_HD void hllc_vjp(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min, T bFh,
   if (both) { SL = z; SR = z; }
   const T denom = hR * (uNR - SR) - hL * (uNL - SL);
   const bool big = vabs(denom) > T(1.0e-14);
-  const T SM = big ? ((SL * hR) * (uNR - SR) - (SR * hL) * (uNL - SL)) / denom : T(0.5) * (uNL + uNR);
+  const T SM = big ? vdiv((SL * hR) * (uNR - SR) - (SR * hL) * (uNL - SL), denom) : T(0.5) * (uNL + uNR);
   int br;   // 0 left flux, 1 right flux, 2 left star, 3 right star
   if (SL >= z) br = 0;
   else if (!(SM >= z) && !(SR >= z)) br = 1;
@@ -70,7 +83,7 @@ GS_HD void hllc_vjp(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min, T bFh,
   const T hh = lft ? hL : hR;
   const T huN = hh * uN;
   const T den = (S - SM) + T(1.0e-30);
-  const T fac = one ? z : (S - uN) / den;
+  const T fac = one ? z : rdiv(S - uN, den);
   const T hs = hh * fac;
   // Fh = huN + S (hs - hh); FN = F_N + S (hs SM - huN); FT = huN uT + S (hs uT - hh uT)
   T gS = bFh * (hs - hh) + bFN * (hs * SM - huN) + bFT * (hs * uT - hh * uT);
@@ -82,8 +95,8 @@ GS_HD void hllc_vjp(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min, T bFh,
   const T gF_N = bFN;
   ghh += ghs * fac;
   const T gfac = ghs * hh;
-  const T gnum = one ? z : gfac / den;
-  const T gden = one ? z : -(gfac * fac) / den;
+  const T gnum = one ? z : rdiv(gfac, den);
+  const T gden = one ? z : rdiv(-(gfac * fac), den);
   gS += gnum + gden;
   T guN = -gnum;
   ghuN += gF_N * uN;
@@ -101,8 +114,8 @@ GS_HD void hllc_vjp(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min, T bFh,
   T gSL = sl ? gS : z, gSR = sr ? gS : z;
   const T gSM = one ? z : gSMs - gden;
   // SM = Nn / denom (|denom| > 1e-14) or 0.5 (uNL + uNR)
-  const T gNn = big ? gSM / denom : z;
-  const T gdn = big ? -(gSM * SM) / denom : z;
+  const T gNn = big ? rdiv(gSM, denom) : z;
+  const T gdn = big ? rdiv(-(gSM * SM), denom) : z;
   gSL += gNn * (hR * (uNR - SR) + SR * hL) + gdn * hL;
   gSR += gNn * (-(SL * hR) - hL * (uNL - SL)) - gdn * hR;
   ghR += gNn * (SL * (uNR - SR)) + gdn * (uNR - SR);
@@ -130,8 +143,8 @@ GS_HD void hllc_vjp(T hL, T uNL, T uTL, T hR, T uNR, T uTR, T g, T h_min, T bFh,
   guNR += T(0.5) * gust;
   gaL += gust;
   gaR -= gust;
-  if (hL > z && aL > z) ghL += gaL * (g / (T(2) * aL));
-  if (hR > z && aR > z) ghR += gaR * (g / (T(2) * aR));
+  if (hL > z && aL > z) ghL += gaL * rdiv(g, T(2) * aL);
+  if (hR > z && aR > z) ghR += gaR * rdiv(g, T(2) * aR);
 }
 
 // Reverse of face(): cotangents of (Fh, FN, FT, srcL, srcR) -> cotangents of the 10
@@ -247,10 +260,10 @@ GS_HD void face_vjp(T bL, T bR, T gzL, T gzR, T hL, T hR, T uNL, T uTL, T uNR, T
 template <typename T>
 GS_HD void prims_vjp(T h, T hu, T hv, T h_min, T gu, T gv, T& gh, T& ghu, T& ghv) {
   if (!(h > h_min)) return;
-  const T u = hu / h, v = hv / h;
-  ghu += gu / h;
-  ghv += gv / h;
-  gh -= (gu * u + gv * v) / h;
+  const T u = vdiv(hu, h), v = vdiv(hv, h);
+  ghu += rdiv(gu, h);
+  ghv += rdiv(gv, h);
+  gh -= rdiv(gu * u + gv * v, h);
 }
 
 // Reverse of update() from its pre-friction values h1 = q0 + dt*rr0, hu1, hv1:
@@ -275,15 +288,15 @@ GS_HD void update_vjp(T h1, T hu1, T hv1, T dt, T n, const Phys<T>& P, T bh, T b
     return;
   }
   const T hs = (h1 > P.h_min) ? h1 : P.h_min;
-  const T u = hu1 / hs;
-  const T v = hv1 / hs;
+  const T u = vdiv(hu1, hs);
+  const T v = vdiv(hv1, hs);
   const T modU = vsqrt(u * u + v * v);
   const T h43 = pow_m43(hs);
   T ne = n, n_cri = z, D = z;
   bool capped = false;
   if (P.has_vcap && modU > P.vcap) {
     D = (((T(1.0e-10) + dt) * P.g) * h43) * (modU + T(1.0e-30));
-    n_cri = vsqrt(T(1) / D);
+    n_cri = vsqrt(vdiv(T(1), D));
     if (n_cri > n) { ne = n_cri; capped = true; }
   }
   const T Cf = ((P.g * ne) * ne) * h43;
@@ -291,9 +304,9 @@ GS_HD void update_vjp(T h1, T hu1, T hv1, T dt, T n, const Phys<T>& P, T bh, T b
   if (P.quadratic) {
     const T two = ((T(2) * dt) * Cf) * modU;
     s = vsqrt(T(1) + T(2) * two);
-    alpha = T(2) / (s + T(1));
+    alpha = vdiv(T(2), s + T(1));
   } else {
-    alpha = T(1) / (T(1) + ((dt * Cf) * modU));
+    alpha = vdiv(T(1), T(1) + ((dt * Cf) * modU));
   }
   // hu = hu1 alpha, hv = hv1 alpha, h = h1
   const T galpha = bhu * hu1 + bhv * hv1;
@@ -303,7 +316,7 @@ GS_HD void update_vjp(T h1, T hu1, T hv1, T dt, T n, const Phys<T>& P, T bh, T b
   T gCf, gmodU = z;
   if (P.quadratic) {
     const T gs = galpha * (-(alpha * alpha) * T(0.5));   // d(2/(s+1))/ds = -alpha^2/2
-    const T gtwo = gs / s;                               // ds/dtwo = 1/s
+    const T gtwo = rdiv(gs, s);                          // ds/dtwo = 1/s
     gCf = gtwo * ((T(2) * dt) * modU);
     gmodU += gtwo * ((T(2) * dt) * Cf);
   } else {
@@ -314,18 +327,18 @@ GS_HD void update_vjp(T h1, T hu1, T hv1, T dt, T n, const Phys<T>& P, T bh, T b
   const T gne = gCf * ((T(2) * P.g) * ne * h43);
   T gh43 = gCf * ((P.g * ne) * ne);
   if (capped) {
-    const T gD = gne * (-(T(0.5) * n_cri) / D);
+    const T gD = gne * rdiv(-(T(0.5) * n_cri), D);
     gh43 += gD * (((T(1.0e-10) + dt) * P.g) * (modU + T(1.0e-30)));
     gmodU += gD * (((T(1.0e-10) + dt) * P.g) * h43);
   } else {
     gn = gne;
   }
-  T ghs = gh43 * ((T(-4) / T(3)) * h43 / hs);
+  T ghs = gh43 * rdiv((T(-4) / T(3)) * h43, hs);
   if (modU > z) {
-    const T gu = gmodU * (u / modU), gv = gmodU * (v / modU);
-    gu1 += gu / hs;
-    gv1 += gv / hs;
-    ghs -= (gu * u + gv * v) / hs;
+    const T gu = gmodU * rdiv(u, modU), gv = gmodU * rdiv(v, modU);
+    gu1 += rdiv(gu, hs);
+    gv1 += rdiv(gv, hs);
+    ghs -= rdiv(gu * u + gv * v, hs);
   }
   if (h1 > P.h_min) g1 += ghs;
 }

@@ -174,15 +174,21 @@ __device__ __forceinline__ void face_slot(const AdjK<T>& a, int axis, int64_t L,
   s[9 * n] = G.gzR;
 }
 
+// The faces an active cell owns: its upper face along each axis, and its lower face when
+// the cell below is not active (an active cell below owns that face). One face per trip of
+// a loop that is not unrolled, so the reverse of a face is compiled once.
 template <typename T>
 __global__ __launch_bounds__(kABlock) void k_adj_faces(AdjK<T> a) {
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * blockDim.x) {
     if (!a.active[c]) continue;
     const int32_t* nb = a.nbr + 4 * c;
-    if (nb[0] >= 0) face_slot<T>(a, 0, c, nb[0], a.xf);
-    if (nb[1] >= 0 && !a.active[nb[1]]) face_slot<T>(a, 0, nb[1], c, a.xf);
-    if (nb[2] >= 0) face_slot<T>(a, 1, c, nb[2], a.yf);
-    if (nb[3] >= 0 && !a.active[nb[3]]) face_slot<T>(a, 1, nb[3], c, a.yf);
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {   // f: upper x, lower x, upper y, lower y
+      const int64_t o = nb[f];
+      if (o < 0 || ((f & 1) && a.active[o])) continue;
+      const int axis = f >> 1;
+      face_slot<T>(a, axis, (f & 1) ? o : c, (f & 1) ? c : o, axis == 0 ? a.xf : a.yf);
+    }
   }
 }
 
