@@ -90,9 +90,16 @@ __global__ __launch_bounds__(kABlock) void k_adj_update(AdjK<T> a) {
       }
     }
     if (a.P.friction) {
-      for (int k = 0; k < a.n_classes; ++k) {   // uniform trip count: every lane shuffles
-        const double v = wave_sum_f64(cls == k ? (double)gn : 0.0);
+      // one butterfly per class present in the wavefront (classes are spatially coherent:
+      // usually one or two), each over all 64 lanes with zeros for the other lanes, so a
+      // class's sum has the bits the loop over every class gave
+      uint64_t pending = __ballot(cls >= 0);
+      while (pending) {   // uniform: every lane shuffles
+        const int k = __shfl(cls, __ffsll((unsigned long long)pending) - 1, 64);
+        const bool mine = cls == k;
+        const double v = wave_sum_f64(mine ? (double)gn : 0.0);
         if ((threadIdx.x & 63) == 0) s_acc[wv][k] += v;
+        pending &= ~__ballot(mine);
       }
     }
   }
