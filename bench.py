@@ -121,6 +121,31 @@ def mixed_timing(nx, ny, bed, steps, warmup):
     return out
 
 
+def flat_layout_leg(nx, ny, bed, steps, warmup, dense_depth=None) -> dict:
+    """The same rain-on-slope case in the flat layout (every cell active, row-sweep tiles):
+    wall and HIP-event time over `steps` steps after `warmup`, and the depth after them
+    against the dense run's depth after as many steps (bit for bit)."""
+    from geoswe_amd.cases import rain_on_slope_c2
+    from geoswe_amd.compressed import CompressedSolver
+    s = rain_on_slope_c2(nx, ny, "float32", bed=bed)
+    s = CompressedSolver.from_dense(s, cfl_linf=True, track_max_depth=False)
+    s.advance(warmup)
+    wall, ev_ms = timed_steps(s, steps)
+    cells = nx * ny
+    kern_ms = ev_ms / steps
+    achieved = BYTES_PER_CELL["flat"] * cells / (kern_ms * 1e-3) / 1e9
+    out = {"value": cells * steps / wall, "unit": "cell-updates/s", "steps": steps, "ms_per_step": 1e3 * wall / steps,
+           "kernel_ms": kern_ms, "tiles": int(len(s.fm.tile_e0)),
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS, "kernel": "gs::k_flat_sweep<float, false, true>",
+                        "bytes_per_cell": BYTES_PER_CELL["flat"]}}
+    if dense_depth is not None:
+        out["equals_dense"] = bool(np.array_equal(s.depth(), dense_depth))
+    log(f"flat layout: {out['ms_per_step']:.3f} ms/step, {out['value'] / 1e9:.2f} G cell-updates/s"
+        + (f", depth equal to the dense run: {out['equals_dense']}" if dense_depth is not None else ""))
+    return out
+
+
 def cpu_baseline(bed_spec, crop=(1024, 1000), steps=80):
     """The NumPy oracle (float32, one thread) on a crop of the same workload."""
     from oracle import generators as GN
@@ -802,6 +827,10 @@ def main():
         f"{achieved:.0f} GB/s algorithmic; copy kernel {copy['GBps_read_plus_write']:.0f} GB/s")
     err64 = None
     mixed = None
+    flat = None
+    if not args.no_legs and args.layout == "dense":
+        log("flat layout leg: the same case, every cell active")
+        flat = flat_layout_leg(nx, ny, bed, args.steps, args.warmup, dense_depth=s.depth())
     if not args.no_fp64 and args.layout == "dense":
         del s
         mixed = mixed_timing(nx, ny, bed, args.steps, args.warmup)
@@ -841,6 +870,7 @@ def main():
         # the same step with float32 planes and float64 arithmetic (Config.compute_dtype):
         # the configuration meeting the float64 depth tolerance, timed over the same steps
         "float64_arithmetic": mixed,
+        "flat_layout": flat,
         "cpu_baseline": cpu,
     }
     if not args.no_legs and (nx, ny) == (25600, 25000) and args.layout == "dense":
@@ -865,7 +895,7 @@ def main():
                                             "adjoint_ms_per_step", "plain_forward_ms_per_step",
                                             "adjoint_over_plain_forward", "forward_cell_updates_per_s", "grad_n",
                                             "fd_n_eps1e-3", "fd_n_eps1e-4", "fd_vs_adjoint_n", "fd_vs_adjoint_bed",
-                                            "torch_f64_autograd") if k in a}
+                                            "fd64_500_steps", "torch_f64_autograd") if k in a}
     print(json.dumps(out), flush=True)
 
 
