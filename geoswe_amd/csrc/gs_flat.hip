@@ -1010,8 +1010,9 @@ static unsigned flat_grid(int64_t nchunks) {
 // the resident grid more than four times: the hardware then keeps every SIMD fed until the
 // last tile, where a resident grid striding over the tiles left a partial last round (and
 // measured 10-14 % slower on the coastal case, 4.9 rounds, and the full 640 M-cell grid,
-// 23). Fewer tiles run on the resident grid, whose smaller launch measured 8 % faster on a
-// 2048^2 domain (1.3 rounds) and kept the mid-size ensemble members' plain runs faster.
+// 23). Fewer tiles run on the resident grid: one wavefront per tile from one resident grid
+// up measured 9 % faster on a 2048^2 domain (1.3 rounds) but 4.5 % slower summed over the
+// 64 ensemble members' plain runs (mostly shorter tiles of smaller domains).
 template <typename K>
 static unsigned sweep_grid(K kern, std::atomic<int>* cache, int64_t ntiles) {
   const unsigned resident = resident_grid(kern, cache, ntiles);
