@@ -20,7 +20,8 @@ GS_BC = {"extrapolate": 0, "periodic": 1, "wall": 2, "fall": 3}
 GS_F32, GS_F64, GS_F32X64 = 0, 1, 2
 FLAG_CLASS_MASK, FLAG_CFL_SKIP, FLAG_INACTIVE, FLAG_STAGE = 0x1F, 0x20, 0x40, 0x80
 FLAG_FORCE = 0x40          # flat layout: sponge / clamp cell (the dense layout's inactive bit)
-MAX_CLASSES = 32
+MAX_CLASSES = 32        # class ids the flags byte holds
+MAX_CLASSES_PLANE = 256  # class ids a class plane holds (GsDense.cls, GsFlat.cls)
 ST_DONE, ST_NONFINITE = 1, 2
 ADJ_MAX_BLOCKS = 8192
 
@@ -53,7 +54,7 @@ class GsDense(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("pitch", C.c_int64), ("bc_x", C.c_int32),
                 ("bc_y", C.c_int32), ("dtype", C.c_int32), ("tile_rows", C.c_int32), ("dx", C.c_double),
                 ("dy", C.c_double), ("q", (C.c_void_p * 3) * 2), ("bed", C.c_void_p), ("flags", C.c_void_p),
-                ("ntab", C.c_void_p), ("n_field", C.c_void_p), ("max_h", C.c_void_p)]
+                ("ntab", C.c_void_p), ("n_field", C.c_void_p), ("max_h", C.c_void_p), ("cls", C.c_void_p)]
 
 
 class GsFlat(C.Structure):
@@ -70,7 +71,7 @@ class GsFlat(C.Structure):
                 ("ring_mode_nb", C.c_void_p), ("ring_mode_kind", C.c_void_p), ("ring_mode_eta", C.c_double),
                 ("n_force_ring", C.c_int64), ("force_ring", C.c_void_p), ("force_ring_row", C.c_void_p),
                 ("ring_ng", C.c_int32), ("ring_nt", C.c_int32), ("ring_bed", C.c_void_p), ("ring_w", C.c_void_p),
-                ("ring_t", C.c_void_p), ("ring_stage", C.c_void_p)]
+                ("ring_t", C.c_void_p), ("ring_stage", C.c_void_p), ("cls", C.c_void_p)]
 
 
 class GsFlatMeshHost(C.Structure):

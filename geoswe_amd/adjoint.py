@@ -64,6 +64,9 @@ class GaugeMisfit:
 
     def __init__(self, solver: CompressedSolver, gauges: np.ndarray, dts: np.ndarray,
                  h_star: Optional[np.ndarray] = None, checkpoint_every: int = 100):
+        if solver._cls is not None:
+            raise NotImplementedError("GaugeMisfit: the reverse kernels read the Manning class from the flags "
+                                      "byte (at most 32 classes); this solver has a class plane")
         if solver.has_coastal_forcing:
             raise NotImplementedError("GaugeMisfit: the coastal forcings (sponge, gauge ring, clamp, infiltration, "
                                       "drain) have no reverse kernels; build the misfit on a solver without them")
@@ -111,7 +114,7 @@ class GaugeMisfit:
     def set_params(self, ntab=None, bed=None) -> None:
         """Manning n per class (n_classes,) and/or the flat bed (N,), host arrays."""
         if ntab is not None:
-            t = np.zeros(L.MAX_CLASSES, self.dtype)
+            t = np.zeros(L.MAX_CLASSES_PLANE, self.dtype)
             nt = np.asarray(ntab, np.float64).ravel()
             if len(nt) != self.n_classes:
                 raise ValueError(f"ntab has {len(nt)} entries, the solver {self.n_classes} classes")

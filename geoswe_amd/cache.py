@@ -78,7 +78,8 @@ def save_cache(solver, cache_dir: str, x0: float = 0.0, y0: float = 0.0, crs_wkt
     for k in range(3):
         sv(f"q{k}", q[k].astype(np.float32))
     sv("bed_f", s._bed.download().astype(np.float32))
-    sv("mcls_f", (s._flags_base & np.uint8(L.FLAG_CLASS_MASK)).astype(np.uint8))
+    sv("mcls_f", s._cls.download().astype(np.uint8) if s._cls is not None
+       else (s._flags_base & np.uint8(L.FLAG_CLASS_MASK)).astype(np.uint8))
     sv("m_tab", s._ntab.download()[:s._n_classes].astype(np.float32))
     rt = rain_tables(s.cfg.rainfall_forcing, s.cfg.rainfall, s.cfg.cfl, s.cfg.g, min(s.mesh.dx, s.mesh.dy),
                      np.float32)
@@ -176,8 +177,13 @@ def from_cache(cache_dir: str, cfl: float = 0.5, h_min: float = 1e-6, g: float =
     cfg = Config(bc_x="fall", bc_y="fall", cfl=cfl, g=g, h_min=h_min, dtype="float32", friction="manning")
     q = np.stack([ld(f"q{k}").astype(np.float32) for k in range(3)])
     m_tab = ld("m_tab").astype(np.float64)
-    s = CompressedSolver(mesh, cfg, fm, fm.unpack(q), fm.unpack(ld("bed_f").astype(np.float32)),
-                         fm.unpack(ld("mcls_f").astype(np.uint8)), m_tab, None, cfl_linf, track_max_depth)
+    mcls = fm.unpack(ld("mcls_f").astype(np.uint8))
+    if len(m_tab) > L.MAX_CLASSES:     # more classes than the flags byte holds: a class plane
+        s = CompressedSolver(mesh, cfg, fm, fm.unpack(q), fm.unpack(ld("bed_f").astype(np.float32)),
+                             np.zeros_like(mcls), m_tab, None, cfl_linf, track_max_depth, cls_pad=mcls)
+    else:
+        s = CompressedSolver(mesh, cfg, fm, fm.unpack(q), fm.unpack(ld("bed_f").astype(np.float32)),
+                             mcls, m_tab, None, cfl_linf, track_max_depth)
     if rain is not None:
         s.set_rain(rain)
         if not rain_cap:

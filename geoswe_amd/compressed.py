@@ -29,7 +29,7 @@ class CompressedSolver:
 
     def __init__(self, mesh, cfg: Config, fm, q_pad: np.ndarray, b_pad: np.ndarray, flags_pad: np.ndarray,
                  ntab: np.ndarray, n_field_pad: Optional[np.ndarray], cfl_linf: bool, track_max_depth: bool = True,
-                 h_min_cfl: Optional[float] = None):
+                 h_min_cfl: Optional[float] = None, cls_pad: Optional[np.ndarray] = None):
         cfg.require_production_scheme("CompressedSolver")
         self.mesh, self.cfg, self.fm = mesh, cfg, fm
         self.dtype = np.dtype(cfg.dtype)
@@ -58,12 +58,15 @@ class CompressedSolver:
         self._rain_lookup = None
         self._rmode = None
         self._tables_taken = False         # an Ensemble or GaugeMisfit holds this solver's device pointers
-        t = np.zeros(L.MAX_CLASSES, self.dtype)
+        t = np.zeros(L.MAX_CLASSES_PLANE, self.dtype)
         t[:len(ntab)] = ntab
         self._ntab = L.DeviceArray.from_host(t)
         self._n_classes = max(1, len(ntab))
         self._n_field = (L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(n_field_pad).astype(self.dtype)))
                          if n_field_pad is not None else None)
+        # Manning classes 0..255 in a plane of their own (tables of more than 32 classes)
+        self._cls = (L.DeviceArray.from_host(np.ascontiguousarray(fm.pack(cls_pad).astype(np.uint8)))
+                     if cls_pad is not None else None)
         self._max_h = L.DeviceArray((N,), self.dtype, zero=True) if track_max_depth else None
         self._runs = [L.DeviceArray.from_host(a) for a in (fm.run_k0, fm.run_len, fm.run_i, fm.run_j0,
                                                             fm.run_d.reshape(-1), fm.chunk_run)]
@@ -106,7 +109,11 @@ class CompressedSolver:
         if s._n_field is not None:
             nf = np.zeros((nxp, nyp), s.dtype)
             nf[g:g + nx, g:g + ny] = s._n_field.download()[:, :ny]
-        out = cls(mesh, cfg, fm, q_pad, b_pad, flags, ntab, nf, cfl_linf, track_max_depth, h_min_cfl)
+        cp = None
+        if s._cls is not None:
+            cp = np.zeros((nxp, nyp), np.uint8)
+            cp[g:g + nx, g:g + ny] = s._cls.download()[:, :ny]
+        out = cls(mesh, cfg, fm, q_pad, b_pad, flags, ntab, nf, cfl_linf, track_max_depth, h_min_cfl, cls_pad=cp)
         c = s._clockv()
         if c.t != 0.0:
             L.call("gs_clock_reset", C.c_void_p(out._clock.ptr), C.c_double(float(c.t)), None)
@@ -267,9 +274,10 @@ class CompressedSolver:
             tab = np.asarray(infil["tab"], np.float64).ravel()
             if self._n_field is not None:
                 raise ValueError("set_infil: the solver has a per-cell Manning n plane and no class per cell")
-            if len(tab) > L.MAX_CLASSES:
-                raise ValueError(f"set_infil: {len(tab)} rates for at most {L.MAX_CLASSES} classes")
-            t = np.zeros(L.MAX_CLASSES, self.dtype)
+            cap = L.MAX_CLASSES_PLANE if self._cls is not None else L.MAX_CLASSES
+            if len(tab) > cap:
+                raise ValueError(f"set_infil: {len(tab)} rates for at most {cap} classes")
+            t = np.zeros(L.MAX_CLASSES_PLANE, self.dtype)
             t[:len(tab)] = tab
             new = L.DeviceArray.from_host(t)
         self._infil = new
@@ -467,6 +475,7 @@ class CompressedSolver:
         f.flags = self._flags.ptr if self._flags is not None else None
         f.n_field = self._n_field.ptr if self._n_field is not None else None
         f.max_h = self._max_h.ptr if self._max_h is not None else None
+        f.cls = self._cls.ptr if self._cls is not None else None
         f.run_k0, f.run_len, f.run_i, f.run_j0, f.run_d, f.chunk_run = (r.ptr for r in self._runs)
         if self.traversal not in ("tiles", "chunks"):
             raise ValueError(f"traversal must be 'tiles' or 'chunks', got {self.traversal!r}")

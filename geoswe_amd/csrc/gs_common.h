@@ -232,8 +232,9 @@ static inline int check_phys(const GsPhys* p) {
   if (!p) return fail(GS_E_ARG, "phys is NULL");
   if (!(p->g > 0.0) || !(p->h_min > 0.0) || !(p->cfl > 0.0) || !(p->dmin > 0.0))
     return fail(GS_E_ARG, "phys: g, h_min, cfl and dmin must be positive");
-  if (p->friction && (p->n_classes < 1 || p->n_classes > 32))
-    return fail(GS_E_ARG, "phys: n_classes must be in [1, 32] with friction on (got %d)", p->n_classes);
+  if (p->friction && (p->n_classes < 1 || p->n_classes > GS_MAX_CLASSES))
+    return fail(GS_E_ARG, "phys: n_classes must be in [1, %d] with friction on (got %d)", GS_MAX_CLASSES,
+                p->n_classes);
   return GS_OK;
 }
 

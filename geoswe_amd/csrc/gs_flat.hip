@@ -37,6 +37,7 @@ struct FlatK {
   int n_classes;
   const T* n_field;
   T* max_h;
+  const uint8_t* cls;   // per-cell class plane (ids 0..255), or NULL: the flags' class bits
   T* rhs[3];
   int which;
   const int64_t* run_k0;
@@ -103,6 +104,7 @@ struct FlatCtx {
   const GS_G uint8_t* flags;
   const GS_G T* n_field;
   GS_G T* max_h;
+  const GS_G uint8_t* cls;
   GS_G T* rhs0;
   GS_G T* rhs1;
   GS_G T* rhs2;
@@ -200,6 +202,7 @@ __device__ __forceinline__ void flat_ctx(const FlatK<T>& a, const FlatClk<T>& k,
   x.bed = gp(a.bed);
   x.flags = gp(a.flags);
   x.n_field = gp(a.n_field);
+  x.cls = gp(a.cls);
   x.max_h = gp(a.max_h);
   x.rhs0 = gp(a.rhs[0]);
   x.rhs1 = gp(a.rhs[1]);
@@ -395,7 +398,8 @@ __device__ __forceinline__ void ga_cell(const FlatCtx<T>& x, int64_t base, uint3
 template <typename T, bool GA>
 __device__ __forceinline__ void active_force(const FlatCtx<T>& x, uint8_t fg, int64_t base, uint32_t off,
                                              uint32_t ob, T& h, T& hu, T& hv) {
-  if (x.infil) infil_cell<T>(x.infil[fg & GS_FLAG_CLASS_MASK], x.dt, x.P.h_min, h, hu, hv);
+  if (x.infil)
+    infil_cell<T>(x.infil[x.cls ? gat(x.cls + base, ob) : (fg & GS_FLAG_CLASS_MASK)], x.dt, x.P.h_min, h, hu, hv);
   const bool fc = (fg & GS_FLAG_FORCE) != 0;
   if (fc && x.sponge)
     sponge_cell<T>(gat(x.sponge + 2 * base, 2 * off), gat(x.sponge + 2 * base + 1, 2 * off), h, hu, hv);
@@ -497,7 +501,7 @@ __device__ __forceinline__ void flat_chunk(const FlatCtx<T>& x, int64_t chunk, i
     a.pre2[k] = hvC + x.dt * rhv;
   }
   T n = T(0);
-  if (P.friction) n = a.n_field ? a.n_field[k] : x.ntab[fg & GS_FLAG_CLASS_MASK];
+  if (P.friction) n = a.n_field ? a.n_field[k] : x.ntab[a.cls ? a.cls[k] : (fg & GS_FLAG_CLASS_MASK)];
   T h, hu, hv;
   update<T>(hC, huC, hvC, rr0, rhu, rhv, x.dt, n, P, h, hu, hv);
   if (fg & GS_FLAG_STAGE) {
@@ -560,8 +564,9 @@ __device__ __forceinline__ T sweep_b(const FlatCtx<T>& x, int64_t base, uint32_t
 // flat_chunk), so all three traversals give the same bits. Every lane of the wavefront
 // must call it. PRE: also write the pre-friction state (the adjoint's recomputation); a
 // separate instantiation keeps those pointers out of the plain step's scalar registers.
-// LEAN: the domain has no per-cell Manning field, no running maximum and no active-cell
-// forcing, so none of their pointers is held (the host checks that they are all null).
+// LEAN: the domain has no per-cell Manning field or class plane, no running maximum and no
+// active-cell forcing, so none of their pointers is held (the host checks that they are all
+// null).
 template <typename T, bool PRE, bool EXT, bool LEAN = false>
 __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int lane, lam_t<T>& lam) {
   const Phys<T>& P = x.P;
@@ -660,7 +665,9 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
         gat(x.pre2 + gC, oC) = hvC + x.dt * rhv;
       }
       T n = T(0);
-      if (P.friction) n = (!LEAN && x.n_field) ? gat(x.n_field + gC, oC) : x.ntab[fg & GS_FLAG_CLASS_MASK];
+      if (P.friction)
+        n = (!LEAN && x.n_field) ? gat(x.n_field + gC, oC)
+                                 : x.ntab[(!LEAN && x.cls) ? gat(x.cls + gC, ob) : (fg & GS_FLAG_CLASS_MASK)];
       T h, hu, hv;
       update<T>(hC, huC, hvC, rr0, rhu, rhv, x.dt, n, P, h, hu, hv);
       if (fg & GS_FLAG_STAGE) {
@@ -705,8 +712,9 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
   if (!ck.live) return;   // the whole grid sees the same clock
   FlatCtx<T> x;
   flat_ctx<T, MODE>(a, ck, x);
-  __shared__ T s_ntab[32];
-  if (MODE == 1 && threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
+  __shared__ T s_ntab[GS_MAX_CLASSES];
+  if (MODE == 1)
+    for (int k = (int)threadIdx.x; k < a.n_classes; k += (int)blockDim.x) s_ntab[k] = a.ntab[k];
   __syncthreads();
   x.ntab = s_ntab;
   const int lane = threadIdx.x & 63;
@@ -722,7 +730,7 @@ __global__ __launch_bounds__(kFBlock) void k_flat_step(FlatK<T> a) {
 // order and XCD-major wavefront order (the loop also serves a grid smaller than that). The float32 plain
 // step is held to 72 VGPRs (7 wavefronts per SIMD instead of 6; a few scalar registers are
 // then reloaded in the row loop), which measured 2-3 % faster. LEAN (a domain with no
-// per-cell Manning field, running maximum or active-cell forcing) drops those pointers:
+// per-cell Manning field or class plane, running maximum or active-cell forcing) drops those pointers:
 // 34 scalar registers spilled to lanes instead of 75, 4 % faster on the coastal case.
 template <typename T, bool PRE, bool LEAN = false>
 __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !PRE ? 7 : 1))) void k_flat_sweep(FlatK<T> a) {
@@ -730,8 +738,8 @@ __global__ __launch_bounds__(kFBlock) __attribute__((amdgpu_waves_per_eu(sizeof(
   if (!ck.live) return;   // the whole grid sees the same clock
   FlatCtx<T> x;
   flat_ctx<T, 1>(a, ck, x);
-  __shared__ T s_ntab[32];
-  if (threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
+  __shared__ T s_ntab[GS_MAX_CLASSES];
+  for (int k = (int)threadIdx.x; k < a.n_classes; k += (int)blockDim.x) s_ntab[k] = a.ntab[k];
   __syncthreads();
   x.ntab = s_ntab;
   const int lane = threadIdx.x & 63;
@@ -754,8 +762,8 @@ __global__ __launch_bounds__(kFBlock) void k_flat_sweep_ext(FlatK<T> a) {
   if (!ck.live) return;
   FlatCtx<T> x;
   flat_ctx<T, 1>(a, ck, x);
-  __shared__ T s_ntab[32];
-  if (threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = a.ntab[threadIdx.x];
+  __shared__ T s_ntab[GS_MAX_CLASSES];
+  for (int k = (int)threadIdx.x; k < a.n_classes; k += (int)blockDim.x) s_ntab[k] = a.ntab[k];
   __syncthreads();
   x.ntab = s_ntab;
   const int lane = threadIdx.x & 63;
@@ -936,6 +944,7 @@ static int build_flat(const GsFlat* f, const GsPhys* p, const GsForcing& fo, GsC
   a.n_classes = f->ntab ? (p->n_classes > 0 ? p->n_classes : 1) : 0;
   a.n_field = (const T*)f->n_field;
   a.max_h = (T*)f->max_h;
+  a.cls = f->cls;
   a.run_k0 = f->run_k0;
   a.run_len = f->run_len;
   a.run_i = f->run_i;
@@ -1030,7 +1039,7 @@ static unsigned sweep_grid_for(int64_t ntiles) {
 // The domain may take the LEAN sweep: nothing it would skip is set.
 template <typename T>
 static bool flat_lean(const FlatK<T>& a) {
-  return !a.n_field && !a.max_h && !a.sponge && !a.clamp && !a.infil && !a.drain;
+  return !a.n_field && !a.cls && !a.max_h && !a.sponge && !a.clamp && !a.infil && !a.drain;
 }
 
 template <typename T>

@@ -85,6 +85,7 @@ struct DenseK {
   int n_classes;
   const S* n_field;
   S* max_h;
+  const uint8_t* cls;   // per-cell class plane (ids 0..255), or NULL: the flags' class bits
   T* rhs[3];
   int which;
   Phys<T> P;
@@ -171,8 +172,9 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T, S> a) {
       eta = (T)c->eta64;
     }
   }
-  __shared__ T s_ntab[32];
-  if (MODE == 1 && threadIdx.x < (unsigned)a.n_classes) s_ntab[threadIdx.x] = (T)a.ntab[threadIdx.x];
+  __shared__ T s_ntab[GS_MAX_CLASSES];
+  if (MODE == 1)
+    for (int k = (int)threadIdx.x; k < a.n_classes; k += (int)blockDim.x) s_ntab[k] = (T)a.ntab[k];
   __syncthreads();
 
   S* const* qin = MODE == 1 ? a.buf[par] : a.buf[a.which];
@@ -280,7 +282,8 @@ __global__ __launch_bounds__(kBlock) void k_dense_sweep(DenseK<T, S> a) {
             else if (a.rain_mode == 2) rr0 = rr0 + (T)frame[(int64_t)i * a.rain_pitch + c];
           }
           T n = T(0);
-          if (P.friction) n = a.n_field ? (T)a.n_field[idx] : s_ntab[fl & GS_FLAG_CLASS_MASK];
+          if (P.friction)
+            n = a.n_field ? (T)a.n_field[idx] : s_ntab[a.cls ? a.cls[idx] : (fl & GS_FLAG_CLASS_MASK)];
           T h, hu, hv;
           update<T>(hC, huC, hvC, rr0, rhu, rhv, dt, n, P, h, hu, hv);
           if (fl & GS_FLAG_STAGE) {
@@ -380,6 +383,7 @@ static int build_dense(const GsDense* d, const GsPhys* p, const GsForcing& f, Gs
   a.ntab = (const S*)d->ntab;
   a.n_classes = d->ntab ? (p->n_classes > 0 ? p->n_classes : 1) : 0;
   a.n_field = (const S*)d->n_field;
+  a.cls = d->cls;
   a.max_h = (S*)d->max_h;
   a.P = make_phys<T>(*p, d->dx, d->dy);
   a.rain_mode = f.n_rain > 0 ? (f.rain_gridded ? 2 : 1) : 0;

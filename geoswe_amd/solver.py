@@ -227,7 +227,8 @@ class Solver2D:
         self._flags_host = None          # (nx, pitch) uint8
         self._flags = None
         self._n_field = None
-        self._ntab = L.DeviceArray((L.MAX_CLASSES,), self.dtype, zero=True)
+        self._cls = None                 # (nx, pitch) uint8 class plane: tables of 33..256 classes
+        self._ntab = L.DeviceArray((L.MAX_CLASSES_PLANE,), self.dtype, zero=True)
         self._n_classes = 1
         self._manning_uniform = float(cfg.manning_n)
         self.inside_mask = None
@@ -273,6 +274,7 @@ class Solver2D:
         d.ntab = self._ntab.ptr
         d.n_field = self._n_field.ptr if self._n_field is not None else None
         d.max_h = self._max_h.ptr if self._max_h is not None else None
+        d.cls = self._cls.ptr if self._cls is not None else None
         return d
 
     # ------------------------------------------------------------------ set-up
@@ -284,7 +286,7 @@ class Solver2D:
         self._flags = L.DeviceArray.from_host(self._flags_host)
 
     def _upload_ntab(self, table):
-        t = np.zeros(L.MAX_CLASSES, self.dtype)
+        t = np.zeros(L.MAX_CLASSES_PLANE, self.dtype)
         t[:len(table)] = np.asarray(table, dtype=np.float64).astype(self.dtype)
         self._ntab.upload(t)
         self._n_classes = max(1, len(table))
@@ -293,6 +295,15 @@ class Solver2D:
         h = np.zeros(self._plane, self.dtype)
         h[:, :self.mesh.ny] = np.asarray(n_int, dtype=np.float64).astype(self.dtype)
         self._n_field = L.DeviceArray.from_host(h)
+        self._cls = None
+
+    def _set_class_plane(self, cls_int, table):
+        """Classes 0..255 in a uint8 plane of their own (more than the flags byte's 32)."""
+        c = np.zeros(self._plane, np.uint8)
+        c[:, :self.mesh.ny] = np.asarray(cls_int, dtype=np.uint8)
+        self._cls = L.DeviceArray.from_host(c)
+        self._n_field = None
+        self._upload_ntab(np.asarray(table)[:L.MAX_CLASSES_PLANE])   # uint8 ids reach 255 at most
 
     def _mark_stage(self, sb):
         g = self.mesh.ngh
@@ -360,8 +371,9 @@ class Solver2D:
             warnings.warn("set_manning_table: friction is off (Config.friction is None), so this roughness "
                           "table will not be applied", stacklevel=2)
         if table.size > L.MAX_CLASSES:
-            # more classes than the flags byte holds: store n = table[cls] per cell instead
-            self._set_n_plane(table[cls_padded[g:g + nx, g:g + ny]].astype(np.float64))
+            # more classes than the flags byte holds: a class plane of their own (up to 256, the
+            # reference's uint8 class array)
+            self._set_class_plane(cls_padded[g:g + nx, g:g + ny], table.astype(np.float64))
             self._manning_table = table
             return
         self._ensure_flags()
@@ -370,6 +382,7 @@ class Solver2D:
         fl |= cls_padded[g:g + nx, g:g + ny].astype(np.uint8)
         self._push_flags()
         self._n_field = None
+        self._cls = None
         self._upload_ntab(table.astype(np.float64))
         self._manning_table = table
 
@@ -393,7 +406,10 @@ class Solver2D:
             fl |= inv.reshape(nx, ny).astype(np.uint8)
             self._push_flags()
             self._n_field = None
+            self._cls = None
             self._upload_ntab(vals.astype(np.float32).astype(np.float64))
+        elif vals.size <= L.MAX_CLASSES_PLANE and self.dtype == np.float32:
+            self._set_class_plane(inv.reshape(nx, ny), vals.astype(np.float32).astype(np.float64))
         else:
             self._set_n_plane(nh)
 

@@ -52,6 +52,10 @@ extern "C" {
  * counterpart (the reference computes a float32 state in float32). */
 #define GS_F32X64 2
 
+/* Manning classes: the flags byte holds ids 0..31; a uint8 class plane (the reference's
+ * per-cell class array, solver.py:1524-1556) holds 0..255 and then takes precedence */
+#define GS_MAX_CLASSES 256
+
 /* per-cell flags byte (dense "flags" plane, flat "flags" array) */
 #define GS_FLAG_CLASS_MASK 0x1F  /* Manning class id (0..31) into the ntab array */
 #define GS_FLAG_CFL_SKIP 0x20    /* left out of the CFL wave-speed maximum (set_cfl_ghost_mask) */
@@ -107,7 +111,8 @@ typedef struct GsPhys {
   int32_t quadratic;   /* 1 quadratic-alpha friction root, 0 linearised */
   int32_t keep_h;      /* 1 keep the sub-threshold depth on dry cells, 0 zero it */
   int32_t cfl_l2;      /* CFL velocity norm: 0 max(|u|,|v|), 1 sqrt(u^2+v^2) */
-  int32_t n_classes;   /* entries of the Manning table (1..32) */
+  int32_t n_classes;   /* entries of the Manning table: 1..32 read through the flags byte,
+                          up to GS_MAX_CLASSES with a class plane (GsDense.cls, GsFlat.cls) */
   int32_t pad_;
 } GsPhys;
 
@@ -145,6 +150,8 @@ typedef struct GsDense {
   const void* ntab;          /* Manning n per class, device array of n_classes (state dtype) */
   const void* n_field;       /* nullable per-cell Manning n plane (overrides ntab) */
   void* max_h;               /* nullable running-max depth plane */
+  const uint8_t* cls;        /* nullable per-cell Manning class plane (ids 0..255 into ntab, in
+                                place of the flags' class bits) */
 } GsDense;
 
 /*
@@ -244,6 +251,8 @@ typedef struct GsFlat {
   const void* ring_w;        /* [rows][ring_ng] state dtype */
   const double* ring_t;      /* [ring_nt] ascending */
   const void* ring_stage;    /* [ring_ng][ring_nt] state dtype */
+  const uint8_t* cls;        /* nullable [N] Manning class ids 0..255 into ntab, in place of the
+                                flags' class bits (also for the per-class infiltration rate) */
 } GsFlat;
 
 /* Host arrays of a flat mesh built by gs_flat_mesh_build (allocated by the library with
