@@ -476,7 +476,7 @@ int gs_ens_autotune(GsEns* e, int trial_steps, void* stream, double* best_ms);
  * every member, bitwise what gs_ens_create + gs_ens_run(nsteps, 0) + gs_ens_destroy give,
  * without any device allocation: the member table, the tile prefix and the last-workgroup
  * counter live in a static 256 KiB device array of the library (each GPU holds its own
- * until the process ends), so at most 414 float32 or 389 float64 members (GS_E_RANGE
+ * until the process ends), so at most 409 float32 or 384 float64 members (GS_E_RANGE
  * beyond). Members use their own tile tables and one occupancy wave of workgroups. The call
  * waits for earlier work on `stream` (an earlier call's steps may still read the array),
  * copies the table synchronously, then enqueues the steps; calls that overlap on one GPU
