@@ -387,7 +387,7 @@ def ensemble_leg(members: int, steps: int, graph_batch: int) -> dict:
     ev_plain = ms.value
     log(f"sequential plain runs: {w_plain:.2f} s wall for {steps} steps x {len(plain)} members")
     ens = Ensemble(mem, graph_batch=graph_batch)
-    tune = ens.autotune(trial_steps=3)
+    tune = ens.autotune(trial_steps=3, heights=(16, 32, 64))
     log(f"autotuned launch: {tune}")
     ens.advance(1, graph_batch=0)       # first launch outside the timed region
     L.sync()

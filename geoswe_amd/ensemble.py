@@ -24,7 +24,7 @@ class Ensemble:
     """Members are ``CompressedSolver`` objects; they stay usable (``depth()``, ``t``,
     ``nsteps``) and must outlive the ensemble, which steps their buffers in place."""
 
-    def __init__(self, members: Sequence[CompressedSolver], graph_batch: int = 50):
+    def __init__(self, members: Sequence[CompressedSolver], graph_batch: int = 50, sweep_rows: Optional[int] = None):
         members = list(members)
         if not members:
             raise ValueError("Ensemble needs at least one member")
@@ -48,25 +48,45 @@ class Ensemble:
             m._ensure_lam()
             m._open_mode()
         n = len(members)
-        self._descs = (L.GsFlat * n)(*[m._desc() for m in members])
-        # Tile height for the whole ensemble: members run side by side, so a small member
-        # needs no short tiles to fill the GPU (its own tables are sized for it alone).
-        self.sweep_rows = sweep_rows_for(sum(int(m.fm.tile_irc[:, 1].sum()) for m in members))
-        self._tiles = []
-        for k, m in enumerate(members):
-            if int(m.fm.tile_irc[:, 1].max(initial=0)) >= self.sweep_rows or m.n_active == 0:
-                continue                       # already at least as tall: keep the member's tiles
-            e0, irc, ent = mesh_sweep_tiles(m.fm, self.sweep_rows)
-            arrs = [L.DeviceArray.from_host(a) for a in (e0, irc.reshape(-1), ent.reshape(-1))]
-            self._tiles.append(arrs)
-            d = self._descs[k]
-            d.n_tiles = len(e0)
-            d.tile_e0, d.tile_irc, d.ent = (a.ptr for a in arrs)
         self._phys = (L.GsPhys * n)(*[m._phys() for m in members])
         self._forc = (L.GsForcing * n)(*[m._fo.s for m in members])
         self._clocks = (C.c_void_p * n)(*[m._clock.ptr for m in members])
+        self._h = C.c_void_p()
+        self._tile_cache = {}
+        # Tile height for the whole ensemble (or the caller's): members run side by side, so a
+        # small member needs no short tiles to fill the GPU (its own tables are sized for it
+        # alone); autotune(heights=...) may pick another.
+        if sweep_rows is not None and not (1 <= int(sweep_rows) <= 1024):
+            raise ValueError(f"sweep_rows must be in 1..1024, got {sweep_rows}")
+        self._build(int(sweep_rows) if sweep_rows is not None
+                    else sweep_rows_for(sum(int(m.fm.tile_irc[:, 1].sum()) for m in members)),
+                    exact=sweep_rows is not None)
+
+    def _build(self, rows: int, exact: bool = True) -> None:
+        """The native ensemble over the members' tiles of `rows` rows (`exact`: every member
+        re-tiled to that height; else members already at least as tall keep their tiles)."""
+        self.close()
+        members = self.members
+        self.sweep_rows = int(rows)
+        descs = (L.GsFlat * len(members))(*[m._desc() for m in members])
+        tiles = []
+        for k, m in enumerate(members):
+            tall = int(m.fm.tile_irc[:, 1].max(initial=0))
+            if m.n_active == 0 or (tall == rows if exact else tall >= rows):
+                continue                       # keep the member's own tiles
+            key = (k, rows)
+            if key not in self._tile_cache:
+                e0, irc, ent = mesh_sweep_tiles(m.fm, rows)
+                self._tile_cache[key] = (len(e0), [L.DeviceArray.from_host(a) for a in (e0, irc.reshape(-1),
+                                                                                        ent.reshape(-1))])
+            nt, arrs = self._tile_cache[key]
+            tiles.append(arrs)
+            d = descs[k]
+            d.n_tiles = nt
+            d.tile_e0, d.tile_irc, d.ent = (a.ptr for a in arrs)
+        self._descs, self._tiles = descs, tiles
         h = C.c_void_p()
-        L.call("gs_ens_create", self._descs, self._phys, self._forc, self._clocks, n, C.byref(h))
+        L.call("gs_ens_create", self._descs, self._phys, self._forc, self._clocks, len(members), C.byref(h))
         self._h = h
 
     # ------------------------------------------------------------------ properties
@@ -138,13 +158,32 @@ class Ensemble:
             m._target = None
             m._open_mode()
 
-    def autotune(self, trial_steps: int = 5) -> dict:
+    def autotune(self, trial_steps: int = 5, heights: Optional[Sequence[int]] = None) -> dict:
         """Time the candidate launch shapes on a snapshot of the state (restored
-        afterwards) and keep the fastest."""
-        best = C.c_double()
-        L.call("gs_ens_autotune", self._h, int(trial_steps), None, C.byref(best))
+        afterwards) and keep the fastest. With `heights`, also the tile height: every member
+        re-tiled to each height in turn, the launch shapes timed for each, and the fastest
+        height and shape kept (the results are the same bits at every height and shape)."""
+        def native():
+            best = C.c_double()
+            L.call("gs_ens_autotune", self._h, int(trial_steps), None, C.byref(best))
+            return best.value if math.isfinite(best.value) else math.inf
+        if not heights:
+            ms = native()
+            out = self.launch
+            out["ms_per_step"] = ms if math.isfinite(ms) else None
+            return out
+        trials = {}
+        for rows in heights:
+            self._build(int(rows))
+            trials[int(rows)] = (native(), self.launch)
+        rows = min(trials, key=lambda r: trials[r][0])
+        self._build(rows)
+        ms, launch = trials[rows]
+        self.set_launch(launch["blocks"], launch["tiles_per_pass"])
         out = self.launch
-        out["ms_per_step"] = best.value if math.isfinite(best.value) else None
+        out["ms_per_step"] = ms if math.isfinite(ms) else None
+        out["sweep_rows"] = rows
+        out["trials"] = {r: round(v[0], 4) for r, v in trials.items()}
         return out
 
     def close(self) -> None:

@@ -74,6 +74,25 @@ def test_ensemble_launch_shapes_and_autotune(cases):
     ens.close()
 
 
+def test_ensemble_tile_height_autotune(cases):
+    """The autotuner re-tiles every member at each candidate height and keeps the fastest;
+    the state it timed on is restored and the members stay bitwise their plain runs."""
+    from geoswe_amd.ensemble import Ensemble
+    plain = flats(cases)
+    for p in plain:
+        p.advance(25)
+    mem = flats(cases)
+    ens = Ensemble(mem, graph_batch=8)
+    got = ens.autotune(trial_steps=2, heights=(8, 16, 32))
+    assert got["sweep_rows"] in (8, 16, 32) and ens.sweep_rows == got["sweep_rows"]
+    assert set(got["trials"]) == {8, 16, 32}
+    assert all(m.nsteps == 0 for m in mem)
+    ens.advance(15)
+    ens.advance(10, graph_batch=0)
+    assert_same(plain, mem)
+    ens.close()
+
+
 def test_ensemble_run_to_t_end(cases):
     from geoswe_amd.ensemble import Ensemble
     plain = flats(cases)
