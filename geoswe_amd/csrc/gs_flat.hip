@@ -592,9 +592,11 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
   T gzxC = (T(0.5) * (bp1 - bm1)) * ix;
   Face<T> fx;
   face<T>(bm1, b0, gzxP, gzxC, hP, hC, uP, vP, uC, vC, hdx, g, hm, fx);
-  // the x-face below the current row, as the products the current cell adds
-  T xFh = fx.Fh * ix, xFN = fx.FN * ix, xFT = fx.FT * ix, xsR = fx.srcR * ix;
-  bool xg = fx.gate;
+  // the x-face below the current row, as the products the current cell adds (zero when
+  // the face is gated off: see the accumulation below)
+  const bool xg = fx.gate;
+  T xFh = xg ? fx.Fh * ix : T(0), xFN = xg ? fx.FN * ix : T(0), xFT = xg ? fx.FT * ix : T(0),
+    xsR = xg ? fx.srcR * ix : T(0);
 
   for (int r = 0; r < nr; ++r) {
     const GS_G uint64_t* Er = E + 3 * (r + 2);   // this row's entry
@@ -608,7 +610,9 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
     const T gzxN = (T(0.5) * (bp2 - b0)) * ix;
     Face<T> fn;
     face<T>(b0, bp1, gzxC, gzxN, hC, hN, uC, vC, uN, vN, hdx, g, hm, fn);
-    const T nFh = fn.Fh * ix, nFN = fn.FN * ix, nFT = fn.FT * ix;
+    const bool ng = fn.gate;
+    const T nFh = ng ? fn.Fh * ix : T(0), nFN = ng ? fn.FN * ix : T(0), nFT = ng ? fn.FT * ix : T(0);
+    const T nsL = ng ? fn.srcL * ix : T(0), nsR = ng ? fn.srcR * ix : T(0);
 
     // y faces of the row: each lane computes its upper face (j, j+1)
     const T bl = lane_lower(b0), br = lane_upper(b0);
@@ -617,36 +621,33 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
     const T gzyR = lane_upper(gzy);
     Face<T> fy;
     face<T>(b0, br, gzy, gzyR, hC, hR, vC, uC, vR, uR, hdy, g, hm, fy);
-    const T yFh = fy.Fh * iy, yFN = fy.FN * iy, yFT = fy.FT * iy;
+    const bool yg = fy.gate;
+    const T yFh = yg ? fy.Fh * iy : T(0), yFN = yg ? fy.FN * iy : T(0), yFT = yg ? fy.FT * iy : T(0);
+    const T ysL = yg ? fy.srcL * iy : T(0);
     const T yFhB = lane_lower(yFh), yFNB = lane_lower(yFN), yFTB = lane_lower(yFT);
-    const T ysRB = lane_lower(fy.srcR * iy);
-    const int ygB = lane_lower((int)fy.gate);
+    const T ysRB = lane_lower(yg ? fy.srcR * iy : T(0));
 
+    // The four faces in the reference's order. A face gated off (both cells dry) adds
+    // zeros: the sums start at +0 and IEEE addition yields -0 only from two -0 terms, so
+    // they never hold -0, and x + 0 = x - 0 = x for every other x; the gate costs one
+    // select per face term instead of one per use.
     T rh = T(0), rhu = T(0), rhv = T(0);
-    if (fn.gate) {
-      rh = rh - nFh;
-      rhu = rhu - nFN;
-      rhv = rhv - nFT;
-      rhu = rhu - fn.srcL * ix;
-    }
-    if (xg) {
-      rh = rh + xFh;
-      rhu = rhu + xFN;
-      rhv = rhv + xFT;
-      rhu = rhu + xsR;
-    }
-    if (fy.gate) {
-      rh = rh - yFh;
-      rhu = rhu - yFT;
-      rhv = rhv - yFN;
-      rhv = rhv - fy.srcL * iy;
-    }
-    if (ygB) {
-      rh = rh + yFhB;
-      rhu = rhu + yFTB;
-      rhv = rhv + yFNB;
-      rhv = rhv + ysRB;
-    }
+    rh = rh - nFh;
+    rhu = rhu - nFN;
+    rhv = rhv - nFT;
+    rhu = rhu - nsL;
+    rh = rh + xFh;
+    rhu = rhu + xFN;
+    rhv = rhv + xFT;
+    rhu = rhu + xsR;
+    rh = rh - yFh;
+    rhu = rhu - yFT;
+    rhv = rhv - yFN;
+    rhv = rhv - ysL;
+    rh = rh + yFhB;
+    rhu = rhu + yFTB;
+    rhv = rhv + yFNB;
+    rhv = rhv + ysRB;
 
     if ((Er[2] >> lane) & 1ull) {   // an active cell: lanes 2..61 only
       const uint32_t ob = oC / (uint32_t)sizeof(T);   // byte offset in a 1-byte plane
@@ -692,7 +693,7 @@ __device__ __forceinline__ void flat_tile(const FlatCtx<T>& x, int64_t t, int la
     // slide the window one row down
     hC = hN; huC = huN; hvC = hvN; uC = uN; vC = vN; gzxC = gzxN;
     bm1 = b0; b0 = bp1; bp1 = bp2;
-    xFh = nFh; xFN = nFN; xFT = nFT; xsR = fn.srcR * ix; xg = fn.gate;
+    xFh = nFh; xFN = nFN; xFT = nFT; xsR = nsR;
     gC = gN; oC = oN; gN = gNN; oN = oNN; sN = sNN;
   }
 }

@@ -169,61 +169,87 @@ int gs_flat_mesh_build(const uint8_t* inside, int nxp, int nyp, int ring, int sw
     }
   }
 
-  // row-sweep tiles: window w covers columns 60w .. 60w+63
+  // tile height from the column grid 60w (windows with an active lane, over all rows)
   const int64_t nwin = (NY - 4) > 0 ? (NY - 4 + kChunk - 1) / kChunk : 1;
   const int64_t nwin1 = nwin < 1 ? 1 : nwin;
-  std::vector<uint64_t> smask((size_t)(NX * nwin1)), amask((size_t)(NX * nwin1));
-  std::vector<int64_t> base((size_t)(NX * nwin1));
   int64_t busy = 0;
-  for (int64_t i = 0; i < NX; ++i) {
-    int64_t before = row_start[i];
+  for (int64_t i = 0; i < NX; ++i)
     for (int64_t w = 0; w < nwin1; ++w) {
-      uint64_t sm = 0, am = 0;
+      uint64_t am = 0;
       for (int l = 0; l < 64; ++l) {
         const int64_t j = kChunk * w + l;
         if (j >= NY) break;
-        if (st[at(i, j)]) sm |= 1ull << l;
         if (inside[at(i, j)]) am |= 1ull << l;
       }
-      am &= kWriteLanes;
-      smask[i * nwin1 + w] = sm;
-      amask[i * nwin1 + w] = am;
-      base[i * nwin1 + w] = before;
-      for (int64_t j = kChunk * w; j < kChunk * (w + 1) && j < NY; ++j) before += st[at(i, j)];
-      busy += am != 0;
+      busy += (am & kWriteLanes) != 0;
     }
-  }
   int64_t rows = sweep_rows;
   if (rows == -1) {
     rows = 1;
     while (rows < max_sweep_rows && busy / (2 * rows) >= tile_target) rows *= 2;
   }
+  // row-sweep tiles: in each band of `rows` rows, 64-lane windows placed from the left, each
+  // starting two columns before the first active column the previous windows left uncovered
+  // (lanes 2..61 are written), trimmed to the band's rows with an active lane
   const int64_t nb = (NX + rows - 1) / rows;
   std::vector<int64_t> tile_e0;
   std::vector<int32_t> tile_irc;
   std::vector<uint64_t> ent;
+  std::vector<uint8_t> colany((size_t)NY);
+  std::vector<int64_t> before;   // stored cells left of each column, rows r0-2 .. r1+1
   int64_t ne = 0;
-  for (int64_t b = 0; b < nb; ++b)
-    for (int64_t w = 0; w < nwin1; ++w) {
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t r0 = b * rows, r1 = (b + 1) * rows < NX ? (b + 1) * rows : NX;
+    bool any = false;
+    for (int64_t j = 0; j < NY; ++j) {
+      uint8_t v = 0;
+      for (int64_t i = r0; i < r1 && !v; ++i) v = inside[at(i, j)] != 0;
+      colany[j] = v;
+      any = any || v;
+    }
+    if (!any) continue;
+    const int64_t lo = r0 - 2 > 0 ? r0 - 2 : 0, hi = r1 + 2 < NX ? r1 + 2 : NX;
+    before.assign((size_t)((hi - lo) * (NY + 1)), 0);
+    for (int64_t i = lo; i < hi; ++i) {
+      int64_t* p = before.data() + (i - lo) * (NY + 1);
+      for (int64_t j = 0; j < NY; ++j) p[j + 1] = p[j] + st[at(i, j)];
+    }
+    int64_t j = 0;
+    while (true) {
+      while (j < NY && !colany[j]) ++j;
+      if (j >= NY) break;
+      const int64_t c0 = j - 2;
+      if (c0 < 0) return fail(GS_E_STATE, "a sweep window would start left of the padded grid");
       int64_t first = -1, last = -1;
-      for (int64_t i = b * rows; i < (b + 1) * rows && i < NX; ++i)
-        if (amask[i * nwin1 + w]) {
+      for (int64_t i = r0; i < r1; ++i) {
+        bool live = false;
+        for (int64_t jj = c0 + 2; jj < c0 + 62 && jj < NY && !live; ++jj) live = inside[at(i, jj)] != 0;
+        if (live) {
           if (first < 0) first = i;
           last = i;
         }
-      if (first < 0) continue;
+      }
       if (first < 2 || last + 2 > NX - 1) return fail(GS_E_STATE, "a sweep tile's stencil rows leave the padded grid");
       tile_e0.push_back(ne);
       tile_irc.push_back((int32_t)first);
       tile_irc.push_back((int32_t)(last - first + 1));
-      tile_irc.push_back((int32_t)(w * kChunk));
+      tile_irc.push_back((int32_t)c0);
       for (int64_t i = first - 2; i <= last + 2; ++i) {
-        ent.push_back((uint64_t)base[i * nwin1 + w]);
-        ent.push_back(smask[i * nwin1 + w]);
-        ent.push_back(amask[i * nwin1 + w]);
+        uint64_t sm = 0, am = 0;
+        for (int l = 0; l < 64; ++l) {
+          const int64_t jj = c0 + l;
+          if (jj >= NY) break;
+          if (st[at(i, jj)]) sm |= 1ull << l;
+          if (inside[at(i, jj)]) am |= 1ull << l;
+        }
+        ent.push_back((uint64_t)(row_start[i] + before[(size_t)((i - lo) * (NY + 1) + c0)]));
+        ent.push_back(sm);
+        ent.push_back(am & kWriteLanes);
         ++ne;
       }
+      j = c0 + 2 + kChunk;
     }
+  }
 
   out->n_stored = n;
   out->n_active = n_active;

@@ -209,66 +209,81 @@ def mesh_sweep_tiles(fm: "FlatMesh", rows: int):
 def build_sweep_tiles(stored: np.ndarray, active: np.ndarray, rows: int = None):
     """Row-sweep tiles of the flat step.
 
-    Window k covers the padded columns CHUNK*k .. CHUNK*k+63 (64 lanes, lanes 2..61 are
-    written, so every active column 2 .. nyp-3 belongs to exactly one window). For every
-    padded row and window one entry holds the flat id of the row's first stored cell at
-    column >= CHUNK*k and the stored- and active-lane masks: the stored cells of a row are
-    consecutive flat ids, so lane l's id is base + popcount(stored & lanes below l). A tile
-    is a window over at most `rows` consecutive rows (bands of `rows` rows, trimmed to the
-    rows with an active lane); its entries are rows i0-2 .. i0+nr+1 (the stencil's two rows
-    each side), stored contiguously. Tiles are ordered band-major, so the tiles that run
-    side by side read neighbouring columns of the same rows.
+    The padded rows are cut into bands of `rows` rows. In each band, windows of 64 lanes are
+    placed from the left: a window starts two columns before the band's first active column
+    not yet covered (lanes 2..61 are written, so it covers that column and the next 59), and
+    the next window starts from the first active column after those 60. Every active column
+    of the band thus belongs to exactly one window, with as few windows as 60-column windows
+    allow (placing each at the first uncovered column is optimal for covering points with
+    intervals of one length). For every row of a window one entry holds the flat id of the
+    row's first stored cell at or after the window's column c0 and the stored- and
+    active-lane masks: the stored cells of a row are consecutive flat ids, so lane l's id is
+    base + popcount(stored & lanes below l). A tile is a window trimmed to the band's rows
+    with an active lane; its entries are rows i0-2 .. i0+nr+1 (the stencil's two rows each
+    side), stored contiguously. Tiles are ordered band-major, so the tiles that run side by
+    side read neighbouring columns of the same rows.
 
     `rows=None` picks the tile height from the domain: the largest power of two <= 64 that
-    still gives about TILE_TARGET tiles, so that a small domain run alone keeps every CU
-    busy (a tile of r rows evaluates 1 + 1/r x-faces per row instead of 2).
+    still gives about TILE_TARGET tiles of the column grid CHUNK*k, so that a small domain run
+    alone keeps every CU busy (a tile of r rows evaluates 1 + 1/r x-faces per row instead
+    of 2).
     """
     stored = np.asarray(stored, dtype=bool)
     active = np.asarray(active, dtype=bool)
     nxp, nyp = stored.shape
     if rows is not None and rows < 1:
         raise ValueError("rows must be >= 1")
-    nwin = max(1, -(-(nyp - 4) // CHUNK))
-    width = CHUNK * nwin + 4
+    if rows is None:
+        nwin = max(1, -(-(nyp - 4) // CHUNK))
+        A = np.zeros((nxp, CHUNK * nwin + 4), bool)
+        A[:, :nyp] = active
+        rows = sweep_rows_for(np.count_nonzero(_window_masks(A, nwin) & WRITE_LANES))   # grid windows
+        del A
+    if active[:, :2].any():
+        raise AssertionError("a sweep window would start left of the padded grid")
+    width = nyp + 64                      # room for a window at any active column
     S = np.zeros((nxp, width), bool)
     S[:, :nyp] = stored
-    smask = _window_masks(S, nwin)
-    A = np.zeros((nxp, width), bool)
-    A[:, :nyp] = active
-    amask = _window_masks(A, nwin) & WRITE_LANES
-    del A
-    if rows is None:
-        rows = sweep_rows_for(np.count_nonzero(amask))        # window rows with an active lane
-    per_block = S[:, :CHUNK * nwin].reshape(nxp, nwin, CHUNK).sum(axis=2, dtype=np.int64)
-    del S
+    Aw = np.zeros((nxp, width), bool)
+    Aw[:, :nyp] = active
     row_start = np.zeros(nxp, np.int64)
     row_start[1:] = np.cumsum(stored.sum(axis=1, dtype=np.int64))[:-1]
-    base = np.zeros((nxp, nwin), np.int64)
-    base[:, 1:] = np.cumsum(per_block, axis=1)[:, :-1]
-    base += row_start[:, None]
-    # tiles: band b x window k, trimmed to its rows with an active lane
-    nb = -(-nxp // rows)
-    live = np.zeros((nb * rows, nwin), bool)
-    live[:nxp] = amask != 0
-    live = live.reshape(nb, rows, nwin)
-    anyl = live.any(axis=1)                                   # (nb, nwin)
-    first = np.argmax(live, axis=1)
-    last = rows - 1 - np.argmax(live[:, ::-1, :], axis=1)
-    bb, kk = np.nonzero(anyl)                                 # band-major order
-    i0 = (bb * rows + first[bb, kk]).astype(np.int64)
-    nr = (last[bb, kk] - first[bb, kk] + 1).astype(np.int64)
-    if i0.size and (i0.min() < 2 or (i0 + nr + 1).max() > nxp - 1):
-        raise AssertionError("a sweep tile's stencil rows leave the padded grid")
-    cnt = nr + 4
-    tile_e0 = np.zeros(i0.size, np.int64)
-    if i0.size:
+    lane = np.arange(64)
+    i0s, nrs, c0s, ents = [], [], [], []
+    for r0 in range(0, nxp, rows):
+        r1 = min(r0 + rows, nxp)
+        cols = np.flatnonzero(active[r0:r1].any(axis=0))
+        if cols.size == 0:
+            continue
+        lo, hi = max(r0 - 2, 0), min(r1 + 2, nxp)
+        before = np.zeros((hi - lo, width + 1), np.int64)      # stored cells left of a column
+        np.cumsum(S[lo:hi], axis=1, out=before[:, 1:])
+        k = 0
+        while k < cols.size:
+            c0 = int(cols[k]) - 2
+            k = int(np.searchsorted(cols, c0 + 2 + CHUNK))
+            live = Aw[r0:r1, c0 + 2:c0 + 62].any(axis=1)
+            first = int(np.argmax(live))
+            last = r1 - r0 - 1 - int(np.argmax(live[::-1]))
+            i0, nr = r0 + first, last - first + 1
+            if i0 < 2 or i0 + nr + 1 > nxp - 1:
+                raise AssertionError("a sweep tile's stencil rows leave the padded grid")
+            rr = np.arange(i0 - 2, i0 + nr + 2)
+            e = np.empty((rr.size, 3), np.uint64)
+            e[:, 0] = (row_start[rr] + before[rr - lo, c0]).astype(np.uint64)
+            cols64 = c0 + lane
+            e[:, 1] = np.packbits(S[rr[:, None], cols64], axis=1, bitorder="little").view("<u8")[:, 0]
+            e[:, 2] = np.packbits(Aw[rr[:, None], cols64], axis=1, bitorder="little").view("<u8")[:, 0] & WRITE_LANES
+            i0s.append(i0)
+            nrs.append(nr)
+            c0s.append(c0)
+            ents.append(e)
+    nt = len(i0s)
+    cnt = np.asarray(nrs, np.int64) + 4
+    tile_e0 = np.zeros(nt, np.int64)
+    if nt:
         tile_e0[1:] = np.cumsum(cnt)[:-1]
-    ne = int(cnt.sum())
-    erow = np.repeat(i0 - 2 - tile_e0, cnt) + np.arange(ne, dtype=np.int64)
-    ewin = np.repeat(kk.astype(np.int64), cnt)
-    ent = np.empty((ne, 3), np.uint64)
-    ent[:, 0] = base[erow, ewin].astype(np.uint64)
-    ent[:, 1] = smask[erow, ewin]
-    ent[:, 2] = amask[erow, ewin]
-    tile_irc = np.stack([i0, nr, kk.astype(np.int64) * CHUNK], axis=1).astype(np.int32)
-    return tile_e0, np.ascontiguousarray(tile_irc), ent
+    ent = np.concatenate(ents) if ents else np.zeros((0, 3), np.uint64)
+    tile_irc = np.stack([np.asarray(i0s, np.int64), np.asarray(nrs, np.int64), np.asarray(c0s, np.int64)],
+                        axis=1).astype(np.int32).reshape(nt, 3)
+    return tile_e0, np.ascontiguousarray(tile_irc), np.ascontiguousarray(ent)
