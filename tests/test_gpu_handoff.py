@@ -48,7 +48,7 @@ def test_dense_planned_dt_many_workgroups(gpu):
 
 
 def test_flat_planned_dt_many_workgroups(gpu):
-    # 3000 x 3000 fully active: about 2350 row-sweep tiles of 64 rows, one wavefront each
+    # 3000 x 3000 fully active: about 9 400 row-sweep tiles of 16 rows, one wavefront each
     from geoswe_amd import CompressedSolver
     nx, ny = 3000, 3000
     rng = np.random.default_rng(12)

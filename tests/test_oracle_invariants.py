@@ -357,12 +357,12 @@ def test_sweep_tiles_cover_every_active_cell_once(rows):
 
 
 def test_sweep_rows_for_domain_size():
-    from geoswe_amd.flatmesh import TILE_TARGET, sweep_rows_for
+    from geoswe_amd.flatmesh import MAX_SWEEP_ROWS, TILE_TARGET, sweep_rows_for
     assert sweep_rows_for(0) == 1 and sweep_rows_for(2 * TILE_TARGET - 1) == 1
-    assert sweep_rows_for(2 * TILE_TARGET) == 2 and sweep_rows_for(10 ** 9) == 64
+    assert sweep_rows_for(2 * TILE_TARGET) == 2 and sweep_rows_for(10 ** 9) == MAX_SWEEP_ROWS == 32
     for busy in (1, 5000, 40000, 300000, 2000000):
         r = sweep_rows_for(busy)
-        assert r == 64 or busy // r >= TILE_TARGET > busy // (2 * r) or r == 1
+        assert r == MAX_SWEEP_ROWS or busy // r >= TILE_TARGET > busy // (2 * r) or r == 1
 
 
 def test_sweep_tiles_random_masks():
