@@ -309,7 +309,8 @@ def coastal_leg(n: int, steps: int, sweep_rows=None, forcings: bool = False) -> 
         "data": "synthetic (seeded fBm DEM, Voronoi Manning classes, fBm rain frames, M2 tide)",
         "config": {"workload": f"coastal {n}x{n}, {nact} active cells ({nact / n / n:.1%}), 8 Manning classes, "
                                f"gridded rain every 300 s, tidal stage on the sea edge, fall edges, CFL 0.5 (L-inf)",
-                   "layout": "flat", "active_cells": int(nact), "stored_cells": int(fm.n_stored)},
+                   "layout": "flat", "active_cells": int(nact), "stored_cells": int(fm.n_stored),
+                   "sweep_tiles": int(len(fm.tile_e0)), "sweep_rows": int(fm.tile_irc[:, 1].max(initial=0))},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": "gs::k_flat_sweep<float, false, true>",
                      "kernel_ms": ms_f, "bytes_per_cell": bpc},
