@@ -181,10 +181,11 @@ typedef struct GsFlat {
   const int32_t* run_d;      /* [n_runs][4] id offsets of rows i+1, i-1, i+2, i-2 */
   const int32_t* chunk_run;  /* [ceil(N/60)] first run whose end is past the chunk start */
   /* Row-sweep tiles (the step's traversal; n_tiles = 0 steps by chunks of the run table).
-   * Window w covers padded columns 60w .. 60w+63 (lanes 2..61 written); entry = (flat id
-   * of the row's first stored cell at column >= 60w, stored-lane mask, active-lane mask)
-   * for one padded row of one window. Tile t sweeps rows i0 .. i0+nr-1 of one window;
-   * its entries are rows i0-2 .. i0+nr+1, from ent[3 * tile_e0[t]] on. */
+   * A tile's window covers padded columns c0 .. c0+63 (lanes 2..61 written; any c0 >= 0, as
+   * long as every active cell is written by exactly one tile); entry = (flat id of the
+   * row's first stored cell at column >= c0, stored-lane mask, active-lane mask) for one
+   * padded row of the window. Tile t sweeps rows i0 .. i0+nr-1 of its window; its entries
+   * are rows i0-2 .. i0+nr+1, from ent[3 * tile_e0[t]] on. */
   int64_t n_tiles;
   const int64_t* tile_e0;    /* [n_tiles] entry index of row i0 - 2 */
   const int32_t* tile_irc;   /* [n_tiles][3] i0, nr, padded column of lane 0 */
