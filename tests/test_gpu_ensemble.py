@@ -140,7 +140,7 @@ def test_ensemble_with_taller_tiles_than_members(cases, monkeypatch, target):
     ens = Ensemble(mem, graph_batch=8)
     assert ens.sweep_rows == EN.sweep_rows_for(sum(int(t[:, 1].sum()) for t in own))
     if target == 1:
-        assert ens.sweep_rows == 64 and len(ens._tiles) == len(mem)   # every member rebuilt
+        assert ens.sweep_rows == FM.MAX_SWEEP_ROWS and len(ens._tiles) == len(mem)   # every member rebuilt
     assert all(np.array_equal(o, m.fm.tile_irc) for o, m in zip(own, mem))
     ens.advance(25)
     assert_same(plain, mem)
