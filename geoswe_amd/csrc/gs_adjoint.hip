@@ -181,20 +181,22 @@ __device__ __forceinline__ void face_slot(const AdjK<T>& a, int axis, int64_t L,
   s[9 * n] = G.gzR;
 }
 
-// The faces an active cell owns: its upper face along each axis, and its lower face when
-// the cell below is not active (an active cell below owns that face). One face per trip of
-// a loop that is not unrolled, so the reverse of a face is compiled once.
+// Every face with an active cell on either side, computed by its lower cell (the slots are
+// indexed by it): each stored cell, active or ring, takes its upper face along each axis
+// when it or the cell above is active. A wavefront then runs at most two face reverses (a
+// scheme in which an active cell also took its lower face beside a ring cell ran up to
+// four, the extra two for the few lanes on the active set's lower edges). One face per
+// trip of a loop that is not unrolled, so the reverse of a face is compiled once.
 template <typename T>
 __global__ __launch_bounds__(kABlock) void k_adj_faces(AdjK<T> a) {
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < a.n; c += (int64_t)gridDim.x * blockDim.x) {
-    if (!a.active[c]) continue;
     const int32_t* nb = a.nbr + 4 * c;
+    const bool act = a.active[c] != 0;
 #pragma unroll 1
-    for (int f = 0; f < 4; ++f) {   // f: upper x, lower x, upper y, lower y
-      const int64_t o = nb[f];
-      if (o < 0 || ((f & 1) && a.active[o])) continue;
-      const int axis = f >> 1;
-      face_slot<T>(a, axis, (f & 1) ? o : c, (f & 1) ? c : o, axis == 0 ? a.xf : a.yf);
+    for (int axis = 0; axis < 2; ++axis) {   // nb[0]: upper x, nb[2]: upper y
+      const int64_t u = nb[2 * axis];
+      if (u < 0 || !(act || a.active[u])) continue;
+      face_slot<T>(a, axis, c, u, axis == 0 ? a.xf : a.yf);
     }
   }
 }
