@@ -383,6 +383,28 @@ def test_sweep_tiles_random_masks():
             inside[4, 4] = True
         fm = build_flat_mesh(inside, 4, sweep_rows=rows)
         assert np.array_equal(_tile_walk(fm, inside), inside.astype(np.int64))
+        # windows placed at the first uncovered active column of each band: never more
+        # tiles than windows on the fixed column grid 60k, and per band the fewest that
+        # 60-column windows allow (a window writes 60 consecutive columns, so active
+        # columns pairwise 60 or more apart need one window each: a set of that many
+        # proves the count minimal)
+        nxp, nyp = inside.shape
+        grid = best = 0
+        for r0 in range(0, nxp, rows):
+            cols = np.flatnonzero(inside[r0:r0 + rows].any(axis=0))
+            grid += len({(c - 2) // 60 for c in cols})
+            k = 0
+            while k < cols.size:                   # greedy cover, restated
+                best += 1
+                k = int(np.searchsorted(cols, cols[k] + 60))
+            need = 0                              # columns pairwise 60 or more apart
+            last = -10 ** 9
+            for c in cols:
+                if c - last >= 60:
+                    need += 1
+                    last = c
+            assert best == need
+        assert len(fm.tile_e0) == best <= grid
 
     check()
 
